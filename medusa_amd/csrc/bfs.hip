@@ -1,0 +1,331 @@
+// BFS superstep engine.
+//
+// Reference: every superstep FunctorHolder<EDGE>::Run(UpdateLevel) sweeps all |E|
+// edges (MultipleGPU/MultiAPIHoster.h:657-678, MedusaRT/APIKernelWrappers.h:34-44):
+// if level[src]==step and level[dst]==-1 then level[dst]=step+1 and vote to
+// continue (Algorithm/UserOperators.h:17-36, MedusaRT/SystemLibGPU.h:19-22); the
+// host loop resets and reads the vote each step (Algorithm/main.cu:283-302,
+// MultipleGPU/MultiUtilities.h:48-69).  Levels are int32, -1 = unreached
+// (Algorithm/Message.h:9-10).
+//
+// Here a superstep only touches the frontier:
+//   push:  frontier queue -> wave-cooperative expansion (lanes share the out-edges
+//          of 64 frontier vertices through an in-wave prefix scan), hubs above
+//          `large` out-degree get split over whole workgroups;
+//   pull:  every unvisited vertex scans its in-edges for a parent in the
+//          frontier bitmap and stops at the first hit;
+//   compaction: one pass builds the next frontier bitmap (ballot per 64 vertices)
+//          and the small/large queues (ballot + in-workgroup prefix scan, one
+//          atomic per workgroup per queue) and counts the frontier's out-edges.
+// The claim `if (level[v] < 0) level[v] = step+1` is a plain store: every
+// writer stores the same value, so the race is benign (as in the reference).
+
+#include <algorithm>
+
+#include "engine.h"
+#include "sched.h"
+
+namespace medusa {
+
+constexpr uint32_t kLargeDefault = 2048;
+constexpr uint32_t kLargeSplit = 16;  // workgroups per hub in the large-vertex push
+
+__global__ void k_bfs_init(uint32_t n, int32_t* __restrict__ level, uint32_t src, uint32_t* __restrict__ q_small,
+                           uint32_t* __restrict__ q_large, uint32_t src_class, unsigned long long* __restrict__ bitmap,
+                           uint32_t nwords) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (uint32_t v = tid; v < n; v += stride) level[v] = (v == src) ? 0 : -1;
+    for (uint32_t w = tid; w < nwords; w += stride) bitmap[w] = (w == src / 64) ? (1ull << (src % 64)) : 0ull;
+    if (tid == 0) {
+        if (src_class == 1) q_small[0] = src;
+        if (src_class == 2) q_large[0] = src;
+    }
+}
+
+__global__ void k_zero_counters(FrontierCounters* c) {
+    c->n_small = 0;
+    c->n_large = 0;
+    c->n_new = 0;
+    c->pad = 0;
+    c->deg_sum = 0;
+    c->probes = 0;
+}
+
+// Wave-cooperative push over a queue of vertices with 1 <= outdeg <= large.
+__global__ __launch_bounds__(256) void k_bfs_push(const uint32_t* __restrict__ q, uint32_t qn,
+                                                  const uint32_t* __restrict__ row_ptr,
+                                                  const uint32_t* __restrict__ col, int32_t* __restrict__ level,
+                                                  int32_t next) {
+    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const uint32_t total_waves = gridDim.x * 4;
+    for (uint32_t base = (blockIdx.x * 4 + wv) * kWave; base < qn; base += total_waves * kWave) {
+        const uint32_t i = base + lane;
+        uint32_t beg = 0, deg = 0;
+        if (i < qn) {
+            const uint32_t u = q[i];
+            beg = row_ptr[u];
+            deg = row_ptr[u + 1] - beg;
+        }
+        const uint32_t incl = wave_incl_scan(deg);
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        s_beg[wv][lane] = beg;
+        s_off[wv][lane] = incl - deg;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t e = lane; e < total; e += kWave) {
+            // owner = last lane whose exclusive offset is <= e (it has deg > 0)
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = kWave / 2; step; step >>= 1)
+                if (s_off[wv][lo + step] <= e) lo += step;
+            const uint32_t v = col[s_beg[wv][lo] + (e - s_off[wv][lo])];
+            if (level[v] < 0) level[v] = next;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Hubs: vertex i of the queue is split into kLargeSplit contiguous edge ranges,
+// one workgroup each.
+__global__ __launch_bounds__(256) void k_bfs_push_large(const uint32_t* __restrict__ q, uint32_t qn,
+                                                        const uint32_t* __restrict__ row_ptr,
+                                                        const uint32_t* __restrict__ col,
+                                                        int32_t* __restrict__ level, int32_t next) {
+    for (uint32_t w = blockIdx.x; w < qn * kLargeSplit; w += gridDim.x) {
+        const uint32_t u = q[w / kLargeSplit], part = w % kLargeSplit;
+        const uint32_t b = row_ptr[u], deg = row_ptr[u + 1] - b;
+        const uint32_t per = (deg + kLargeSplit - 1) / kLargeSplit;
+        const uint32_t lo = b + min(deg, part * per), hi = b + min(deg, (part + 1) * per);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+            const uint32_t v = col[k];
+            if (level[v] < 0) level[v] = next;
+        }
+    }
+}
+
+// Bottom-up: an unvisited vertex joins if any in-neighbour is in the frontier bitmap.
+__global__ __launch_bounds__(256) void k_bfs_pull(uint32_t n, const uint32_t* __restrict__ cptr,
+                                                  const uint32_t* __restrict__ csrc,
+                                                  const unsigned long long* __restrict__ bitmap,
+                                                  int32_t* __restrict__ level, int32_t next,
+                                                  FrontierCounters* __restrict__ ctr) {
+    unsigned long long probes = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        if (level[v] >= 0) continue;
+        const uint32_t b = cptr[v], e = cptr[v + 1];
+        for (uint32_t k = b; k < e; ++k) {
+            const uint32_t u = csrc[k];
+            ++probes;
+            if ((bitmap[u >> 6] >> (u & 63)) & 1ull) {
+                level[v] = next;
+                break;
+            }
+        }
+    }
+    probes = wave_sum_u64(probes);
+    if ((threadIdx.x & (kWave - 1)) == 0 && probes) atomicAdd(&ctr->probes, probes);
+}
+
+constexpr uint32_t kCompactK = 8;                          // 64-vertex chunks per wave per tile
+constexpr uint32_t kCompactTile = 4 * kCompactK * kWave;   // vertices per workgroup tile
+
+// Next frontier = {v : level[v] == want}: bitmap word per 64 vertices, queues by
+// out-degree class, and frontier counters.
+__global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
+                                                     const uint32_t* __restrict__ row_ptr, uint32_t large,
+                                                     uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
+                                                     unsigned long long* __restrict__ bitmap, uint32_t nwords,
+                                                     FrontierCounters* __restrict__ ctr) {
+    __shared__ uint32_t s_cnt[4][3];
+    __shared__ unsigned long long s_deg[4];
+    __shared__ uint32_t s_base[2];
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (uint32_t tile = blockIdx.x; (uint64_t)tile * kCompactTile < n; tile += gridDim.x) {
+        unsigned long long msm[kCompactK], mlg[kCompactK];
+        uint32_t csm = 0, clg = 0, cnew = 0;
+        unsigned long long dsum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kCompactK; ++k) {
+            const uint32_t chunk = tile * (4 * kCompactK) + k * 4 + wv;
+            const uint32_t v = chunk * kWave + lane;
+            const bool pred = v < n && level[v] == want;
+            const unsigned long long mask = __ballot(pred);
+            if (lane == 0 && chunk < nwords) bitmap[chunk] = mask;
+            uint32_t deg = 0;
+            if (pred) deg = row_ptr[v + 1] - row_ptr[v];
+            msm[k] = __ballot(pred && deg > 0 && deg <= large);
+            mlg[k] = __ballot(pred && deg > large);
+            cnew += __popcll(mask);
+            csm += __popcll(msm[k]);
+            clg += __popcll(mlg[k]);
+            dsum += deg;
+        }
+        dsum = wave_sum_u64(dsum);
+        if (lane == 0) {
+            s_cnt[wv][0] = csm;
+            s_cnt[wv][1] = clg;
+            s_cnt[wv][2] = cnew;
+            s_deg[wv] = dsum;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tsm = 0, tlg = 0, tnew = 0;
+            unsigned long long tdeg = 0;
+            for (int w = 0; w < 4; ++w) {
+                tsm += s_cnt[w][0];
+                tlg += s_cnt[w][1];
+                tnew += s_cnt[w][2];
+                tdeg += s_deg[w];
+            }
+            s_base[0] = tsm ? atomicAdd(&ctr->n_small, tsm) : 0u;
+            s_base[1] = tlg ? atomicAdd(&ctr->n_large, tlg) : 0u;
+            if (tnew) atomicAdd(&ctr->n_new, tnew);
+            if (tdeg) atomicAdd(&ctr->deg_sum, tdeg);
+        }
+        __syncthreads();
+        uint32_t osm = s_base[0], olg = s_base[1];
+        for (uint32_t w = 0; w < wv; ++w) {
+            osm += s_cnt[w][0];
+            olg += s_cnt[w][1];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kCompactK; ++k) {
+            const uint32_t v = (tile * (4 * kCompactK) + k * 4 + wv) * kWave + lane;
+            if ((msm[k] >> lane) & 1ull) q_small[osm + lanes_below(msm[k])] = v;
+            if ((mlg[k] >> lane) & 1ull) q_large[olg + lanes_below(mlg[k])] = v;
+            osm += __popcll(msm[k]);
+            olg += __popcll(mlg[k]);
+        }
+        __syncthreads();
+    }
+}
+
+static void bfs_run(medusa_graph* g, uint32_t src, int32_t* out, const medusa_opts& o, medusa_stats& st) {
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const uint32_t nwords = (n + 63) / 64;
+    const uint32_t large = o.large_degree ? o.large_degree : kLargeDefault;
+    g->level.ensure(n);
+    g->q_small.ensure(n);
+    g->q_large.ensure(n);
+    g->bitmap.ensure(nwords);
+    hipStream_t s = g->stream;
+    const uint32_t deg_src = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
+    const uint32_t src_class = deg_src == 0 ? 0u : (deg_src <= large ? 1u : 2u);
+    const bool timing = o.timing != 0;
+    g->trace.clear();
+
+    MDS_HIP(hipEventRecord(g->ev_begin, s));
+    hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->level.p, src, g->q_small.p,
+                       g->q_large.p, src_class, g->bitmap.p, nwords);
+    MDS_LAUNCHED();
+
+    Frontier f{};
+    f.step = 0;
+    f.n = n;
+    f.m = m;
+    f.n_f = 1;
+    f.m_f = deg_src;
+    f.n_u = n - 1;
+    f.m_u = m - deg_src;
+    uint32_t n_small = src_class == 1, n_large = src_class == 2;
+    uint64_t reached_edges = deg_src, reached = 1, bytes = 4ull * n + nwords * 8ull;
+    bool pull = o.direction == MEDUSA_DIR_PULL;
+    Sched sched(o, Sched::kBfs);
+    size_t ev = 0;
+    for (uint32_t t = 0;; ++t) {
+        f.step = t;
+        if (o.direction == MEDUSA_DIR_PUSH) pull = false;
+        else if (o.direction == MEDUSA_DIR_PULL) pull = true;
+        else pull = sched.bfs_pull(f, pull);
+        if (timing) {
+            ensure_events(g, ev + 2);
+            MDS_HIP(hipEventRecord(g->events[ev], s));
+        }
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, dev_counters<FrontierCounters>(g));
+        MDS_LAUNCHED();
+        if (!pull) {
+            if (n_small) {
+                const uint32_t waves = (n_small + kWave - 1) / kWave;
+                hipLaunchKernelGGL(k_bfs_push, dim3(grid_for(waves, 4, 16384)), dim3(256), 0, s, g->q_small.p, n_small,
+                                   g->row_ptr.p, g->col.p, g->level.p, (int32_t)(t + 1));
+                MDS_LAUNCHED();
+            }
+            if (n_large) {
+                hipLaunchKernelGGL(k_bfs_push_large, dim3(grid_for((uint64_t)n_large * kLargeSplit, 1, 16384)),
+                                   dim3(256), 0, s, g->q_large.p, n_large, g->row_ptr.p, g->col.p, g->level.p,
+                                   (int32_t)(t + 1));
+                MDS_LAUNCHED();
+            }
+        } else {
+            hipLaunchKernelGGL(k_bfs_pull, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, g->cptr.p, g->csrc.p,
+                               (const unsigned long long*)g->bitmap.p, g->level.p, (int32_t)(t + 1), dev_counters<FrontierCounters>(g));
+            MDS_LAUNCHED();
+        }
+        if (timing) MDS_HIP(hipEventRecord(g->events[ev + 1], s));
+        hipLaunchKernelGGL(k_bfs_compact, dim3(grid_for(n, kCompactTile, 4096)), dim3(256), 0, s, n,
+                           (const int32_t*)g->level.p, (int32_t)(t + 1), g->row_ptr.p, large, g->q_small.p,
+                           g->q_large.p, g->bitmap.p, nwords, dev_counters<FrontierCounters>(g));
+        MDS_LAUNCHED();
+        const FrontierCounters c = read_counters<FrontierCounters>(g);
+        double kms = 0;
+        if (timing) {
+            float ms = 0;
+            MDS_HIP(hipEventElapsedTime(&ms, g->events[ev], g->events[ev + 1]));
+            kms = ms;
+            st.kernel_ms += ms;
+            ++st.kernel_launches;
+        }
+        // byte model (see DESIGN): push = queue + offsets + col + level probe + claims;
+        // pull = level scan + probed in-edges (source + bitmap word) + claims
+        uint64_t kb;
+        if (!pull) {
+            kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * c.n_new;
+            ++st.push_steps;
+        } else {
+            kb = 4ull * n + 4ull * (n + 1) + 8ull * c.probes + 4ull * c.n_new;
+            ++st.pull_steps;
+        }
+        st.kernel_bytes += kb;
+        bytes += kb + 4ull * n + nwords * 8ull + 12ull * c.n_new;
+        sched.record(g, f, pull ? 1.0 : 0.0, kms);
+        ++st.supersteps;
+        if (c.n_new == 0) break;
+        f.n_f = c.n_new;
+        f.m_f = c.deg_sum;
+        f.n_u -= c.n_new;
+        f.m_u -= c.deg_sum;
+        n_small = c.n_small;
+        n_large = c.n_large;
+        reached += c.n_new;
+        reached_edges += c.deg_sum;
+    }
+    MDS_HIP(hipEventRecord(g->ev_end, s));
+    MDS_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+    st.device_ms = ms;
+    st.edges_traversed = reached_edges;
+    st.reached = (uint32_t)reached;
+    st.alg_bytes = bytes;
+    copy_out(g, out, g->level.p, (size_t)n * 4);
+}
+
+}  // namespace medusa
+
+using namespace medusa;
+
+extern "C" int medusa_bfs(medusa_graph* g, uint32_t source, int32_t* levels, const medusa_opts* opts,
+                          medusa_stats* stats) {
+    return guard([&] {
+        enter(g);
+        MDS_CHECK(levels, MEDUSA_ERR_ARG, "bfs: null output");
+        MDS_CHECK(source < g->n, MEDUSA_ERR_ARG, "bfs: source %u out of range (n=%u)", source, g->n);
+        medusa_opts o{};
+        if (opts) o = *opts;
+        MDS_CHECK(o.direction >= 0 && o.direction <= 2, MEDUSA_ERR_ARG, "bfs: bad direction %d", o.direction);
+        medusa_stats st{};
+        bfs_run(g, source, levels, o, st);
+        if (stats) *stats = st;
+    });
+}

@@ -1,0 +1,258 @@
+// Device-side graph object and helpers shared by the engine's HIP sources.
+//
+// One medusa_graph owns every device allocation for one graph on one GPU:
+// CSR (out-edges) for push supersteps, CSC (in-edges, sources ascending) for
+// pull supersteps and combines, plus lazily built per-algorithm layouts and
+// work buffers.  Nothing lives in global symbols (the reference bound its
+// arrays to __constant__ symbols, Algorithm/DeviceDataStructure.h:7-24), so any
+// number of graphs can coexist in one process.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "common.h"
+
+namespace medusa {
+
+#define MDS_HIP(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            ::medusa::fail(MEDUSA_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                           __LINE__);                                                                \
+    } while (0)
+
+#define MDS_LAUNCHED() MDS_HIP(hipGetLastError())
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// RAII device array.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) {
+            hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+            if (e != hipSuccess) {
+                p = nullptr;
+                fail(e == hipErrorOutOfMemory ? MEDUSA_ERR_NOMEM : MEDUSA_ERR_HIP, "hipMalloc(%zu bytes): %s",
+                     count * sizeof(T), hipGetErrorString(e));
+            }
+            n = count;
+        }
+    }
+    void ensure(size_t count) {
+        if (n < count) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) MDS_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// RAII pinned host array (hipHostMalloc; no registration of existing memory).
+template <class T>
+struct HBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HBuf() = default;
+    HBuf(const HBuf&) = delete;
+    HBuf& operator=(const HBuf&) = delete;
+    ~HBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t count) {
+        if (n >= count) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        MDS_HIP(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+};
+
+// Frontier counters written by the compaction kernels, read back each superstep.
+struct FrontierCounters {
+    uint32_t n_small;    // frontier vertices with 1 <= outdeg <= large threshold (wave-cooperative queue)
+    uint32_t n_large;    // frontier vertices with outdeg > threshold (workgroup queue)
+    uint32_t n_new;      // vertices that joined the frontier (any outdeg)
+    uint32_t pad;
+    unsigned long long deg_sum;  // sum of their out-degrees
+    unsigned long long probes;   // in-edges examined by a pull superstep
+};
+
+// One contiguous run of PageRank vertices handled with the same lane mapping.
+struct PrRun {
+    uint32_t begin;     // first internal vertex id
+    uint32_t count;     // vertices in the run
+    uint32_t kind;      // 0..6: groups of 2^kind lanes; kPrWave: wave per vertex; kPrZero: no in-edges
+    uint32_t wg_begin;  // first workgroup of this run in the launch
+};
+constexpr uint32_t kPrWave = 7;
+constexpr uint32_t kPrZero = 8;
+constexpr int kPrMaxRuns = 24;
+struct PrRunTable {
+    PrRun r[kPrMaxRuns];
+    uint32_t nruns;
+    uint32_t total_wgs;
+};
+
+// In-degree-sorted pull layout for PageRank (built on first use).
+// Internal id order: non-dangling vertices first, then dangling (outdeg 0);
+// within each, in-degree descending, ties by original id.  Every lane-mapping
+// class is therefore one contiguous id range, writes of rank/contrib are
+// contiguous, and the dangling vertices form the tail [dangle_begin, n).
+struct PrLayout {
+    uint32_t dangle_begin = 0;
+    PrRunTable table{};
+    DBuf<uint32_t> perm;    // internal -> original
+    DBuf<uint32_t> iperm;   // original -> internal
+    DBuf<uint32_t> cptr;    // in-edge offsets in internal order (n+1)
+    DBuf<uint32_t> crow;    // in-edge sources, internal ids, original-source-ascending per vertex (m)
+    DBuf<uint32_t> outdeg;  // out-degree per internal id (n)
+    DBuf<uint32_t> optr;    // out-edge offsets in internal order (n+1), push variant
+    DBuf<uint32_t> oslot;   // in-edge slot of every out-edge (m), push variant
+    // work
+    DBuf<float> rank, rank_prev, contrib0, contrib1, msg, base, unperm;
+    DBuf<float> partials;   // (s, c) pairs of the dangling-sum blocks
+    DBuf<uint32_t> counter;
+    DBuf<double> resid;
+    // captured superstep sequence, reused while (steps, variant, damping) stay the same
+    hipGraphExec_t exec = nullptr;
+    uint32_t exec_steps = 0;
+    int exec_variant = -1;
+    float exec_damping = -1.f;
+    ~PrLayout() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+struct TraceRec {
+    double f[MEDUSA_TRACE_FIELDS];
+};
+
+}  // namespace medusa
+
+struct medusa_graph {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    bool weighted = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // host copies (layouts are derived from them)
+    std::vector<uint32_t> h_row_ptr, h_col, h_w, h_cptr, h_csrc, h_cw, h_slot;
+    // device CSR / CSC
+    medusa::DBuf<uint32_t> row_ptr, col, w, cptr, csrc, cw, slot;
+    // traversal work buffers
+    medusa::DBuf<int32_t> level;
+    medusa::DBuf<uint32_t> dist, q_small, q_large, q_aux, stamp;
+    medusa::DBuf<unsigned long long> bitmap;
+    // per-superstep counters (device) and their pinned host mirror; each engine
+    // views the 256 bytes as its own counter struct
+    medusa::DBuf<unsigned long long> ctr_raw;
+    medusa::HBuf<unsigned long long> h_ctr_raw;
+    std::unique_ptr<medusa::PrLayout> pr;
+    std::vector<medusa::TraceRec> trace;
+    std::vector<hipEvent_t> events;  // timing pool
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+};
+
+namespace medusa {
+
+// Makes g's device current and returns it.
+void enter(const medusa_graph* g);
+// Grows the timing-event pool to at least `count` events.
+void ensure_events(medusa_graph* g, size_t count);
+// Copies `bytes` from device pointer `src` to `dst`, which may be host or device memory.
+void copy_out(medusa_graph* g, void* dst, const void* src, size_t bytes);
+// Sums pairwise (start, stop) event intervals [0, pairs) in milliseconds.
+double sum_event_pairs(medusa_graph* g, size_t pairs);
+// Number of workgroups for a grid-stride launch over `items` with `per_wg` items each.
+inline uint32_t grid_for(uint64_t items, uint32_t per_wg, uint32_t cap = 8192) {
+    uint64_t g = (items + per_wg - 1) / per_wg;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+void build_pr_layout(medusa_graph* g);
+
+constexpr size_t kCounterWords = 32;  // 256 bytes
+template <class T>
+T* dev_counters(medusa_graph* g) {
+    static_assert(sizeof(T) <= kCounterWords * 8, "counter struct too large");
+    return reinterpret_cast<T*>(g->ctr_raw.p);
+}
+template <class T>
+T* host_counters(medusa_graph* g) {
+    return reinterpret_cast<T*>(g->h_ctr_raw.p);
+}
+// Copies the device counters to the pinned mirror and waits for the stream.
+template <class T>
+T read_counters(medusa_graph* g) {
+    MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, g->ctr_raw.p, sizeof(T), hipMemcpyDeviceToHost, g->stream));
+    MDS_HIP(hipStreamSynchronize(g->stream));
+    return *host_counters<T>(g);
+}
+
+}  // namespace medusa
+
+// ---- device-side primitives ---------------------------------------------
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+namespace medusa {
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+// Number of set bits of `mask` in lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, kWave);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    return x;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
+    return x;
+}
+
+// Compensated summation used by every FP32 combine.  A pair (s, c) stands
+// for the value s - c.  kahan_add is Kahan's update; kahan_combine merges two
+// pairs with an error-free TwoSum of the leading parts.  Both are exact
+// identities against the pair (0, 0), so zero-padded lanes do not change a sum.
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+__device__ __forceinline__ void kahan_combine(float& s, float& c, float s2, float c2) {
+    const float t = s + s2;
+    const float bp = t - s;
+    const float err = (s - (t - bp)) + (s2 - bp);
+    c = (c + c2) - err;
+    s = t;
+}
+
+}  // namespace medusa
+#endif

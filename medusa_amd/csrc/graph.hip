@@ -1,0 +1,294 @@
+// Graph storage: CSR/CSC upload, the C-ABI library entry points, and the
+// PageRank pull layout.
+//
+// Reference counterparts: VertexArray::build (CSR offsets, Algorithm/VertexDataType.cu:60-80),
+// EdgeArray::buildAA (CSR order, CSC offsets msg_index, msgDstID slot map, segment-head
+// flags; SSSP/EdgeDataType.cu:138-186), D_*::Fill uploads and InitDeviceDS
+// (MultipleGPU/MultiPublicAPI.h:144-308), sortByDegree relabelling for MEG/HY
+// (MedusaRT/GraphConverter.cu:136-165, MultiPublicAPI.h:52-53).
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <numeric>
+
+#include "engine.h"
+
+namespace medusa {
+
+static thread_local std::string t_last_error;
+
+void set_last_error(const char* msg) { t_last_error = msg; }
+void clear_last_error() { t_last_error.clear(); }
+
+void fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Error(code, buf);
+}
+
+void enter(const medusa_graph* g) {
+    MDS_CHECK(g != nullptr, MEDUSA_ERR_ARG, "null graph handle");
+    MDS_HIP(hipSetDevice(g->device));
+}
+
+void ensure_events(medusa_graph* g, size_t count) {
+    while (g->events.size() < count) {
+        hipEvent_t e;
+        MDS_HIP(hipEventCreate(&e));
+        g->events.push_back(e);
+    }
+}
+
+double sum_event_pairs(medusa_graph* g, size_t pairs) {
+    double ms = 0;
+    for (size_t i = 0; i < pairs; ++i) {
+        float t = 0;
+        MDS_HIP(hipEventElapsedTime(&t, g->events[2 * i], g->events[2 * i + 1]));
+        ms += t;
+    }
+    return ms;
+}
+
+void copy_out(medusa_graph* g, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    hipPointerAttribute_t attr{};
+    hipError_t e = hipPointerGetAttributes(&attr, dst);
+    (void)hipGetLastError();  // host pointers unknown to HIP report an error; that is the "host" answer
+    const bool dev = (e == hipSuccess && attr.type == hipMemoryTypeDevice);
+    MDS_HIP(hipMemcpyAsync(dst, src, bytes, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g->stream));
+    MDS_HIP(hipStreamSynchronize(g->stream));
+}
+
+// Internal id order for PageRank: (dangling last, in-degree descending, original id).
+void build_pr_layout(medusa_graph* g) {
+    if (g->pr) return;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    auto L = std::make_unique<PrLayout>();
+    std::vector<uint32_t> perm(n), iperm(n), indeg(n), outdeg(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        indeg[v] = g->h_cptr[v + 1] - g->h_cptr[v];
+        outdeg[v] = g->h_row_ptr[v + 1] - g->h_row_ptr[v];
+    }
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+        const bool da = outdeg[a] == 0, db = outdeg[b] == 0;
+        if (da != db) return db;  // non-dangling first
+        if (indeg[a] != indeg[b]) return indeg[a] > indeg[b];
+        return a < b;
+    });
+    for (uint32_t p = 0; p < n; ++p) iperm[perm[p]] = p;
+    std::vector<uint32_t> cptr(n + 1, 0), crow(m), od(n), optr(n + 1, 0), oslot(m);
+    for (uint32_t p = 0; p < n; ++p) {
+        cptr[p + 1] = cptr[p] + indeg[perm[p]];
+        od[p] = outdeg[perm[p]];
+        optr[p + 1] = optr[p] + od[p];
+    }
+    parallel_blocks(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t p = b; p < e; ++p) {
+            const uint32_t v = perm[p];
+            const uint32_t s0 = g->h_cptr[v], s1 = g->h_cptr[v + 1];
+            uint32_t o = cptr[p];
+            for (uint32_t s = s0; s < s1; ++s) crow[o++] = iperm[g->h_csrc[s]];
+        }
+    });
+    // push variant: for every out-edge of internal vertex p (in CSR order), its in-edge slot
+    parallel_blocks(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t p = b; p < e; ++p) {
+            const uint32_t v = perm[p];
+            uint32_t o = optr[p];
+            for (uint32_t k = g->h_row_ptr[v]; k < g->h_row_ptr[v + 1]; ++k) {
+                const uint32_t dst = g->h_col[k];
+                // original CSC slot of edge k, re-based to the destination's internal segment
+                oslot[o++] = cptr[iperm[dst]] + (g->h_slot[k] - g->h_cptr[dst]);
+            }
+        }
+    });
+    uint32_t dangle_begin = n;
+    for (uint32_t p = 0; p < n; ++p)
+        if (od[p] == 0) {
+            dangle_begin = p;
+            break;
+        }
+    L->dangle_begin = dangle_begin;
+    // runs of equal lane mapping
+    auto kind_of = [](uint32_t deg) -> uint32_t {
+        if (deg == 0) return kPrZero;
+        if (deg > 64) return kPrWave;
+        uint32_t k = 0;
+        while ((1u << k) < deg) ++k;
+        return k;
+    };
+    auto per_wg = [](uint32_t kind) -> uint32_t {
+        if (kind == kPrZero) return 256;
+        if (kind == kPrWave) return 4;
+        return 4 * (64u >> kind);
+    };
+    PrRunTable& T = L->table;
+    T.nruns = 0;
+    uint32_t wg = 0;
+    for (uint32_t p = 0; p < n;) {
+        const uint32_t kind = kind_of(cptr[p + 1] - cptr[p]);
+        const bool dang = od[p] == 0;
+        uint32_t q = p + 1;
+        while (q < n && kind_of(cptr[q + 1] - cptr[q]) == kind && (od[q] == 0) == dang) ++q;
+        MDS_CHECK(T.nruns < (uint32_t)kPrMaxRuns, MEDUSA_ERR_INTERNAL, "pagerank layout: too many runs");
+        PrRun& r = T.r[T.nruns++];
+        r.begin = p;
+        r.count = q - p;
+        r.kind = kind;
+        r.wg_begin = wg;
+        wg += (r.count + per_wg(kind) - 1) / per_wg(kind);
+        p = q;
+    }
+    T.total_wgs = wg;
+    hipStream_t s = g->stream;
+    L->perm.upload(perm.data(), n, s);
+    L->iperm.upload(iperm.data(), n, s);
+    L->cptr.upload(cptr.data(), (size_t)n + 1, s);
+    L->crow.upload(crow.data(), m, s);
+    L->outdeg.upload(od.data(), n, s);
+    L->optr.upload(optr.data(), (size_t)n + 1, s);
+    L->oslot.upload(oslot.data(), m, s);
+    MDS_HIP(hipStreamSynchronize(s));
+    g->pr = std::move(L);
+}
+
+}  // namespace medusa
+
+using namespace medusa;
+
+extern "C" const char* medusa_last_error(void) { return t_last_error.c_str(); }
+
+extern "C" const char* medusa_version(void) { return "medusa-mi355x 0.1 (gfx950)"; }
+
+extern "C" int medusa_device_count(int* count) {
+    return guard([&] {
+        MDS_CHECK(count, MEDUSA_ERR_ARG, "null count");
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+        (void)hipGetLastError();
+        *count = c;
+    });
+}
+
+extern "C" int medusa_graph_create(uint32_t n, uint64_t m, const uint32_t* row_ptr, const uint32_t* col,
+                                   const uint32_t* weight, medusa_graph** out) {
+    return guard([&] {
+        MDS_CHECK(out, MEDUSA_ERR_ARG, "graph_create: null out");
+        *out = nullptr;
+        MDS_CHECK(n > 0, MEDUSA_ERR_ARG, "graph_create: n must be > 0");
+        MDS_CHECK(m < (1ull << 32), MEDUSA_ERR_ARG, "graph_create: m must be < 2^32");
+        MDS_CHECK(row_ptr && (m == 0 || col), MEDUSA_ERR_ARG, "graph_create: null CSR array");
+        MDS_CHECK(row_ptr[0] == 0 && row_ptr[n] == m, MEDUSA_ERR_ARG,
+                  "graph_create: row_ptr[0] must be 0 and row_ptr[n] must equal m");
+        for (uint32_t v = 0; v < n; ++v)
+            MDS_CHECK(row_ptr[v] <= row_ptr[v + 1], MEDUSA_ERR_ARG, "graph_create: row_ptr decreases at %u", v);
+        for (uint64_t e = 0; e < m; ++e)
+            MDS_CHECK(col[e] < n, MEDUSA_ERR_ARG, "graph_create: col[%llu]=%u out of range", (unsigned long long)e,
+                      col[e]);
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;
+        (void)hipGetLastError();
+        MDS_CHECK(ndev > 0, MEDUSA_ERR_HIP, "graph_create: no GPU visible");
+
+        auto g = std::make_unique<medusa_graph>();
+        g->n = n;
+        g->m = m;
+        g->weighted = weight != nullptr;
+        MDS_HIP(hipGetDevice(&g->device));
+        MDS_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        MDS_HIP(hipEventCreate(&g->ev_begin));
+        MDS_HIP(hipEventCreate(&g->ev_end));
+        g->h_row_ptr.assign(row_ptr, row_ptr + n + 1);
+        g->h_col.assign(col, col + m);
+        if (weight) g->h_w.assign(weight, weight + m);
+        // CSC: sources ascending within each destination (a stable pass over CSR);
+        // slot[e] = CSC position of CSR edge e (the reference's msgDstID).
+        g->h_cptr.assign((size_t)n + 1, 0);
+        for (uint64_t e = 0; e < m; ++e) ++g->h_cptr[col[e] + 1];
+        for (uint32_t v = 0; v < n; ++v) g->h_cptr[v + 1] += g->h_cptr[v];
+        g->h_csrc.resize(m);
+        g->h_slot.resize(m);
+        if (weight) g->h_cw.resize(m);
+        std::vector<uint32_t> fill(g->h_cptr.begin(), g->h_cptr.end() - 1);
+        for (uint32_t u = 0; u < n; ++u)
+            for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+                const uint32_t pos = fill[col[e]]++;
+                g->h_csrc[pos] = u;
+                g->h_slot[e] = pos;
+                if (weight) g->h_cw[pos] = weight[e];
+            }
+        hipStream_t s = g->stream;
+        g->row_ptr.upload(g->h_row_ptr.data(), (size_t)n + 1, s);
+        g->col.upload(g->h_col.data(), m, s);
+        g->cptr.upload(g->h_cptr.data(), (size_t)n + 1, s);
+        g->csrc.upload(g->h_csrc.data(), m, s);
+        g->slot.upload(g->h_slot.data(), m, s);
+        if (weight) {
+            g->w.upload(g->h_w.data(), m, s);
+            g->cw.upload(g->h_cw.data(), m, s);
+        }
+        g->ctr_raw.alloc(kCounterWords);
+        g->h_ctr_raw.ensure(kCounterWords);
+        MDS_HIP(hipStreamSynchronize(s));
+        *out = g.release();
+    });
+}
+
+extern "C" int medusa_graph_destroy(medusa_graph* g) {
+    return guard([&] {
+        if (!g) return;
+        (void)hipSetDevice(g->device);
+        if (g->stream) (void)hipStreamSynchronize(g->stream);
+        for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
+        if (g->ev_begin) (void)hipEventDestroy(g->ev_begin);
+        if (g->ev_end) (void)hipEventDestroy(g->ev_end);
+        hipStream_t s = g->stream;
+        delete g;  // DBuf destructors free device memory
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+extern "C" int medusa_graph_info(const medusa_graph* g, uint32_t* n, uint64_t* m, uint32_t* weighted) {
+    return guard([&] {
+        MDS_CHECK(g, MEDUSA_ERR_ARG, "null graph");
+        if (n) *n = g->n;
+        if (m) *m = g->m;
+        if (weighted) *weighted = g->weighted ? 1u : 0u;
+    });
+}
+
+extern "C" int medusa_graph_device_arrays(const medusa_graph* g, const uint32_t** row_ptr, const uint32_t** col,
+                                          const uint32_t** weight, const uint32_t** csc_ptr,
+                                          const uint32_t** csc_src, const uint32_t** msg_slot) {
+    return guard([&] {
+        MDS_CHECK(g, MEDUSA_ERR_ARG, "null graph");
+        if (row_ptr) *row_ptr = g->row_ptr.p;
+        if (col) *col = g->col.p;
+        if (weight) *weight = g->w.p;
+        if (csc_ptr) *csc_ptr = g->cptr.p;
+        if (csc_src) *csc_src = g->csrc.p;
+        if (msg_slot) *msg_slot = g->slot.p;
+    });
+}
+
+extern "C" int medusa_trace_count(const medusa_graph* g, uint32_t* count) {
+    return guard([&] {
+        MDS_CHECK(g && count, MEDUSA_ERR_ARG, "null argument");
+        *count = (uint32_t)g->trace.size();
+    });
+}
+
+extern "C" int medusa_trace_read(const medusa_graph* g, double* out, uint32_t capacity) {
+    return guard([&] {
+        MDS_CHECK(g && (out || capacity == 0), MEDUSA_ERR_ARG, "null argument");
+        const size_t k = std::min<size_t>(capacity, g->trace.size());
+        for (size_t i = 0; i < k; ++i)
+            for (int f = 0; f < MEDUSA_TRACE_FIELDS; ++f) out[i * MEDUSA_TRACE_FIELDS + f] = g->trace[i].f[f];
+    });
+}

@@ -1,0 +1,368 @@
+// PageRank superstep engine (FP32, compensated combine in a fixed order).
+//
+// Reference: MPG/UserOperators.h:175-270 runs per superstep InitMessageBuffer(0.0)
+// (MPG/InitMessage.h), SendMsg* scatter of rank/outdeg into per-in-edge message
+// slots (MPG/UserOperators.h:29-163), a CUDPP segmented ADD scan (MPG/MsgCombiner.h:68-117)
+// and UpdateVertex rank = 0.15 + 0.85*sum (MPG/UserOperators.h:16-26); fixed 100
+// supersteps (MPG/main.cu:166).  Here the formula is the normalised one with
+// dangling mass redistributed:
+//     r_0 = 1/n,  r_{t+1}(v) = (1-d)/n + d*(S_t(v) + D_t/n)
+//     S_t(v) = sum over in-edges u->v of c_t(u),  c_t(u) = r_t(u)/outdeg(u)
+//     D_t = sum of r_t(u) over outdeg(u) == 0.
+//
+// Canonical combine order (identical for every kernel variant and lane mapping,
+// so push/pull and every binning give bit-identical ranks):
+//   a segment of values x_0..x_{L-1} (in-edges in ascending original source id,
+//   or dangling vertices in internal order) is cut into blocks of 4096; in a
+//   block, lane j (0..63) Kahan-adds x_{j}, x_{j+64}, ... in order starting from
+//   (0,0); lanes are then merged by kahan_combine down a tree with offsets
+//   32,16,8,4,2,1 (lane i absorbs lane i+off); block results are merged left to
+//   right with kahan_combine; the value is s - c.
+// Scalars: d_f = (float)d, tele = (float)((1-d)/n), base_t = fmaf(d_f, D_t/(float)n, tele),
+// r_{t+1}(v) = fmaf(d_f, S_t(v), base_t), c = r/(float)outdeg (IEEE division).
+
+#include <cmath>
+
+#include "engine.h"
+
+namespace medusa {
+
+constexpr uint32_t kBlock = 4096;  // elements per canonical block (64 lanes x 64 rounds)
+
+// Tree-merge of the (s, c) pairs held by groups of g lanes (g a power of two,
+// groups aligned within the wave); lane 0 of each group ends with the result.
+__device__ __forceinline__ void group_tree(float& s, float& c, uint32_t g) {
+    for (uint32_t off = g >> 1; off; off >>= 1) {
+        const float s2 = __shfl_down(s, off, kWave);
+        const float c2 = __shfl_down(c, off, kWave);
+        kahan_combine(s, c, s2, c2);
+    }
+}
+
+// Canonical sum of `len` values x(k), computed by one whole wave; every lane
+// returns the same (s - c).
+template <class Load>
+__device__ __forceinline__ float wave_segment_sum(uint32_t len, Load load) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    float as = 0.f, ac = 0.f;
+    for (uint32_t b0 = 0; b0 < len; b0 += kBlock) {
+        const uint32_t lim = min(len - b0, kBlock);
+        float s = 0.f, c = 0.f;
+        uint32_t k = lane;
+        // four loads in flight per lane, added in ascending order
+        for (; k + 3 * kWave < lim; k += 4 * kWave) {
+            const float x0 = load(b0 + k), x1 = load(b0 + k + kWave), x2 = load(b0 + k + 2 * kWave),
+                        x3 = load(b0 + k + 3 * kWave);
+            kahan_add(s, c, x0);
+            kahan_add(s, c, x1);
+            kahan_add(s, c, x2);
+            kahan_add(s, c, x3);
+        }
+        for (; k < lim; k += kWave) kahan_add(s, c, load(b0 + k));
+        group_tree(s, c, kWave);
+        s = __shfl(s, 0, kWave);
+        c = __shfl(c, 0, kWave);
+        if (b0 == 0) {
+            as = s;
+            ac = c;
+        } else {
+            kahan_combine(as, ac, s, c);
+        }
+    }
+    return as - ac;
+}
+
+// One superstep for every vertex.  kFromMsg: values come from the message
+// array in slot order (push variant) instead of contrib[src] (pull variant).
+template <bool kFromMsg>
+__global__ __launch_bounds__(256) void k_pr_main(PrRunTable tab, const uint32_t* __restrict__ cptr,
+                                                 const uint32_t* __restrict__ crow,
+                                                 const float* __restrict__ vals,
+                                                 const uint32_t* __restrict__ outdeg,
+                                                 const float* __restrict__ base_p, float d,
+                                                 float* __restrict__ rank_out, float* __restrict__ contrib_out) {
+    const uint32_t wg = blockIdx.x;
+    uint32_t ri = 0;
+    while (ri + 1 < tab.nruns && wg >= tab.r[ri + 1].wg_begin) ++ri;
+    const uint32_t begin = tab.r[ri].begin;
+    const uint32_t end = begin + tab.r[ri].count;
+    const uint32_t kind = tab.r[ri].kind;
+    const uint32_t local = wg - tab.r[ri].wg_begin;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const float base = *base_p;
+
+    auto finish = [&](uint32_t p, float S) {
+        const float r = fmaf(d, S, base);
+        rank_out[p] = r;
+        const uint32_t od = outdeg[p];
+        if (od) contrib_out[p] = r / (float)od;
+    };
+
+    if (kind == kPrZero) {
+        const uint32_t p = begin + local * 256 + threadIdx.x;
+        if (p < end) finish(p, 0.f);
+        return;
+    }
+    if (kind == kPrWave) {
+        const uint32_t p = begin + local * 4 + wave;
+        if (p >= end) return;  // uniform across the wave
+        const uint32_t b = cptr[p], len = cptr[p + 1] - b;
+        const float S = wave_segment_sum(len, [&](uint32_t k) {
+            return kFromMsg ? vals[b + k] : vals[crow[b + k]];
+        });
+        if (lane == 0) finish(p, S);
+        return;
+    }
+    // groups of g = 2^kind lanes, one vertex per group, in-degree in (g/2, g]
+    const uint32_t g = 1u << kind;
+    const uint32_t p = begin + (local * 4 + wave) * (kWave >> kind) + (lane >> kind);
+    const uint32_t j = lane & (g - 1);
+    const bool valid = p < end;
+    float s = 0.f, c = 0.f;
+    if (valid) {
+        const uint32_t b = cptr[p], len = cptr[p + 1] - b;
+        if (j < len) s = kFromMsg ? vals[b + j] : vals[crow[b + j]];
+    }
+    group_tree(s, c, g);
+    if (valid && j == 0) finish(p, s - c);
+}
+
+// Dangling mass D = canonical sum of rank[dbegin + i], i < nd; the last
+// workgroup to finish merges the block partials in order and writes
+// base = fmaf(d, D / nf, tele).
+__global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ rank, uint32_t dbegin, uint32_t nd,
+                                                   float* __restrict__ partials, uint32_t* __restrict__ counter,
+                                                   float d, float nf, float tele, float* __restrict__ base_out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t nblk = (nd + kBlock - 1) / kBlock;
+    const uint32_t b = blockIdx.x * 4 + wave;
+    if (b < nblk) {
+        const uint32_t lo = b * kBlock, lim = min(nd - lo, kBlock);
+        const float* x = rank + dbegin + lo;
+        float s = 0.f, c = 0.f;
+        for (uint32_t k = lane; k < lim; k += kWave) kahan_add(s, c, x[k]);
+        group_tree(s, c, kWave);
+        if (lane == 0) {
+            __hip_atomic_store(&partials[2 * b], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&partials[2 * b + 1], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    __shared__ uint32_t s_last;
+    if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1u : 0u;
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __threadfence();
+    float as = 0.f, ac = 0.f;
+    for (uint32_t i = 0; i < nblk; ++i) {
+        const float s = __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const float c = __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i == 0) {
+            as = s;
+            ac = c;
+        } else {
+            kahan_combine(as, ac, s, c);
+        }
+    }
+    const float D = as - ac;
+    *base_out = fmaf(d, D / nf, tele);
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_pr_init(uint32_t n, float r0, const uint32_t* __restrict__ outdeg, float* __restrict__ rank,
+                          float* __restrict__ contrib) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        rank[p] = r0;
+        const uint32_t od = outdeg[p];
+        if (od) contrib[p] = r0 / (float)od;
+    }
+}
+
+// Push variant: every out-edge (internal CSR order) writes its source's
+// contribution into its destination's in-edge slot.
+__global__ void k_pr_scatter(uint32_t n, const uint32_t* __restrict__ optr, const uint32_t* __restrict__ oslot,
+                             const float* __restrict__ contrib, float* __restrict__ msg) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t p = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; p < n; p += waves) {
+        const uint32_t b = optr[p], e = optr[p + 1];
+        if (b == e) continue;
+        const float x = contrib[p];
+        for (uint32_t k = b + lane; k < e; k += kWave) msg[oslot[k]] = x;
+    }
+}
+
+__global__ void k_pr_unpermute(uint32_t n, const uint32_t* __restrict__ iperm, const float* __restrict__ rank,
+                               float* __restrict__ out) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        out[v] = rank[iperm[v]];
+}
+
+__global__ void k_pr_resid(uint32_t n, const float* __restrict__ a, const float* __restrict__ b,
+                           double* __restrict__ out) {
+    double acc = 0.0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        acc += fabs((double)a[p] - (double)b[p]);
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(out, acc);
+}
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+__global__ void k_set_f32(float* p, float v) { *p = v; }
+
+static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, const medusa_opts& o,
+                   medusa_stats& st) {
+    build_pr_layout(g);
+    PrLayout& L = *g->pr;
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const uint32_t nd = n - L.dangle_begin;
+    const uint32_t nblk = (nd + kBlock - 1) / kBlock;
+    const bool push = o.variant == MEDUSA_PR_PUSH;
+    MDS_CHECK(o.variant == MEDUSA_PR_PULL || push, MEDUSA_ERR_ARG, "pagerank: unknown variant %d", o.variant);
+    L.rank.ensure(n);
+    L.contrib0.ensure(n);
+    L.contrib1.ensure(n);
+    L.base.ensure(2);
+    L.partials.ensure(2 * (size_t)std::max(nblk, 1u));
+    L.counter.ensure(1);
+    L.resid.ensure(1);
+    L.rank_prev.ensure(n);
+    L.unperm.ensure(n);
+    if (push) L.msg.ensure(m);
+    hipStream_t s = g->stream;
+    MDS_HIP(hipMemsetAsync(L.counter.p, 0, sizeof(uint32_t), s));
+    MDS_HIP(hipMemsetAsync(L.contrib0.p, 0, (size_t)n * 4, s));
+    MDS_HIP(hipMemsetAsync(L.contrib1.p, 0, (size_t)n * 4, s));
+
+    const float d_f = damping;
+    const double dd = (double)damping;
+    const float nf = (float)n;
+    const float tele = (float)((1.0 - dd) / (double)n);
+    const float r0 = (float)(1.0 / (double)n);
+    const uint32_t dangle_wgs = std::max(1u, (nblk + 3) / 4);
+
+    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, 256)), dim3(256), 0, s, n, r0, L.outdeg.p, L.rank.p, L.contrib0.p);
+    MDS_LAUNCHED();
+    hipLaunchKernelGGL(k_pr_dangle, dim3(dangle_wgs), dim3(256), 0, s, L.rank.p, L.dangle_begin, nd, L.partials.p,
+                       L.counter.p, d_f, nf, tele, L.base.p);
+    MDS_LAUNCHED();
+
+    const bool timing = o.timing != 0;
+    if (timing) ensure_events(g, 2 * (size_t)steps);
+    float* rk[2] = {L.rank.p, L.rank_prev.p};
+    float* ct[2] = {L.contrib0.p, L.contrib1.p};
+
+    auto enqueue_steps = [&](bool with_events) {
+        for (uint32_t t = 0; t < steps; ++t) {
+            float* r_out = rk[(t + 1) & 1];
+            const float* c_in = ct[t & 1];
+            float* c_out = ct[(t + 1) & 1];
+            if (push) {
+                hipLaunchKernelGGL(k_pr_scatter, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, n, L.optr.p,
+                                   L.oslot.p, c_in, L.msg.p);
+                MDS_LAUNCHED();
+            }
+            if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t], s));
+            if (push)
+                hipLaunchKernelGGL(k_pr_main<true>, dim3(L.table.total_wgs), dim3(256), 0, s, L.table, L.cptr.p,
+                                   L.crow.p, (const float*)L.msg.p, L.outdeg.p, (const float*)L.base.p, d_f, r_out,
+                                   c_out);
+            else
+                hipLaunchKernelGGL(k_pr_main<false>, dim3(L.table.total_wgs), dim3(256), 0, s, L.table, L.cptr.p,
+                                   L.crow.p, c_in, L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
+            MDS_LAUNCHED();
+            if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t + 1], s));
+            if (t + 1 < steps) {
+                hipLaunchKernelGGL(k_pr_dangle, dim3(dangle_wgs), dim3(256), 0, s, (const float*)r_out, L.dangle_begin,
+                                   nd, L.partials.p, L.counter.p, d_f, nf, tele, L.base.p);
+                MDS_LAUNCHED();
+            }
+        }
+    };
+
+    MDS_HIP(hipEventRecord(g->ev_begin, s));
+    if (o.use_graph && !timing && steps > 0) {
+        // the captured sequence only depends on (steps, variant, damping): buffers are fixed per layout
+        if (!L.exec || L.exec_steps != steps || L.exec_variant != o.variant || L.exec_damping != damping) {
+            if (L.exec) (void)hipGraphExecDestroy(L.exec);
+            L.exec = nullptr;
+            L.exec_variant = -1;
+            // capture must not include the event record issued above
+            MDS_HIP(hipStreamSynchronize(s));
+            hipGraph_t graph = nullptr;
+            MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_steps(false);
+            } catch (...) {
+                hipGraph_t dummy = nullptr;
+                (void)hipStreamEndCapture(s, &dummy);
+                if (dummy) (void)hipGraphDestroy(dummy);
+                throw;
+            }
+            MDS_HIP(hipStreamEndCapture(s, &graph));
+            hipError_t e = hipGraphInstantiate(&L.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            MDS_HIP(e);
+            L.exec_steps = steps;
+            L.exec_variant = o.variant;
+            L.exec_damping = damping;
+        }
+        MDS_HIP(hipGraphLaunch(L.exec, s));
+    } else {
+        enqueue_steps(timing);
+    }
+    MDS_HIP(hipEventRecord(g->ev_end, s));
+    float* final_rank = rk[steps & 1];
+    const float* prev_rank = rk[(steps + 1) & 1];
+    if (steps > 0) {
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, s, L.resid.p);
+        MDS_LAUNCHED();
+        hipLaunchKernelGGL(k_pr_resid, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, (const float*)final_rank,
+                           prev_rank, L.resid.p);
+        MDS_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_pr_unpermute, dim3(grid_for(n, 256)), dim3(256), 0, s, n, L.iperm.p,
+                       (const float*)final_rank, L.unperm.p);
+    MDS_LAUNCHED();
+    MDS_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+    st.device_ms = ms;
+    if (timing) st.kernel_ms = sum_event_pairs(g, steps);
+    st.kernel_launches = steps;
+    // byte model of the main kernel per launch: in-edge offsets, in-edge sources (pull only),
+    // gathered or slot values, out-degree read, rank write, contrib write (non-dangling)
+    const uint64_t main_bytes = 4ull * (n + 1) + (push ? 4ull * m : 8ull * m) + 8ull * n + 4ull * (n - nd);
+    const uint64_t scatter_bytes = push ? (4ull * (n + 1) + 8ull * m + 4ull * n) : 0;
+    st.kernel_bytes = main_bytes * steps;
+    st.alg_bytes = (main_bytes + scatter_bytes + 4ull * nd) * steps;
+    st.edges_traversed = (uint64_t)steps * m;
+    st.supersteps = steps;
+    st.reached = n;
+    if (push) st.push_steps = steps; else st.pull_steps = steps;
+    if (steps > 0) MDS_HIP(hipMemcpy(&st.residual_l1, L.resid.p, sizeof(double), hipMemcpyDeviceToHost));
+    copy_out(g, out, L.unperm.p, (size_t)n * 4);
+}
+
+}  // namespace medusa
+
+using namespace medusa;
+
+extern "C" int medusa_pagerank(medusa_graph* g, float damping, uint32_t steps, float* rank, const medusa_opts* opts,
+                               medusa_stats* stats) {
+    return guard([&] {
+        enter(g);
+        MDS_CHECK(rank, MEDUSA_ERR_ARG, "pagerank: null output");
+        MDS_CHECK(damping >= 0.f && damping < 1.f, MEDUSA_ERR_ARG, "pagerank: damping must be in [0,1)");
+        MDS_CHECK(g->n < (1u << 24), MEDUSA_ERR_ARG, "pagerank: n must be < 2^24 (exact in FP32)");
+        medusa_opts o{};
+        if (opts) o = *opts;
+        medusa_stats st{};
+        pr_run(g, damping, steps, rank, o, st);
+        if (stats) *stats = st;
+    });
+}

@@ -1,0 +1,119 @@
+"""GPU parity for PageRank.
+
+Two bars: the FP32 ranks are bit-identical to the oracle's FP32 restatement of
+the engine's canonical compensated-combine order (so every kernel variant and
+lane mapping is pinned), and within L1 <= 1e-6 of the FP64 oracle.
+"""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import medusa_amd as md
+from medusa_amd import workloads as wl
+from conftest import csr_from_edges, random_graph
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.loads((pathlib.Path(__file__).parent / "golden" / "golden.json").read_text())
+L1_TOL = 1e-6  # FP32 + Kahan vs FP64, summed over all vertices (ranks sum to 1)
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def _check(G, rp, col, oracle, steps=30, variants=(md.PR_PULL, md.PR_PUSH)):
+    want32 = oracle.pagerank_f32(rp, col, 0.85, steps)
+    want64 = oracle.pagerank_f64(rp, col, 0.85, steps)
+    for v in variants:
+        for use_graph in (0, 1):
+            r, st = G.pagerank(0.85, steps, variant=v, use_graph=use_graph)
+            assert np.array_equal(_bits(r), _bits(want32)), (v, use_graph)
+            assert st.supersteps == steps
+    r, _ = G.pagerank(0.85, steps)
+    l1 = np.abs(r.astype(np.float64) - want64).sum()
+    assert l1 <= L1_TOL, l1
+    return r, want64
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_pagerank_golden(oracle, name):
+    g = GOLDEN[name]
+    rp, col, _ = csr_from_edges(g["n"], g["edges"], g["weights"])
+    with md.Graph(rp, col) as G:
+        for steps, want in g["pagerank_engine"].items():
+            r, _ = G.pagerank(0.85, int(steps))
+            np.testing.assert_allclose(r, want, rtol=2e-6)
+            assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, int(steps))))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_pagerank_random(oracle, seed):
+    n = 5000 * (seed + 1)
+    rp, col, _ = random_graph(n, 10 * n, seed)
+    with md.Graph(rp, col) as G:
+        _check(G, rp, col, oracle, steps=20)
+
+
+def test_pagerank_all_lane_classes(oracle):
+    """In-degrees spanning every lane-group class (1..64), the wave class (>64) and the
+    multi-block case (>4096), plus many dangling vertices."""
+    rng = np.random.default_rng(11)
+    n = 30000
+    parts = []
+    for deg in (1, 2, 3, 5, 9, 17, 33, 64, 65, 300, 4096, 4097, 9000):
+        v = rng.integers(0, n)
+        parts.append(np.stack([rng.integers(0, n, deg), np.full(deg, v)], 1))
+    parts.append(np.stack([rng.integers(0, n // 2, 60000), rng.integers(0, n, 60000)], 1))
+    rp, col, _ = csr_from_edges(n, np.concatenate(parts))
+    assert (np.diff(rp.astype(np.int64)) == 0).mean() > 0.3  # dangling-heavy
+    with md.Graph(rp, col) as G:
+        _check(G, rp, col, oracle, steps=15)
+
+
+def test_pagerank_edge_cases(oracle):
+    # no edges: every vertex dangling
+    with md.Graph(np.zeros(6, np.uint32), np.array([], np.uint32)) as G:
+        r, _ = G.pagerank(0.85, 5)
+        np.testing.assert_allclose(r, 0.2, rtol=1e-7)
+    rp, col, _ = csr_from_edges(3, [(0, 1), (1, 2), (2, 0)])
+    with md.Graph(rp, col) as G:
+        r, _ = G.pagerank(0.85, 0)
+        assert np.array_equal(_bits(r), _bits(np.full(3, np.float32(1 / 3))))
+        r, st = G.pagerank(0.85, 7)
+        assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 7)))
+        with pytest.raises(md.MedusaError):
+            G.pagerank(1.5, 3)
+
+
+def test_pagerank_damping_and_repeat(oracle):
+    rp, col, _ = random_graph(8000, 50000, 4)
+    with md.Graph(rp, col) as G:
+        for d in (0.5, 0.85, 0.99):
+            r1, _ = G.pagerank(d, 12, use_graph=1)
+            r2, _ = G.pagerank(d, 12)
+            assert np.array_equal(_bits(r1), _bits(r2))
+            assert np.array_equal(_bits(r1), _bits(oracle.pagerank_f32(rp, col, d, 12)))
+
+
+def test_pagerank_er_config(oracle):
+    rp, col = wl.er_graph()
+    with md.Graph(rp, col) as G:
+        _check(G, rp, col, oracle, steps=30)
+
+
+@pytest.mark.slow
+def test_pagerank_rmat20_config(oracle):
+    rp, col = wl.rmat_graph()
+    with md.Graph(rp, col) as G:
+        r, r64 = _check(G, rp, col, oracle, steps=30, variants=(md.PR_PULL,))
+        assert abs(r.astype(np.float64).sum() - 1.0) < 1e-5
+        # FP64 residual between the last two iterates is reported and small
+        _, st = G.pagerank(0.85, 30)
+        assert 0 < st.residual_l1 < 1e-2

@@ -1,0 +1,211 @@
+"""GPU parity for BFS, SSSP and multi-source BFS: bit-exact against the CPU oracle."""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import medusa_amd as md
+from medusa_amd import workloads as wl
+from conftest import csr_from_edges, random_graph
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.loads((pathlib.Path(__file__).parent / "golden" / "golden.json").read_text())
+U32MAX = 0xFFFFFFFF
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+
+
+def _dirs():
+    return [md.DIR_PUSH, md.DIR_PULL, md.DIR_AUTO]
+
+
+# ---- BFS -------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_bfs_golden(name):
+    g = GOLDEN[name]
+    rp, col, _ = csr_from_edges(g["n"], g["edges"], g["weights"])
+    with md.Graph(rp, col) as G:
+        for s, want in g["bfs"].items():
+            for d in _dirs():
+                lv, st = G.bfs(int(s), direction=d)
+                assert lv.tolist() == want, (s, d)
+                assert st.reached == sum(1 for x in want if x >= 0)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_bfs_random_vs_oracle(oracle, seed):
+    n = 3000 + 500 * seed
+    rp, col, _ = random_graph(n, 8 * n, seed)
+    with md.Graph(rp, col) as G:
+        for src in (0, n - 1, seed * 97 % n):
+            want = oracle.bfs_queue(rp, col, src)
+            for d in _dirs():
+                lv, _ = G.bfs(src, direction=d)
+                assert np.array_equal(lv, want), (src, d)
+
+
+def test_bfs_hubs_use_large_path(oracle):
+    # a few hubs with thousands of out-edges, and a small threshold so the
+    # workgroup-split kernel handles them
+    rng = np.random.default_rng(3)
+    n = 20000
+    src = np.concatenate([rng.integers(0, n, 60000), np.repeat([5, 77, 901], 9000)])
+    dst = rng.integers(0, n, len(src))
+    rp, col, _ = csr_from_edges(n, np.stack([src, dst], 1))
+    with md.Graph(rp, col) as G:
+        for large in (0, 64, 1):
+            for s in (5, 3):
+                lv, _ = G.bfs(s, direction=md.DIR_PUSH, large_degree=large)
+                assert np.array_equal(lv, oracle.bfs_queue(rp, col, s)), (large, s)
+
+
+def test_bfs_edge_cases(oracle):
+    with md.Graph(np.array([0, 0], np.uint32), np.array([], np.uint32)) as G:
+        lv, st = G.bfs(0)
+        assert lv.tolist() == [0] and st.supersteps == 1
+    # isolated source in a larger graph, self-loops and duplicates
+    rp, col, _ = csr_from_edges(6, [(1, 1), (1, 2), (1, 2), (2, 3), (3, 1)])
+    with md.Graph(rp, col) as G:
+        assert G.bfs(0)[0].tolist() == [0, -1, -1, -1, -1, -1]
+        assert G.bfs(1)[0].tolist() == [-1, 0, 1, 2, -1, -1]
+        with pytest.raises(md.MedusaError):
+            G.bfs(6)
+
+
+def test_bfs_er_config(oracle):
+    rp, col = wl.er_graph()
+    want = oracle.bfs_queue(rp, col, 0)
+    sweep, steps = oracle.bfs_sweep(rp, col, 0)
+    assert np.array_equal(want, sweep)
+    with md.Graph(rp, col) as G:
+        for d in _dirs():
+            lv, st = G.bfs(0, direction=d)
+            assert np.array_equal(lv, want), d
+            assert st.supersteps == steps  # one superstep per level plus the empty one
+        out = np.zeros_like(want)
+        deg = np.diff(rp.astype(np.int64))
+        assert G.bfs(0, out=out)[1].edges_traversed == deg[want >= 0].sum()
+
+
+@pytest.mark.slow
+def test_bfs_rmat20(oracle):
+    rp, col = wl.rmat_graph()
+    with md.Graph(rp, col) as G:
+        for s in (0, wl.max_outdeg_vertex(rp), 123457):
+            want = oracle.bfs_queue(rp, col, s)
+            for d in _dirs():
+                lv, _ = G.bfs(s, direction=d)
+                assert np.array_equal(lv, want), (s, d)
+
+
+def test_bfs_device_output_and_repeatability(oracle):
+    torch = pytest.importorskip("torch")
+    rp, col, _ = random_graph(5000, 40000, 9)
+    with md.Graph(rp, col) as G:
+        out = torch.empty(5000, dtype=torch.int32, device="cuda")
+        G.bfs(17, out=out)
+        a = out.cpu().numpy()
+        b, _ = G.bfs(17)
+        assert np.array_equal(a, b)
+        assert np.array_equal(a, oracle.bfs_queue(rp, col, 17))
+
+
+# ---- SSSP ------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_sssp_golden(name):
+    g = GOLDEN[name]
+    rp, col, w = csr_from_edges(g["n"], g["edges"], g["weights"])
+    with md.Graph(rp, col, w) as G:
+        for s, want in g["sssp"].items():
+            want = [U32MAX if x is None else x for x in want]
+            for delta in (0, 1, 3, U32MAX):
+                d, _ = G.sssp(int(s), delta=delta)
+                assert d.tolist() == want, (s, delta)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_sssp_random_vs_dijkstra(oracle, seed):
+    n = 4000
+    rp, col, _ = random_graph(n, 6 * n, seed)
+    w = np.random.default_rng(seed).integers(1, 1001, len(col)).astype(np.uint32)
+    with md.Graph(rp, col, w) as G:
+        for src in (0, 1234):
+            want = oracle.sssp_dijkstra(rp, col, w, src)
+            for delta in (0, 1, 50, 1000, U32MAX):
+                d, _ = G.sssp(src, delta=delta)
+                assert np.array_equal(d, want), (src, delta)
+
+
+def test_sssp_large_weights_and_unweighted_error(oracle):
+    rp, col, w = csr_from_edges(4, [(0, 1), (1, 2), (0, 2), (2, 3)], [0x7FFFFFFF, 0x7FFFFFFF, 0xFFFFFF00, 0xF0])
+    with md.Graph(rp, col, w) as G:
+        d, _ = G.sssp(0)
+        assert d.tolist() == oracle.sssp_dijkstra(rp, col, w, 0).tolist()
+    with md.Graph(rp, col) as G:
+        with pytest.raises(md.MedusaError) as e:
+            G.sssp(0)
+        assert e.value.code == -4
+
+
+def test_sssp_lattice_small(oracle):
+    rp, col, w = wl.lattice_graph(64, 64, seed=3)
+    src = wl.lattice_source(64, 64)
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    assert np.array_equal(want, oracle.sssp_bsp(rp, col, w, src)[0])
+    with md.Graph(rp, col, w) as G:
+        for delta in (0, 1, 500, 5000, U32MAX):
+            d, _ = G.sssp(src, delta=delta)
+            assert np.array_equal(d, want), delta
+
+
+@pytest.mark.slow
+def test_sssp_lattice_config(oracle):
+    rp, col, w = wl.lattice_graph()
+    src = wl.lattice_source()
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        for delta in (0, 2000):
+            d, st = G.sssp(src, delta=delta)
+            assert np.array_equal(d, want), delta
+            deg = np.diff(rp.astype(np.int64))
+            assert st.edges_traversed == deg[want != U32MAX].sum()
+
+
+# ---- multi-source BFS ----------------------------------------------------------------
+
+def test_mbfs_small(oracle):
+    rp, col, _ = random_graph(3000, 15000, 5)
+    with md.Graph(rp, col) as G:
+        for srcs in ([0], [0, 0, 5], list(range(0, 3000, 47))[:64], [2999] * 3):
+            want = oracle.mbfs(rp, col, np.array(srcs, np.uint32))
+            for d in _dirs():
+                out, st = G.mbfs(srcs, direction=d)
+                assert np.array_equal(out, want), (len(srcs), d)
+
+
+def test_mbfs_matches_plain_bfs(oracle):
+    rp, col = wl.er_graph(8192, 65536, seed=1)
+    srcs = wl.mbfs_sources(rp, 64, seed=4)
+    with md.Graph(rp, col) as G:
+        out, st = G.mbfs(srcs)
+        for i, s in enumerate(srcs):
+            lv, _ = G.bfs(int(s))
+            assert np.array_equal(out[i], np.where(lv < 0, 255, lv).astype(np.uint8))
+
+
+@pytest.mark.slow
+def test_mbfs_rmat20_config(oracle):
+    rp, col = wl.rmat_graph()
+    srcs = wl.mbfs_sources(rp, 64, seed=4)
+    want = oracle.mbfs(rp, col, srcs)
+    with md.Graph(rp, col) as G:
+        out, st = G.mbfs(srcs)
+        assert np.array_equal(out, want)
+        deg = np.diff(rp.astype(np.int64))
+        assert st.edges_traversed == sum(deg[want[i] != 255].sum() for i in range(len(srcs)))
