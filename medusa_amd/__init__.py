@@ -85,6 +85,13 @@ def lib():
     if _LIB is None:
         if not _LIBPATH.exists():
             raise MedusaError(-5, f"{_LIBPATH} not built (run medusa_amd/build.py)")
+        # One HIP runtime per process: when PyTorch is installed, load it first so the
+        # library's libamdhip64.so.7 dependency binds to the runtime torch already holds
+        # (two runtimes in one process leave one of them without a GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(str(_LIBPATH))
         sig = {
             "medusa_last_error": ([], ctypes.c_char_p),

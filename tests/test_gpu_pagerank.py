@@ -70,7 +70,7 @@ def test_pagerank_all_lane_classes(oracle):
     for deg in (1, 2, 3, 5, 9, 17, 33, 64, 65, 300, 4096, 4097, 9000):
         v = rng.integers(0, n)
         parts.append(np.stack([rng.integers(0, n, deg), np.full(deg, v)], 1))
-    parts.append(np.stack([rng.integers(0, n // 2, 60000), rng.integers(0, n, 60000)], 1))
+    parts.append(np.stack([rng.integers(0, n // 3, 60000), rng.integers(0, n, 60000)], 1))
     rp, col, _ = csr_from_edges(n, np.concatenate(parts))
     assert (np.diff(rp.astype(np.int64)) == 0).mean() > 0.3  # dangling-heavy
     with md.Graph(rp, col) as G:
