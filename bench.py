@@ -30,7 +30,22 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--scale", type=int, default=20)
+    p.add_argument("--profile", action="store_true",
+                   help="short run for hardware counters: 3 PageRank runs (non-graph launches) and one "
+                        "256 MiB device copy as the byte-count calibration; prints no JSON")
     return p.parse_args()
+
+
+def profile_pass(G, out, torch):
+    """Kernels for a counter pass: the pull kernel 90 times, then a plain copy of known size."""
+    for _ in range(3):
+        G.pagerank(0.85, STEPS_PR, out=out)
+    torch.cuda.synchronize()
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("profile pass done: 90 pull launches, copy of 268435456 bytes", flush=True)
 
 
 def traffic_from_profiles():
@@ -61,6 +76,10 @@ def main():
     n, m = len(rp) - 1, len(col)
     G = md.Graph(rp, col)
     out = torch.empty(n, dtype=torch.float32, device="cuda")
+    if a.profile:
+        profile_pass(G, out, torch)
+        G.close()
+        return
 
     # warmup (also builds the pull layout and the captured superstep graph)
     for _ in range(max(1, a.warmup)):

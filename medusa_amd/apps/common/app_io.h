@@ -1,0 +1,64 @@
+// Graph input and result output for the example applications.
+//
+// Graph file: either a GTgraph text file ("p sp N M" / "a u v w", 1-based, read by
+// GTGraph::get_parition) or a binary CSR file starting with the magic "MDSCSR01":
+//   u32 n, u64 m, u32 has_weight, u32 row_ptr[n+1], u32 col[m], u32 weight[m] (if has_weight).
+// Binary arcs are added in CSR order to the GraphIR.
+#pragma once
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+inline void LoadGraph(const char* path, GraphPartition* gp) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.good()) throw std::runtime_error(std::string("cannot open ") + path);
+    char magic[8] = {0};
+    f.read(magic, 8);
+    if (f.gcount() == 8 && std::memcmp(magic, "MDSCSR01", 8) == 0) {
+        uint32_t n = 0, has_w = 0;
+        uint64_t m = 0;
+        f.read(reinterpret_cast<char*>(&n), 4);
+        f.read(reinterpret_cast<char*>(&m), 8);
+        f.read(reinterpret_cast<char*>(&has_w), 4);
+        std::vector<uint32_t> rp(n + 1), col(m), w(has_w ? m : 0);
+        f.read(reinterpret_cast<char*>(rp.data()), (n + 1) * 4ull);
+        f.read(reinterpret_cast<char*>(col.data()), m * 4ull);
+        if (has_w) f.read(reinterpret_cast<char*>(w.data()), m * 4ull);
+        if (!f) throw std::runtime_error("truncated CSR file");
+        gp[0].graph.resize((int)n);
+        gp[0].l2p.resize(n);
+        for (uint32_t v = 0; v < n; ++v) gp[0].l2p[v] = (int)v;
+        // AddEdge prepends per source, so add each source's arcs in reverse to keep CSR order
+        for (uint32_t u = 0; u < n; ++u)
+            for (uint32_t k = rp[u + 1]; k-- > rp[u];) {
+                Edge e;
+                e.srcVertexID = (int)u;
+                e.dstVertexID = (int)col[k];
+                e.weight = has_w ? (double)w[k] : 0.0;
+                gp[0].graph.AddEdge(e);
+            }
+        return;
+    }
+    GTGraph gt;
+    gt.get_parition(gp, path, nullptr);
+}
+
+template <class T>
+inline void WriteArray(const char* path, const T* data, size_t count) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot write ") + path);
+    std::fwrite(data, sizeof(T), count, f);
+    std::fclose(f);
+}
+
+// The reference's BFS/SSSP root: the first vertex of maximum out-degree (Algorithm/main.cu:209-212).
+inline int MaxOutDegreeVertex(const VertexArray& V) {
+    int best = 0;
+    for (int v = 1; v < V.size; ++v)
+        if (V.edge_count[v] > V.edge_count[best]) best = v;
+    return best;
+}

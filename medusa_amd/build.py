@@ -51,7 +51,31 @@ def _compile(src: pathlib.Path, verbose: bool) -> pathlib.Path:
     return obj
 
 
-def build(verbose: bool = False) -> pathlib.Path:
+APPS = HERE / "apps"
+APP_BIN = APPS / "bin"
+
+
+def _build_app(name: str, verbose: bool) -> pathlib.Path:
+    """Compiles apps/<name>/main.cpp (Medusa's user API, header-only) against the library."""
+    out = APP_BIN / f"{name}_app"
+    cmd = [hipcc(), "-x", "hip", *HIP_FLAGS, *COMMON, str(APPS / name / "main.cpp"), "-o", str(out),
+           f"-L{HERE}", "-lmedusa_amd", "-Wl,-rpath,$ORIGIN/../.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"app build failed: {name}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build_apps(verbose: bool = False):
+    APP_BIN.mkdir(exist_ok=True)
+    names = sorted(d.name for d in APPS.iterdir() if (d / "main.cpp").exists())
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
+        return list(ex.map(lambda n: _build_app(n, verbose), names))
+
+
+def build(verbose: bool = False, apps: bool = True) -> pathlib.Path:
     OBJ.mkdir(exist_ok=True)
     srcs = sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.cpp")])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
@@ -65,6 +89,8 @@ def build(verbose: bool = False) -> pathlib.Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    if apps:
+        build_apps(verbose)
     return LIB
 
 

@@ -97,28 +97,32 @@ struct FrontierCounters {
 
 // One contiguous run of PageRank vertices handled with the same lane mapping.
 struct PrRun {
-    uint32_t begin;     // first internal vertex id
-    uint32_t count;     // vertices in the run
-    uint32_t kind;      // 0..6: groups of 2^kind lanes; kPrWave: wave per vertex; kPrZero: no in-edges
-    uint32_t wg_begin;  // first workgroup of this run in the launch
+    uint32_t begin;       // first internal vertex id
+    uint32_t count;       // vertices in the run
+    uint32_t kind;        // kPrLane / kPrWave / kPrZero
+    uint32_t wg_begin;    // first workgroup of this run in the launch
+    uint32_t deg;         // kPrLane: the in-degree shared by every vertex of the run (1..64)
+    uint32_t ell_stride;  // kPrLane: column stride of the run's ELL slab (count rounded up to 64)
+    uint64_t ell_base;    // kPrLane: offset of the slab in the ELL array
 };
-constexpr uint32_t kPrWave = 7;
-constexpr uint32_t kPrZero = 8;
-constexpr int kPrMaxRuns = 24;
-struct PrRunTable {
-    PrRun r[kPrMaxRuns];
-    uint32_t nruns;
-    uint32_t total_wgs;
-};
+constexpr uint32_t kPrLane = 0;  // one lane per vertex, in-degree 1..64, ELL slab
+constexpr uint32_t kPrWave = 1;  // one wave per vertex, in-degree > 64
+constexpr uint32_t kPrZero = 2;  // no in-edges
+constexpr uint32_t kPrLaneMax = 64;
 
 // In-degree-sorted pull layout for PageRank (built on first use).
 // Internal id order: non-dangling vertices first, then dangling (outdeg 0);
-// within each, in-degree descending, ties by original id.  Every lane-mapping
-// class is therefore one contiguous id range, writes of rank/contrib are
+// within each, in-degree descending, ties by original id.  Every run of equal
+// in-degree (<= 64) is one contiguous id range whose in-edge sources are stored
+// as an ELL slab, column-major and 64-aligned, so the k-th source of 64
+// consecutive vertices is one 256-byte load; writes of rank/contrib are
 // contiguous, and the dangling vertices form the tail [dangle_begin, n).
 struct PrLayout {
     uint32_t dangle_begin = 0;
-    PrRunTable table{};
+    std::vector<PrRun> runs;
+    uint32_t total_wgs = 0;
+    DBuf<PrRun> d_runs;
+    DBuf<uint32_t> ell;     // ELL slabs of the kPrLane runs
     DBuf<uint32_t> perm;    // internal -> original
     DBuf<uint32_t> iperm;   // original -> internal
     DBuf<uint32_t> cptr;    // in-edge offsets in internal order (n+1)

@@ -115,38 +115,52 @@ void build_pr_layout(medusa_graph* g) {
             break;
         }
     L->dangle_begin = dangle_begin;
-    // runs of equal lane mapping
+    // runs: equal in-degree for 1..64 (lane per vertex), >64 (wave per vertex), 0
     auto kind_of = [](uint32_t deg) -> uint32_t {
         if (deg == 0) return kPrZero;
-        if (deg > 64) return kPrWave;
-        uint32_t k = 0;
-        while ((1u << k) < deg) ++k;
-        return k;
+        if (deg > kPrLaneMax) return kPrWave;
+        return kPrLane;
     };
-    auto per_wg = [](uint32_t kind) -> uint32_t {
-        if (kind == kPrZero) return 256;
-        if (kind == kPrWave) return 4;
-        return 4 * (64u >> kind);
-    };
-    PrRunTable& T = L->table;
-    T.nruns = 0;
+    auto per_wg = [](uint32_t kind) -> uint32_t { return kind == kPrWave ? 4u : 256u; };
+    L->runs.clear();
     uint32_t wg = 0;
+    uint64_t ell_size = 0;
     for (uint32_t p = 0; p < n;) {
-        const uint32_t kind = kind_of(cptr[p + 1] - cptr[p]);
+        const uint32_t deg = cptr[p + 1] - cptr[p];
+        const uint32_t kind = kind_of(deg);
         const bool dang = od[p] == 0;
         uint32_t q = p + 1;
-        while (q < n && kind_of(cptr[q + 1] - cptr[q]) == kind && (od[q] == 0) == dang) ++q;
-        MDS_CHECK(T.nruns < (uint32_t)kPrMaxRuns, MEDUSA_ERR_INTERNAL, "pagerank layout: too many runs");
-        PrRun& r = T.r[T.nruns++];
+        while (q < n && kind_of(cptr[q + 1] - cptr[q]) == kind && (od[q] == 0) == dang &&
+               (kind != kPrLane || cptr[q + 1] - cptr[q] == deg))
+            ++q;
+        PrRun r{};
         r.begin = p;
         r.count = q - p;
         r.kind = kind;
         r.wg_begin = wg;
+        if (kind == kPrLane) {
+            r.deg = deg;
+            r.ell_stride = (r.count + 63u) & ~63u;
+            r.ell_base = ell_size;
+            ell_size += (uint64_t)deg * r.ell_stride;
+        }
         wg += (r.count + per_wg(kind) - 1) / per_wg(kind);
+        L->runs.push_back(r);
         p = q;
     }
-    T.total_wgs = wg;
+    L->total_wgs = wg;
+    // ELL slabs: element k of vertex i of a run at ell_base + k*stride + i (padding = 0)
+    std::vector<uint32_t> ell(std::max<uint64_t>(ell_size, 1), 0u);
+    for (const PrRun& r : L->runs) {
+        if (r.kind != kPrLane) continue;
+        for (uint32_t i = 0; i < r.count; ++i) {
+            const uint32_t b = cptr[r.begin + i];
+            for (uint32_t k = 0; k < r.deg; ++k) ell[r.ell_base + (uint64_t)k * r.ell_stride + i] = crow[b + k];
+        }
+    }
     hipStream_t s = g->stream;
+    L->ell.upload(ell.data(), ell.size(), s);
+    L->d_runs.upload(L->runs.data(), L->runs.size(), s);
     L->perm.upload(perm.data(), n, s);
     L->iperm.upload(iperm.data(), n, s);
     L->cptr.upload(cptr.data(), (size_t)n + 1, s);

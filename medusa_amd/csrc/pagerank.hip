@@ -40,7 +40,8 @@ __device__ __forceinline__ void group_tree(float& s, float& c, uint32_t g) {
 }
 
 // Canonical sum of `len` values x(k), computed by one whole wave; every lane
-// returns the same (s - c).
+// returns the same (s - c).  Each lane keeps 16 loads in flight and adds them
+// in ascending order.
 template <class Load>
 __device__ __forceinline__ float wave_segment_sum(uint32_t len, Load load) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -48,17 +49,17 @@ __device__ __forceinline__ float wave_segment_sum(uint32_t len, Load load) {
     for (uint32_t b0 = 0; b0 < len; b0 += kBlock) {
         const uint32_t lim = min(len - b0, kBlock);
         float s = 0.f, c = 0.f;
-        uint32_t k = lane;
-        // four loads in flight per lane, added in ascending order
-        for (; k + 3 * kWave < lim; k += 4 * kWave) {
-            const float x0 = load(b0 + k), x1 = load(b0 + k + kWave), x2 = load(b0 + k + 2 * kWave),
-                        x3 = load(b0 + k + 3 * kWave);
-            kahan_add(s, c, x0);
-            kahan_add(s, c, x1);
-            kahan_add(s, c, x2);
-            kahan_add(s, c, x3);
+        for (uint32_t r0 = 0; r0 * kWave < lim; r0 += 16) {
+            float x[16];
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j) {
+                const uint32_t k = lane + (r0 + j) * kWave;
+                x[j] = k < lim ? load(b0 + k) : 0.f;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j)
+                if (lane + (r0 + j) * kWave < lim) kahan_add(s, c, x[j]);
         }
-        for (; k < lim; k += kWave) kahan_add(s, c, load(b0 + k));
         group_tree(s, c, kWave);
         s = __shfl(s, 0, kWave);
         c = __shfl(c, 0, kWave);
@@ -72,22 +73,83 @@ __device__ __forceinline__ float wave_segment_sum(uint32_t len, Load load) {
     return as - ac;
 }
 
+// 16-leaf canonical subtree in registers: slot j absorbs slot j+off for
+// off = 8, 4, 2, 1.  Absent leaves are (0,0), an exact identity.
+__device__ __forceinline__ void tree16(float (&s)[16], float (&c)[16]) {
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < off; ++j) kahan_combine(s[j], c[j], s[j + off], c[j + off]);
+}
+
+// Leaves r, r+st, ..., r+15*st of a vertex of in-degree `deg` (one lane), reduced
+// by tree16.  Element i of the vertex sits at col[i*stride] (ELL slab, pull) or at
+// vals[seg + i] (message slots, push).
+template <bool kFromMsg>
+__device__ __forceinline__ void lane_subtree(float& S, float& C, uint32_t r, uint32_t st, uint32_t deg,
+                                             const uint32_t* __restrict__ col, uint32_t stride,
+                                             const float* __restrict__ vals, uint32_t seg) {
+    uint32_t idx[16];
+    float s[16], c[16];
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t i = r + st * j;
+        idx[j] = (i < deg) ? (kFromMsg ? seg + i : col[(size_t)i * stride]) : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+        s[j] = (r + st * j < deg) ? vals[idx[j]] : 0.f;
+        c[j] = 0.f;
+    }
+    tree16(s, c);
+    S = s[0];
+    C = c[0];
+}
+
+// Canonical sum of one vertex of in-degree deg <= 64 held by a single lane.
+// The 64-slot tree splits into 1, 2 or 4 interleaved 16-leaf subtrees.
+template <bool kFromMsg>
+__device__ __forceinline__ float lane_vertex_sum(uint32_t deg, const uint32_t* __restrict__ col, uint32_t stride,
+                                                 const float* __restrict__ vals, uint32_t seg) {
+    float s0, c0, s1, c1;
+    if (deg <= 16) {
+        lane_subtree<kFromMsg>(s0, c0, 0, 1, deg, col, stride, vals, seg);
+    } else if (deg <= 32) {
+        lane_subtree<kFromMsg>(s0, c0, 0, 2, deg, col, stride, vals, seg);
+        lane_subtree<kFromMsg>(s1, c1, 1, 2, deg, col, stride, vals, seg);
+        kahan_combine(s0, c0, s1, c1);
+    } else {
+        float s2, c2, s3, c3;
+        lane_subtree<kFromMsg>(s0, c0, 0, 4, deg, col, stride, vals, seg);
+        lane_subtree<kFromMsg>(s2, c2, 2, 4, deg, col, stride, vals, seg);
+        kahan_combine(s0, c0, s2, c2);
+        lane_subtree<kFromMsg>(s1, c1, 1, 4, deg, col, stride, vals, seg);
+        lane_subtree<kFromMsg>(s3, c3, 3, 4, deg, col, stride, vals, seg);
+        kahan_combine(s1, c1, s3, c3);
+        kahan_combine(s0, c0, s1, c1);
+    }
+    return s0 - c0;
+}
+
 // One superstep for every vertex.  kFromMsg: values come from the message
 // array in slot order (push variant) instead of contrib[src] (pull variant).
 template <bool kFromMsg>
-__global__ __launch_bounds__(256) void k_pr_main(PrRunTable tab, const uint32_t* __restrict__ cptr,
+__global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs, uint32_t nruns,
+                                                 const uint32_t* __restrict__ ell, const uint32_t* __restrict__ cptr,
                                                  const uint32_t* __restrict__ crow,
                                                  const float* __restrict__ vals,
                                                  const uint32_t* __restrict__ outdeg,
                                                  const float* __restrict__ base_p, float d,
                                                  float* __restrict__ rank_out, float* __restrict__ contrib_out) {
     const uint32_t wg = blockIdx.x;
-    uint32_t ri = 0;
-    while (ri + 1 < tab.nruns && wg >= tab.r[ri + 1].wg_begin) ++ri;
-    const uint32_t begin = tab.r[ri].begin;
-    const uint32_t end = begin + tab.r[ri].count;
-    const uint32_t kind = tab.r[ri].kind;
-    const uint32_t local = wg - tab.r[ri].wg_begin;
+    uint32_t lo = 0, hi = nruns;  // last run with wg_begin <= wg
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (runs[mid].wg_begin <= wg) lo = mid;
+        else hi = mid;
+    }
+    const PrRun R = runs[lo];
+    const uint32_t local = wg - R.wg_begin;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const float base = *base_p;
 
@@ -98,14 +160,9 @@ __global__ __launch_bounds__(256) void k_pr_main(PrRunTable tab, const uint32_t*
         if (od) contrib_out[p] = r / (float)od;
     };
 
-    if (kind == kPrZero) {
-        const uint32_t p = begin + local * 256 + threadIdx.x;
-        if (p < end) finish(p, 0.f);
-        return;
-    }
-    if (kind == kPrWave) {
-        const uint32_t p = begin + local * 4 + wave;
-        if (p >= end) return;  // uniform across the wave
+    if (R.kind == kPrWave) {
+        const uint32_t p = R.begin + local * 4 + wave;
+        if (p >= R.begin + R.count) return;  // uniform across the wave
         const uint32_t b = cptr[p], len = cptr[p + 1] - b;
         const float S = wave_segment_sum(len, [&](uint32_t k) {
             return kFromMsg ? vals[b + k] : vals[crow[b + k]];
@@ -113,18 +170,15 @@ __global__ __launch_bounds__(256) void k_pr_main(PrRunTable tab, const uint32_t*
         if (lane == 0) finish(p, S);
         return;
     }
-    // groups of g = 2^kind lanes, one vertex per group, in-degree in (g/2, g]
-    const uint32_t g = 1u << kind;
-    const uint32_t p = begin + (local * 4 + wave) * (kWave >> kind) + (lane >> kind);
-    const uint32_t j = lane & (g - 1);
-    const bool valid = p < end;
-    float s = 0.f, c = 0.f;
-    if (valid) {
-        const uint32_t b = cptr[p], len = cptr[p + 1] - b;
-        if (j < len) s = kFromMsg ? vals[b + j] : vals[crow[b + j]];
+    const uint32_t i = local * 256 + threadIdx.x;
+    if (i >= R.count) return;
+    const uint32_t p = R.begin + i;
+    if (R.kind == kPrZero) {
+        finish(p, 0.f);
+        return;
     }
-    group_tree(s, c, g);
-    if (valid && j == 0) finish(p, s - c);
+    const uint32_t seg = kFromMsg ? cptr[p] : 0u;
+    finish(p, lane_vertex_sum<kFromMsg>(R.deg, ell + R.ell_base + i, R.ell_stride, vals, seg));
 }
 
 // Dangling mass D = canonical sum of rank[dbegin + i], i < nd; the last
@@ -140,7 +194,17 @@ __global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ ran
         const uint32_t lo = b * kBlock, lim = min(nd - lo, kBlock);
         const float* x = rank + dbegin + lo;
         float s = 0.f, c = 0.f;
-        for (uint32_t k = lane; k < lim; k += kWave) kahan_add(s, c, x[k]);
+        for (uint32_t r0 = 0; r0 * kWave < lim; r0 += 16) {
+            float v[16];
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j) {
+                const uint32_t k = lane + (r0 + j) * kWave;
+                v[j] = k < lim ? x[k] : 0.f;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j)
+                if (lane + (r0 + j) * kWave < lim) kahan_add(s, c, v[j]);
+        }
         group_tree(s, c, kWave);
         if (lane == 0) {
             __hip_atomic_store(&partials[2 * b], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -201,12 +265,15 @@ __global__ void k_pr_unpermute(uint32_t n, const uint32_t* __restrict__ iperm, c
 
 __global__ void k_pr_resid(uint32_t n, const float* __restrict__ a, const float* __restrict__ b,
                            double* __restrict__ out) {
+    __shared__ double s_part[4];
     double acc = 0.0;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
         acc += fabs((double)a[p] - (double)b[p]);
 #pragma unroll
     for (int off = kWave / 2; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(out, acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, s_part[0] + s_part[1] + s_part[2] + s_part[3]);
 }
 
 __global__ void k_zero_f64(double* p) { *p = 0.0; }
@@ -268,12 +335,13 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
             }
             if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t], s));
             if (push)
-                hipLaunchKernelGGL(k_pr_main<true>, dim3(L.table.total_wgs), dim3(256), 0, s, L.table, L.cptr.p,
-                                   L.crow.p, (const float*)L.msg.p, L.outdeg.p, (const float*)L.base.p, d_f, r_out,
-                                   c_out);
+                hipLaunchKernelGGL(k_pr_main<true>, dim3(L.total_wgs), dim3(256), 0, s, (const PrRun*)L.d_runs.p,
+                                   (uint32_t)L.runs.size(), (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p,
+                                   (const float*)L.msg.p, L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             else
-                hipLaunchKernelGGL(k_pr_main<false>, dim3(L.table.total_wgs), dim3(256), 0, s, L.table, L.cptr.p,
-                                   L.crow.p, c_in, L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
+                hipLaunchKernelGGL(k_pr_main<false>, dim3(L.total_wgs), dim3(256), 0, s, (const PrRun*)L.d_runs.p,
+                                   (uint32_t)L.runs.size(), (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p, c_in,
+                                   L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             MDS_LAUNCHED();
             if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t + 1], s));
             if (t + 1 < steps) {
@@ -321,7 +389,7 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     if (steps > 0) {
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, s, L.resid.p);
         MDS_LAUNCHED();
-        hipLaunchKernelGGL(k_pr_resid, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, n, (const float*)final_rank,
+        hipLaunchKernelGGL(k_pr_resid, dim3(grid_for(n, 256, 512)), dim3(256), 0, s, n, (const float*)final_rank,
                            prev_rank, L.resid.p);
         MDS_LAUNCHED();
     }
@@ -334,9 +402,15 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     st.device_ms = ms;
     if (timing) st.kernel_ms = sum_event_pairs(g, steps);
     st.kernel_launches = steps;
-    // byte model of the main kernel per launch: in-edge offsets, in-edge sources (pull only),
-    // gathered or slot values, out-degree read, rank write, contrib write (non-dangling)
-    const uint64_t main_bytes = 4ull * (n + 1) + (push ? 4ull * m : 8ull * m) + 8ull * n + 4ull * (n - nd);
+    // byte model of the main kernel per launch: in-edge sources (pull only; ELL slab or CSC),
+    // gathered contribution or message slot per in-edge, offsets of the vertices that read
+    // them (wave runs; every vertex in the push variant), out-degree read, rank write,
+    // contrib write (non-dangling vertices)
+    uint64_t wave_vertices = 0;
+    for (const PrRun& r : L.runs)
+        if (r.kind == kPrWave) wave_vertices += r.count;
+    const uint64_t main_bytes = (push ? 4ull * m + 8ull * n : 8ull * m + 8ull * wave_vertices) + 8ull * n +
+                                4ull * (n - nd);
     const uint64_t scatter_bytes = push ? (4ull * (n + 1) + 8ull * m + 4ull * n) : 0;
     st.kernel_bytes = main_bytes * steps;
     st.alg_bytes = (main_bytes + scatter_bytes + 4ull * nd) * steps;

@@ -1,0 +1,99 @@
+"""The example applications written against Medusa's user API (include/medusa/medusa.h):
+BFS, SSSP, PageRank (reference formula) and multi-source BFS, run as separate
+processes on the GPU and checked against the CPU oracle."""
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import random_graph
+
+pytestmark = pytest.mark.gpu
+BIN = pathlib.Path(__file__).resolve().parent.parent / "medusa_amd" / "apps" / "bin"
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+
+
+def write_csr(path, rp, col, w=None):
+    with open(path, "wb") as f:
+        f.write(b"MDSCSR01")
+        f.write(np.uint32(len(rp) - 1).tobytes())
+        f.write(np.uint64(len(col)).tobytes())
+        f.write(np.uint32(0 if w is None else 1).tobytes())
+        f.write(np.asarray(rp, np.uint32).tobytes())
+        f.write(np.asarray(col, np.uint32).tobytes())
+        if w is not None:
+            f.write(np.asarray(w, np.uint32).tobytes())
+
+
+def write_gtgraph(path, rp, col, w):
+    n = len(rp) - 1
+    with open(path, "w") as f:
+        f.write("c generated\n")
+        f.write(f"p sp {n} {len(col)}\n")
+        for u in range(n):
+            for e in range(rp[u], rp[u + 1]):
+                f.write(f"a {u + 1} {col[e] + 1} {w[e]}\n")
+
+
+def run(app, *args):
+    r = subprocess.run([str(BIN / app), *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_bfs_app(tmp_path, oracle):
+    rp, col, _ = random_graph(5000, 30000, 1)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col)
+    for root in (0, 4321):
+        out = tmp_path / "lv.bin"
+        run("bfs_app", g, out, root)
+        assert np.array_equal(np.fromfile(out, np.int32), oracle.bfs_queue(rp, col, root))
+    # default root: first vertex of maximum out-degree
+    run("bfs_app", g, out)
+    root = int(np.argmax(np.diff(rp.astype(np.int64))))
+    assert np.array_equal(np.fromfile(out, np.int32), oracle.bfs_queue(rp, col, root))
+
+
+def test_sssp_app_csr_and_gtgraph(tmp_path, oracle):
+    rp, col, _ = random_graph(3000, 15000, 2)
+    w = np.random.default_rng(2).integers(1, 1001, len(col)).astype(np.uint32)
+    want = oracle.sssp_dijkstra(rp, col, w, 7)
+    wantf = np.where(want == 0xFFFFFFFF, np.float32(np.finfo(np.float32).max), want.astype(np.float32))
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col, w)
+    out = tmp_path / "d.bin"
+    run("sssp_app", g, out, 7)
+    assert np.array_equal(np.fromfile(out, np.float32), wantf)
+    gt = tmp_path / "g.gr"
+    write_gtgraph(gt, rp, col, w)
+    run("sssp_app", gt, out, 7)
+    assert np.array_equal(np.fromfile(out, np.float32), wantf)
+
+
+def test_pagerank_app_reference_formula(tmp_path, oracle):
+    rp, col, _ = random_graph(4000, 40000, 3)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col)
+    out = tmp_path / "r.bin"
+    run("pagerank_app", g, out, 100)
+    got = np.fromfile(out, np.float32)
+    want = oracle.pagerank_ref(rp, col, 100)
+    # same formula; the combiner's summation order differs from the sequential CPU sum
+    np.testing.assert_allclose(got, want, rtol=2e-5)
+
+
+def test_mbfs_app(tmp_path, oracle):
+    rp, col, _ = random_graph(4000, 20000, 4)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col)
+    srcs = [0, 17, 17, 3999, 250]
+    out = tmp_path / "m.bin"
+    run("mbfs_app", g, out, *srcs)
+    got = np.fromfile(out, np.uint8).reshape(len(srcs), -1)
+    assert np.array_equal(got, oracle.mbfs(rp, col, np.array(srcs, np.uint32)))
