@@ -148,6 +148,31 @@ int medusa_mbfs(medusa_graph* g, const uint32_t* sources, uint32_t k, uint8_t* l
 int medusa_trace_count(const medusa_graph* g, uint32_t* count);
 int medusa_trace_read(const medusa_graph* g, double* out, uint32_t capacity);
 
+/* ---- learned scheduler ------------------------------------------------------
+ * A small MLP cost model (16 inputs -> 32 -> 32 -> 1, leaky ReLU) trained on the GPU by the
+ * library's own forward/backward kernels.  Input row layout (MEDUSA_POLICY_INPUTS floats) is
+ * produced by medusa_policy_features(); the engine evaluates every candidate action at each
+ * superstep and takes the lowest predicted cost.  Results never depend on the choice.
+ * No reference counterpart. */
+#define MEDUSA_POLICY_INPUTS 16
+#define MEDUSA_POLICY_PARAMS 1633
+#define MEDUSA_POLICY_DIRECTION 1 /* BFS / MBFS: push or pull per superstep */
+#define MEDUSA_POLICY_DELTA 2     /* SSSP: bucket width at each threshold advance */
+int medusa_policy_create(uint32_t kind, uint64_t seed, medusa_policy** out);
+int medusa_policy_destroy(medusa_policy* p);
+/* Reads (get != NULL) and/or replaces (set != NULL) the MEDUSA_POLICY_PARAMS parameters. */
+int medusa_policy_params(medusa_policy* p, float* get, const float* set, uint32_t count);
+/* Full-batch Adam on mean squared error over n rows X[n][16] -> y[n]. */
+int medusa_policy_train(medusa_policy* p, const float* X, const float* y, uint32_t n, uint32_t iters, float lr,
+                        float* final_loss);
+/* GPU forward pass over n rows. */
+int medusa_policy_predict(medusa_policy* p, const float* X, uint32_t n, float* out);
+/* GPU gradient of the mean squared error at the current parameters (and the loss). */
+int medusa_policy_gradients(medusa_policy* p, const float* X, const float* y, uint32_t n, float* grad, float* loss);
+/* Feature row for one trace record (MEDUSA_TRACE_FIELDS doubles) and a candidate action:
+ * DIRECTION kind: action 0 = push, 1 = pull; DELTA kind: action = bucket width. */
+int medusa_policy_features(uint32_t kind, const double* trace_record, double action, float* row);
+
 /* ---- synthetic graph generators (host) -----------------------------------
  * Deterministic from the seed (xoshiro256** streams seeded by splitmix64), independent of
  * thread count.  Arcs are written as (src[i], dst[i], weight[i]); weight may be NULL.

@@ -31,7 +31,12 @@ EXPORTED = [
     "medusa_graph_info", "medusa_graph_device_arrays", "medusa_bfs", "medusa_sssp", "medusa_pagerank",
     "medusa_mbfs", "medusa_trace_count", "medusa_trace_read", "medusa_gen_erdos_renyi", "medusa_gen_rmat",
     "medusa_gen_lattice", "medusa_gen_chung_lu", "medusa_gen_weights", "medusa_csr_from_arcs",
+    "medusa_policy_create", "medusa_policy_destroy", "medusa_policy_params", "medusa_policy_train",
+    "medusa_policy_predict", "medusa_policy_gradients", "medusa_policy_features",
 ]
+POLICY_INPUTS = 16
+POLICY_PARAMS = 1633
+POLICY_DIRECTION, POLICY_DELTA = 1, 2
 
 
 class MedusaError(RuntimeError):
@@ -117,6 +122,13 @@ def lib():
                                     ctypes.c_int),
             "medusa_gen_weights": ([_u64, _u32, _u32, _u64, _vp], ctypes.c_int),
             "medusa_csr_from_arcs": ([_u32, _u64, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+            "medusa_policy_create": ([_u32, _u64, ctypes.POINTER(_vp)], ctypes.c_int),
+            "medusa_policy_destroy": ([_vp], ctypes.c_int),
+            "medusa_policy_params": ([_vp, _vp, _vp, _u32], ctypes.c_int),
+            "medusa_policy_train": ([_vp, _vp, _vp, _u32, _u32, _f32, ctypes.POINTER(_f32)], ctypes.c_int),
+            "medusa_policy_predict": ([_vp, _vp, _u32, _vp], ctypes.c_int),
+            "medusa_policy_gradients": ([_vp, _vp, _vp, _u32, _vp, ctypes.POINTER(_f32)], ctypes.c_int),
+            "medusa_policy_features": ([_u32, _vp, _f64, _vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -303,3 +315,64 @@ class Graph:
         out = np.zeros((c.value, TRACE_FIELDS), np.float64)
         _check(lib().medusa_trace_read(self.handle, _ptr(out), c.value))
         return out
+
+
+def policy_features(kind, record, action):
+    """Feature row (float32[16]) of one trace record for a candidate action (host only)."""
+    rec = np.ascontiguousarray(record, np.float64)
+    row = np.zeros(POLICY_INPUTS, np.float32)
+    _check(lib().medusa_policy_features(kind, _ptr(rec), float(action), _ptr(row)))
+    return row
+
+
+class Policy:
+    """Learned per-superstep scheduler (MLP cost model trained on the GPU)."""
+
+    def __init__(self, kind, seed=0):
+        h = _vp()
+        _check(lib().medusa_policy_create(kind, seed, ctypes.byref(h)))
+        self.handle = h
+        self.kind = kind
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _check(lib().medusa_policy_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        out = np.zeros(POLICY_PARAMS, np.float32)
+        _check(lib().medusa_policy_params(self.handle, _ptr(out), None, POLICY_PARAMS))
+        return out
+
+    @params.setter
+    def params(self, value):
+        v = np.ascontiguousarray(value, np.float32)
+        _check(lib().medusa_policy_params(self.handle, None, _ptr(v), POLICY_PARAMS))
+
+    def train(self, X, y, iters=2000, lr=3e-3):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        loss = _f32(0)
+        _check(lib().medusa_policy_train(self.handle, _ptr(X), _ptr(y), len(y), iters, lr, ctypes.byref(loss)))
+        return loss.value
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(len(X), np.float32)
+        _check(lib().medusa_policy_predict(self.handle, _ptr(X), len(X), _ptr(out)))
+        return out
+
+    def gradients(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(POLICY_PARAMS, np.float32)
+        loss = _f32(0)
+        _check(lib().medusa_policy_gradients(self.handle, _ptr(X), _ptr(y), len(y), _ptr(g), ctypes.byref(loss)))
+        return g, loss.value

@@ -19,6 +19,15 @@
 //          atomic per workgroup per queue) and counts the frontier's out-edges.
 // The claim `if (level[v] < 0) level[v] = step+1` is a plain store: every
 // writer stores the same value, so the race is benign (as in the reference).
+//
+// Two drivers run the same kernels:
+//   device-driven (default): the last workgroup of each compaction decides the next
+//     superstep (direction by the heuristic or the learned policy, queue sizes, the
+//     termination vote) into a control block in HBM; kBfsBatch supersteps of
+//     [expand, compact+decide] are captured once as a hipGraph and replayed until the
+//     control block says done, so the host waits once per batch, not per superstep;
+//   host-driven (timing or trace requested): the host reads the counters after every
+//     superstep, times the expansion with hipEvents and logs the trace record.
 
 #include <algorithm>
 
@@ -29,6 +38,22 @@ namespace medusa {
 
 constexpr uint32_t kLargeDefault = 2048;
 constexpr uint32_t kLargeSplit = 16;  // workgroups per hub in the large-vertex push
+constexpr uint32_t kBfsBatch = 8;     // supersteps per captured graph
+constexpr uint32_t kExpandGrid = 2048;
+constexpr uint32_t kLargeGroups = 256;  // workgroups of the expand kernel given to hubs when pushing
+
+// Control block of the device-driven loop.
+struct BfsCtl {
+    int32_t step;       // superstep being run (= level of the frontier)
+    uint32_t pull;      // direction of this superstep
+    uint32_t done;      // the vote: no vertex joined the frontier
+    uint32_t n_small, n_large;
+    uint32_t supersteps, push_steps, pull_steps;
+    uint32_t mode;      // 0 heuristic, 1 push, 2 pull, 3 learned policy
+    uint32_t spare;
+    uint32_t pad[2];
+    unsigned long long n, m, n_f, m_f, n_u, m_u, reached, reached_edges, kernel_bytes, alg_bytes;
+};
 
 __global__ void k_bfs_init(uint32_t n, int32_t* __restrict__ level, uint32_t src, uint32_t* __restrict__ q_small,
                            uint32_t* __restrict__ q_large, uint32_t src_class, unsigned long long* __restrict__ bitmap,
@@ -51,15 +76,35 @@ __global__ void k_zero_counters(FrontierCounters* c) {
     c->probes = 0;
 }
 
-// Wave-cooperative push over a queue of vertices with 1 <= outdeg <= large.
-__global__ __launch_bounds__(256) void k_bfs_push(const uint32_t* __restrict__ q, uint32_t qn,
-                                                  const uint32_t* __restrict__ row_ptr,
-                                                  const uint32_t* __restrict__ col, int32_t* __restrict__ level,
-                                                  int32_t next) {
-    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
+// Counters and control block as if a compaction had produced the frontier {src}.
+__global__ void k_bfs_seed_ctl(BfsCtl* ctl, FrontierCounters* c, uint32_t n, unsigned long long m, uint32_t deg_src,
+                               uint32_t src_class, uint32_t mode) {
+    c->n_small = src_class == 1;
+    c->n_large = src_class == 2;
+    c->n_new = 1;
+    c->pad = 0;
+    c->deg_sum = deg_src;
+    c->probes = 0;
+    BfsCtl z{};
+    z.step = -1;
+    z.mode = mode;
+    z.n = n;
+    z.m = m;
+    z.n_u = n;
+    z.m_u = m;
+    *ctl = z;
+}
+
+// ---- superstep bodies (shared by both drivers) -----------------------------------
+
+// Wave-cooperative push over a queue of vertices with 1 <= outdeg <= large; `wave0`
+// and `nwaves` select which waves of the launch take part.
+__device__ __forceinline__ void push_small(const uint32_t* __restrict__ q, uint32_t qn,
+                                           const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                           int32_t* __restrict__ level, int32_t next, uint32_t wave0, uint32_t nwaves,
+                                           uint32_t (*s_beg)[kWave], uint32_t (*s_off)[kWave]) {
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const uint32_t total_waves = gridDim.x * 4;
-    for (uint32_t base = (blockIdx.x * 4 + wv) * kWave; base < qn; base += total_waves * kWave) {
+    for (uint32_t base = wave0 * kWave; base < qn; base += nwaves * kWave) {
         const uint32_t i = base + lane;
         uint32_t beg = 0, deg = 0;
         if (i < qn) {
@@ -85,13 +130,11 @@ __global__ __launch_bounds__(256) void k_bfs_push(const uint32_t* __restrict__ q
     }
 }
 
-// Hubs: vertex i of the queue is split into kLargeSplit contiguous edge ranges,
-// one workgroup each.
-__global__ __launch_bounds__(256) void k_bfs_push_large(const uint32_t* __restrict__ q, uint32_t qn,
-                                                        const uint32_t* __restrict__ row_ptr,
-                                                        const uint32_t* __restrict__ col,
-                                                        int32_t* __restrict__ level, int32_t next) {
-    for (uint32_t w = blockIdx.x; w < qn * kLargeSplit; w += gridDim.x) {
+// Hubs: vertex i of the queue is split into kLargeSplit contiguous edge ranges.
+__device__ __forceinline__ void push_large(const uint32_t* __restrict__ q, uint32_t qn,
+                                           const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                           int32_t* __restrict__ level, int32_t next, uint32_t wg0, uint32_t nwg) {
+    for (uint32_t w = wg0; w < qn * kLargeSplit; w += nwg) {
         const uint32_t u = q[w / kLargeSplit], part = w % kLargeSplit;
         const uint32_t b = row_ptr[u], deg = row_ptr[u + 1] - b;
         const uint32_t per = (deg + kLargeSplit - 1) / kLargeSplit;
@@ -104,11 +147,10 @@ __global__ __launch_bounds__(256) void k_bfs_push_large(const uint32_t* __restri
 }
 
 // Bottom-up: an unvisited vertex joins if any in-neighbour is in the frontier bitmap.
-__global__ __launch_bounds__(256) void k_bfs_pull(uint32_t n, const uint32_t* __restrict__ cptr,
-                                                  const uint32_t* __restrict__ csrc,
-                                                  const unsigned long long* __restrict__ bitmap,
-                                                  int32_t* __restrict__ level, int32_t next,
-                                                  FrontierCounters* __restrict__ ctr) {
+__device__ __forceinline__ void pull_all(uint32_t n, const uint32_t* __restrict__ cptr,
+                                         const uint32_t* __restrict__ csrc,
+                                         const unsigned long long* __restrict__ bitmap, int32_t* __restrict__ level,
+                                         int32_t next, FrontierCounters* __restrict__ ctr) {
     unsigned long long probes = 0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         if (level[v] >= 0) continue;
@@ -126,16 +168,16 @@ __global__ __launch_bounds__(256) void k_bfs_pull(uint32_t n, const uint32_t* __
     if ((threadIdx.x & (kWave - 1)) == 0 && probes) atomicAdd(&ctr->probes, probes);
 }
 
-constexpr uint32_t kCompactK = 8;                          // 64-vertex chunks per wave per tile
-constexpr uint32_t kCompactTile = 4 * kCompactK * kWave;   // vertices per workgroup tile
+constexpr uint32_t kCompactK = 8;                         // 64-vertex chunks per wave per tile
+constexpr uint32_t kCompactTile = 4 * kCompactK * kWave;  // vertices per workgroup tile
 
 // Next frontier = {v : level[v] == want}: bitmap word per 64 vertices, queues by
 // out-degree class, and frontier counters.
-__global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
-                                                     const uint32_t* __restrict__ row_ptr, uint32_t large,
-                                                     uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
-                                                     unsigned long long* __restrict__ bitmap, uint32_t nwords,
-                                                     FrontierCounters* __restrict__ ctr) {
+__device__ __forceinline__ void compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
+                                        const uint32_t* __restrict__ row_ptr, uint32_t large,
+                                        uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
+                                        unsigned long long* __restrict__ bitmap, uint32_t nwords,
+                                        FrontierCounters* __restrict__ ctr) {
     __shared__ uint32_t s_cnt[4][3];
     __shared__ unsigned long long s_deg[4];
     __shared__ uint32_t s_base[2];
@@ -200,19 +242,299 @@ __global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* 
     }
 }
 
-static void bfs_run(medusa_graph* g, uint32_t src, int32_t* out, const medusa_opts& o, medusa_stats& st) {
+// ---- host-driven kernels ------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_bfs_push(const uint32_t* __restrict__ q, uint32_t qn,
+                                                  const uint32_t* __restrict__ row_ptr,
+                                                  const uint32_t* __restrict__ col, int32_t* __restrict__ level,
+                                                  int32_t next) {
+    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
+    push_small(q, qn, row_ptr, col, level, next, blockIdx.x * 4 + threadIdx.x / kWave, gridDim.x * 4, s_beg, s_off);
+}
+
+__global__ __launch_bounds__(256) void k_bfs_push_large(const uint32_t* __restrict__ q, uint32_t qn,
+                                                        const uint32_t* __restrict__ row_ptr,
+                                                        const uint32_t* __restrict__ col,
+                                                        int32_t* __restrict__ level, int32_t next) {
+    push_large(q, qn, row_ptr, col, level, next, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void k_bfs_pull(uint32_t n, const uint32_t* __restrict__ cptr,
+                                                  const uint32_t* __restrict__ csrc,
+                                                  const unsigned long long* __restrict__ bitmap,
+                                                  int32_t* __restrict__ level, int32_t next,
+                                                  FrontierCounters* __restrict__ ctr) {
+    pull_all(n, cptr, csrc, bitmap, level, next, ctr);
+}
+
+__global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
+                                                     const uint32_t* __restrict__ row_ptr, uint32_t large,
+                                                     uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
+                                                     unsigned long long* __restrict__ bitmap, uint32_t nwords,
+                                                     FrontierCounters* __restrict__ ctr) {
+    compact(n, level, want, row_ptr, large, q_small, q_large, bitmap, nwords, ctr);
+}
+
+// ---- device-driven kernels -----------------------------------------------------------
+
+// Expansion of the current superstep: pull over every vertex, or push where the first
+// kLargeGroups workgroups take the hub queue and the rest the wave-cooperative queue.
+__global__ __launch_bounds__(256) void k_bfs_expand_dev(const BfsCtl* __restrict__ ctl,
+                                                        const uint32_t* __restrict__ q_small,
+                                                        const uint32_t* __restrict__ q_large,
+                                                        const uint32_t* __restrict__ row_ptr,
+                                                        const uint32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ cptr,
+                                                        const uint32_t* __restrict__ csrc,
+                                                        const unsigned long long* __restrict__ bitmap,
+                                                        int32_t* __restrict__ level, uint32_t n,
+                                                        FrontierCounters* __restrict__ ctr) {
+    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
+    if (ctl->done) return;
+    const int32_t next = ctl->step + 1;
+    if (ctl->pull) {
+        pull_all(n, cptr, csrc, bitmap, level, next, ctr);
+        return;
+    }
+    if (blockIdx.x < kLargeGroups) {
+        const uint32_t nl = ctl->n_large;
+        if (nl) push_large(q_large, nl, row_ptr, col, level, next, blockIdx.x, kLargeGroups);
+        return;
+    }
+    const uint32_t wg = blockIdx.x - kLargeGroups, nwg = gridDim.x - kLargeGroups;
+    push_small(q_small, ctl->n_small, row_ptr, col, level, next, wg * 4 + threadIdx.x / kWave, nwg * 4, s_beg, s_off);
+}
+
+// Compaction of the current superstep (device-driven).
+__global__ __launch_bounds__(256) void k_bfs_compact_dev(const BfsCtl* __restrict__ ctl, uint32_t n,
+                                                         const int32_t* __restrict__ level,
+                                                         const uint32_t* __restrict__ row_ptr, uint32_t large,
+                                                         uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
+                                                         unsigned long long* __restrict__ bitmap, uint32_t nwords,
+                                                         FrontierCounters* __restrict__ ctr) {
+    if (ctl->done) return;
+    compact(n, level, ctl->step + 1, row_ptr, large, q_small, q_large, bitmap, nwords, ctr);
+}
+
+// One workgroup after the compaction: its counters are visible at the kernel boundary,
+// so the decision needs no fence.  Decides the next superstep into the control block.
+__global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, uint32_t n, uint32_t nwords,
+                                                    FrontierCounters* __restrict__ ctr,
+                                                    const float* __restrict__ policy) {
+    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
+    __shared__ float s_cost[2];
+    if (ctl->done) return;
+    const uint32_t n_new = ctr->n_new;
+    const uint32_t n_small = ctr->n_small;
+    const uint32_t n_large = ctr->n_large;
+    const unsigned long long deg_sum = ctr->deg_sum;
+    const unsigned long long probes = ctr->probes;
+    BfsCtl c = *ctl;
+    // byte model of the superstep just finished (same as the host-driven driver)
+    const unsigned long long kb = c.pull ? 4ull * n + 4ull * (n + 1) + 8ull * probes + 4ull * n_new
+                                         : 12ull * c.n_f + 8ull * c.m_f + 8ull * n_new;
+    c.kernel_bytes += kb;
+    c.alg_bytes += kb + 4ull * n + nwords * 8ull + 12ull * n_new;
+    if (n_new == 0) {
+        c.done = 1;
+    } else {
+        c.step += 1;
+        c.n_f = n_new;
+        c.m_f = deg_sum;
+        c.n_u -= n_new;
+        c.m_u -= deg_sum;
+        c.reached += n_new;
+        c.reached_edges += deg_sum;
+        c.n_small = n_small;
+        c.n_large = n_large;
+    }
+    bool pull = c.pull != 0;
+    if (!c.done) {
+        if (c.mode == 1) {
+            pull = false;
+        } else if (c.mode == 2) {
+            pull = true;
+        } else if (c.mode == 3 && policy) {
+            const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
+                                                     (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
+            for (int a = 0; a < 2; ++a) {
+                if (threadIdx.x == 0) {
+                    float x[MEDUSA_POLICY_INPUTS];
+                    direction_row(rec, (double)a, x);
+                    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
+                }
+                __syncthreads();
+                const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
+                if (threadIdx.x == 0) s_cost[a] = cost;
+                __syncthreads();
+            }
+            pull = s_cost[1] < s_cost[0];
+        } else {
+            // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
+            if (!pull)
+                pull = (double)c.m_f > (double)c.m_u / Sched::kAlpha;
+            else
+                pull = !((double)c.n_f < (double)c.n / Sched::kBeta);
+        }
+        c.pull = pull ? 1u : 0u;
+        c.supersteps += 1;
+        if (pull)
+            c.pull_steps += 1;
+        else
+            c.push_steps += 1;
+    }
+    __syncthreads();  // every wave has read the control block and counters
+    if (threadIdx.x == 0) {
+        *ctl = c;
+        ctr->n_small = 0;
+        ctr->n_large = 0;
+        ctr->n_new = 0;
+        ctr->deg_sum = 0;
+        ctr->probes = 0;
+    }
+}
+
+// First decision (superstep 0) from the seeded counters; one workgroup.
+__global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ ctl, FrontierCounters* __restrict__ ctr,
+                                                          const float* __restrict__ policy) {
+    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
+    __shared__ float s_cost[2];
+    BfsCtl c = *ctl;
+    c.step = 0;
+    c.n_f = ctr->n_new;
+    c.m_f = ctr->deg_sum;
+    c.n_u -= c.n_f;
+    c.m_u -= c.m_f;
+    c.reached = c.n_f;
+    c.reached_edges = c.m_f;
+    c.n_small = ctr->n_small;
+    c.n_large = ctr->n_large;
+    bool pull = false;
+    if (c.mode == 2) {
+        pull = true;
+    } else if (c.mode == 3 && policy) {
+        const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
+                                                 (double)c.n, (double)c.m, 0.0, 0.0};
+        for (int a = 0; a < 2; ++a) {
+            if (threadIdx.x == 0) {
+                float x[MEDUSA_POLICY_INPUTS];
+                direction_row(rec, (double)a, x);
+                for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
+            }
+            __syncthreads();
+            const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
+            if (threadIdx.x == 0) s_cost[a] = cost;
+            __syncthreads();
+        }
+        pull = s_cost[1] < s_cost[0];
+    } else if (c.mode == 0) {
+        pull = (double)c.m_f > (double)c.m_u / Sched::kAlpha;
+    }
+    c.pull = pull ? 1u : 0u;
+    c.supersteps = 1;
+    c.push_steps = pull ? 0 : 1;
+    c.pull_steps = pull ? 1 : 0;
+    __syncthreads();  // every wave has read the control block and counters
+    if (threadIdx.x == 0) {
+        *ctl = c;
+        ctr->n_small = 0;
+        ctr->n_large = 0;
+        ctr->n_new = 0;
+        ctr->deg_sum = 0;
+        ctr->probes = 0;
+    }
+}
+
+static uint32_t bfs_mode(const medusa_opts& o) {
+    if (o.direction == MEDUSA_DIR_PUSH) return 1;
+    if (o.direction == MEDUSA_DIR_PULL) return 2;
+    if (o.policy && policy_kind(o.policy) == MEDUSA_POLICY_DIRECTION) return 3;
+    return 0;
+}
+
+// Device-driven run: seed, first decision, then replays of a captured batch of supersteps.
+static void bfs_run_device(medusa_graph* g, uint32_t src, int32_t* out, const medusa_opts& o, medusa_stats& st) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const uint32_t nwords = (n + 63) / 64;
     const uint32_t large = o.large_degree ? o.large_degree : kLargeDefault;
-    g->level.ensure(n);
-    g->q_small.ensure(n);
-    g->q_large.ensure(n);
-    g->bitmap.ensure(nwords);
+    hipStream_t s = g->stream;
+    const uint32_t deg_src = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
+    const uint32_t src_class = deg_src == 0 ? 0u : (deg_src <= large ? 1u : 2u);
+    const uint32_t mode = bfs_mode(o);
+    const float* policy = mode == 3 ? policy_device_params(o.policy) : nullptr;
+    g->bfs_ctl.ensure(1);
+    BfsCtl* ctl = reinterpret_cast<BfsCtl*>(g->bfs_ctl.p);
+    static_assert(sizeof(BfsCtl) <= sizeof(BfsCtlStorage), "control block storage");
+    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
+
+    // capture (once per graph, threshold and policy buffer) the batch of supersteps
+    GraphCache& gc = g->bfs_graph;
+    const uintptr_t key[4] = {(uintptr_t)g->level.p, (uintptr_t)policy, large, (uintptr_t)g->bitmap.p};
+    if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
+        gc.reset();
+        MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (uint32_t k = 0; k < kBfsBatch; ++k) {
+            hipLaunchKernelGGL(k_bfs_expand_dev, dim3(kExpandGrid), dim3(256), 0, s, (const BfsCtl*)ctl,
+                               (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
+                               g->cptr.p, g->csrc.p, (const unsigned long long*)g->bitmap.p, g->level.p, n, ctr);
+            hipLaunchKernelGGL(k_bfs_compact_dev, dim3(grid_for(n, kCompactTile, 4096)), dim3(256), 0, s,
+                               (const BfsCtl*)ctl, n, (const int32_t*)g->level.p, g->row_ptr.p, large, g->q_small.p,
+                               g->q_large.p, g->bitmap.p, nwords, ctr);
+            hipLaunchKernelGGL(k_bfs_decide, dim3(1), dim3(256), 0, s, ctl, n, nwords, ctr, policy);
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        MDS_HIP(e);
+        std::copy(key, key + 4, gc.key);
+    }
+
+    MDS_HIP(hipEventRecord(g->ev_begin, s));
+    hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->level.p, src, g->q_small.p,
+                       g->q_large.p, src_class, g->bitmap.p, nwords);
+    MDS_LAUNCHED();
+    hipLaunchKernelGGL(k_bfs_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, deg_src, src_class,
+                       mode);
+    MDS_LAUNCHED();
+    hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, ctl, ctr, policy);
+    MDS_LAUNCHED();
+    BfsCtl h{};
+    for (uint32_t batch = 0;; ++batch) {
+        MDS_HIP(hipGraphLaunch(gc.exec, s));
+        MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
+        MDS_HIP(hipStreamSynchronize(s));
+        std::memcpy(&h, g->h_ctr_raw.p, sizeof(BfsCtl));
+        if (h.done) break;
+        MDS_CHECK(batch < (n + kBfsBatch) / kBfsBatch + 2, MEDUSA_ERR_INTERNAL, "bfs: superstep loop did not end");
+    }
+    MDS_HIP(hipEventRecord(g->ev_end, s));
+    MDS_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+    st.device_ms = ms;
+    st.supersteps = h.supersteps;
+    st.push_steps = h.push_steps;
+    st.pull_steps = h.pull_steps;
+    st.edges_traversed = h.reached_edges;
+    st.reached = (uint32_t)h.reached;
+    st.kernel_bytes = h.kernel_bytes;
+    st.alg_bytes = h.alg_bytes + 4ull * n + nwords * 8ull;
+    copy_out(g, out, g->level.p, (size_t)n * 4);
+}
+
+// Host-driven run: counters read back after every superstep (timing and tracing).
+static void bfs_run_host(medusa_graph* g, uint32_t src, int32_t* out, const medusa_opts& o, medusa_stats& st) {
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const uint32_t nwords = (n + 63) / 64;
+    const uint32_t large = o.large_degree ? o.large_degree : kLargeDefault;
     hipStream_t s = g->stream;
     const uint32_t deg_src = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
     const uint32_t src_class = deg_src == 0 ? 0u : (deg_src <= large ? 1u : 2u);
     const bool timing = o.timing != 0;
+    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
     g->trace.clear();
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
@@ -221,7 +543,6 @@ static void bfs_run(medusa_graph* g, uint32_t src, int32_t* out, const medusa_op
     MDS_LAUNCHED();
 
     Frontier f{};
-    f.step = 0;
     f.n = n;
     f.m = m;
     f.n_f = 1;
@@ -230,19 +551,16 @@ static void bfs_run(medusa_graph* g, uint32_t src, int32_t* out, const medusa_op
     f.m_u = m - deg_src;
     uint32_t n_small = src_class == 1, n_large = src_class == 2;
     uint64_t reached_edges = deg_src, reached = 1, bytes = 4ull * n + nwords * 8ull;
-    bool pull = o.direction == MEDUSA_DIR_PULL;
+    bool pull = false;
     Sched sched(o, Sched::kBfs);
-    size_t ev = 0;
+    if (timing) ensure_events(g, 2);
     for (uint32_t t = 0;; ++t) {
         f.step = t;
         if (o.direction == MEDUSA_DIR_PUSH) pull = false;
         else if (o.direction == MEDUSA_DIR_PULL) pull = true;
         else pull = sched.bfs_pull(f, pull);
-        if (timing) {
-            ensure_events(g, ev + 2);
-            MDS_HIP(hipEventRecord(g->events[ev], s));
-        }
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, dev_counters<FrontierCounters>(g));
+        if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, ctr);
         MDS_LAUNCHED();
         if (!pull) {
             if (n_small) {
@@ -259,25 +577,25 @@ static void bfs_run(medusa_graph* g, uint32_t src, int32_t* out, const medusa_op
             }
         } else {
             hipLaunchKernelGGL(k_bfs_pull, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, g->cptr.p, g->csrc.p,
-                               (const unsigned long long*)g->bitmap.p, g->level.p, (int32_t)(t + 1), dev_counters<FrontierCounters>(g));
+                               (const unsigned long long*)g->bitmap.p, g->level.p, (int32_t)(t + 1), ctr);
             MDS_LAUNCHED();
         }
-        if (timing) MDS_HIP(hipEventRecord(g->events[ev + 1], s));
+        if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
         hipLaunchKernelGGL(k_bfs_compact, dim3(grid_for(n, kCompactTile, 4096)), dim3(256), 0, s, n,
                            (const int32_t*)g->level.p, (int32_t)(t + 1), g->row_ptr.p, large, g->q_small.p,
-                           g->q_large.p, g->bitmap.p, nwords, dev_counters<FrontierCounters>(g));
+                           g->q_large.p, g->bitmap.p, nwords, ctr);
         MDS_LAUNCHED();
         const FrontierCounters c = read_counters<FrontierCounters>(g);
         double kms = 0;
         if (timing) {
             float ms = 0;
-            MDS_HIP(hipEventElapsedTime(&ms, g->events[ev], g->events[ev + 1]));
+            MDS_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
             kms = ms;
             st.kernel_ms += ms;
             ++st.kernel_launches;
         }
-        // byte model (see DESIGN): push = queue + offsets + col + level probe + claims;
-        // pull = level scan + probed in-edges (source + bitmap word) + claims
+        // byte model: push = queue + offsets + col + level probe + claims;
+        // pull = level scan + offsets + probed in-edges (source + bitmap word) + claims
         uint64_t kb;
         if (!pull) {
             kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * c.n_new;
@@ -324,8 +642,14 @@ extern "C" int medusa_bfs(medusa_graph* g, uint32_t source, int32_t* levels, con
         medusa_opts o{};
         if (opts) o = *opts;
         MDS_CHECK(o.direction >= 0 && o.direction <= 2, MEDUSA_ERR_ARG, "bfs: bad direction %d", o.direction);
+        const uint32_t n = g->n;
+        g->level.ensure(n);
+        g->q_small.ensure(n);
+        g->q_large.ensure(n);
+        g->bitmap.ensure((n + 63) / 64);
         medusa_stats st{};
-        bfs_run(g, source, levels, o, st);
+        if (o.timing || o.trace) bfs_run_host(g, source, levels, o, st);
+        else bfs_run_device(g, source, levels, o, st);
         if (stats) *stats = st;
     });
 }

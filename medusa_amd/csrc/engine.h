@@ -149,6 +149,26 @@ struct TraceRec {
     double f[MEDUSA_TRACE_FIELDS];
 };
 
+// An instantiated hipGraph of captured supersteps and the buffers it was captured with.
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    uintptr_t key[4] = {0, 0, 0, 0};
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    ~GraphCache() { reset(); }
+};
+
+// 256 bytes of device memory for a device-driven loop's control block.
+struct CtlStorage {
+    unsigned long long w[32];
+};
+using BfsCtlStorage = CtlStorage;
+
 }  // namespace medusa
 
 struct medusa_graph {
@@ -169,6 +189,9 @@ struct medusa_graph {
     // views the 256 bytes as its own counter struct
     medusa::DBuf<unsigned long long> ctr_raw;
     medusa::HBuf<unsigned long long> h_ctr_raw;
+    // device-driven superstep loops: control blocks and captured superstep batches
+    medusa::DBuf<medusa::CtlStorage> bfs_ctl, mbfs_ctl, sssp_ctl;
+    medusa::GraphCache bfs_graph, mbfs_graph, sssp_graph;
     std::unique_ptr<medusa::PrLayout> pr;
     std::vector<medusa::TraceRec> trace;
     std::vector<hipEvent_t> events;  // timing pool

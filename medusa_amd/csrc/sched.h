@@ -2,9 +2,58 @@
 // the learned policy hook, and the per-superstep trace that trains it.
 #pragma once
 
+#include <algorithm>
+#include <cmath>
+
 #include "engine.h"
+#include "mlp.h"
 
 namespace medusa {
+
+__host__ __device__ inline float lg1(double x) { return (float)log2(1.0 + (x > 0.0 ? x : 0.0)); }
+
+// Direction rows (record = step, n_f, m_f, n_u, m_u, n, m, ...): frontier and unvisited
+// vertex/edge shares on log scales, mean degree, step, size, and the candidate action
+// (0 push, 1 pull) with its interactions.
+__host__ __device__ inline void direction_row(const double* r, double action, float* x) {
+    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
+    const double n = r[5] > 1.0 ? r[5] : 1.0, m = r[6] > 1.0 ? r[6] : 1.0;
+    const float ln = lg1(n), lm = lg1(m);
+    const float a = action != 0.0 ? 1.f : 0.f;
+    x[0] = lg1(r[1]) / ln;
+    x[1] = lg1(r[2]) / lm;
+    x[2] = lg1(r[3]) / ln;
+    x[3] = lg1(r[4]) / lm;
+    x[4] = lg1(m / n) / 8.f;
+    x[5] = (float)(r[0] < 64.0 ? r[0] : 64.0) / 64.f;
+    x[6] = ln / 32.f;
+    x[8] = a;
+    x[9] = a * x[0];
+    x[10] = a * x[1];
+    x[11] = a * x[2];
+    x[12] = a * x[3];
+    x[13] = 1.f - a;
+}
+
+// Delta rows (record = phase, far entries, mean weight, threshold, 0, n, m, ...): pending far
+// entries, threshold in units of the mean weight, mean degree, size, phase index, and the
+// candidate width in units of the mean weight with its square and interactions.
+__host__ __device__ inline void delta_row(const double* r, double action, float* x) {
+    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
+    const double n = r[5] > 1.0 ? r[5] : 1.0, m = r[6] > 1.0 ? r[6] : 1.0;
+    const float ln = lg1(n);
+    const double wmean = r[2] > 1.0 ? r[2] : 1.0;
+    const float la = (float)log2((action > 1.0 ? action : 1.0) / wmean);
+    x[0] = lg1(r[1]) / ln;
+    x[1] = lg1(r[3] / wmean) / 16.f;
+    x[2] = lg1(m / n) / 8.f;
+    x[3] = ln / 32.f;
+    x[4] = (float)(r[0] < 4096.0 ? r[0] : 4096.0) / 4096.f;
+    x[5] = la / 8.f;
+    x[6] = (la / 8.f) * (la / 8.f);
+    x[7] = x[0] * la / 8.f;
+    x[8] = x[1] * la / 8.f;
+}
 
 // Frontier features at the start of a superstep.
 struct Frontier {
@@ -31,26 +80,76 @@ struct Sched {
         if (!cur_pull) return (double)f.m_f > (double)f.m_u / kAlpha;
         return !((double)f.n_f < (double)f.n / kBeta);
     }
-    bool bfs_pull(const Frontier& f, bool cur_pull) const { return heuristic_pull(f, cur_pull); }
-    // SSSP bucket width for the next threshold advance.
-    uint32_t sssp_delta(const Frontier& f, uint32_t dflt) const {
-        (void)f;
-        return dflt;
+    // Direction for the next superstep: the learned policy's cheaper action when one is
+    // given (and of the right kind), else the heuristic.
+    bool bfs_pull(const Frontier& f, bool cur_pull) const {
+        if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_DIRECTION) return heuristic_pull(f, cur_pull);
+        double rec[MEDUSA_TRACE_FIELDS];
+        to_record(f, cur_pull ? 1.0 : 0.0, 0.0, rec);
+        float x[MEDUSA_POLICY_INPUTS];
+        policy_row(MEDUSA_POLICY_DIRECTION, rec, 0.0, x);
+        const float c_push = mlp_eval_host(policy_params(o.policy), x);
+        policy_row(MEDUSA_POLICY_DIRECTION, rec, 1.0, x);
+        const float c_pull = mlp_eval_host(policy_params(o.policy), x);
+        return c_pull < c_push;
+    }
+    // SSSP bucket width at a threshold advance: the learned policy picks among
+    // wmean * 2^k (k = -2..8) the width with the lowest predicted time per unit of distance.
+    uint32_t sssp_delta(const Frontier& f, uint32_t dflt, uint32_t wmean, uint32_t thr) const {
+        if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_DELTA || o.delta) return dflt;
+        double rec[MEDUSA_TRACE_FIELDS];
+        to_sssp_record(f, wmean, thr, 0.0, 0.0, rec);
+        float best = 0.f;
+        uint32_t pick = dflt;
+        for (int k = -2; k <= 8; ++k) {
+            const double d = std::max(1.0, std::ldexp((double)wmean, k));
+            float x[MEDUSA_POLICY_INPUTS];
+            policy_row(MEDUSA_POLICY_DELTA, rec, d, x);
+            const float c = mlp_eval_host(policy_params(o.policy), x);
+            if (k == -2 || c < best) {
+                best = c;
+                pick = (uint32_t)d;
+            }
+        }
+        return pick;
+    }
+    static void to_record(const Frontier& f, double action, double kms, double* r) {
+        r[0] = f.step;
+        r[1] = (double)f.n_f;
+        r[2] = (double)f.m_f;
+        r[3] = (double)f.n_u;
+        r[4] = (double)f.m_u;
+        r[5] = f.n;
+        r[6] = (double)f.m;
+        r[7] = action;
+        r[8] = kms;
+    }
+    // SSSP phase record: phase index, far-queue entries, mean weight, threshold, n, m, delta, ms.
+    static void to_sssp_record(const Frontier& f, uint32_t wmean, uint32_t thr, double delta, double ms, double* r) {
+        r[0] = f.step;
+        r[1] = (double)f.n_f;
+        r[2] = wmean;
+        r[3] = thr;
+        r[4] = 0;
+        r[5] = f.n;
+        r[6] = (double)f.m;
+        r[7] = delta;
+        r[8] = ms;
+    }
+    static void policy_row(uint32_t kind, const double* rec, double action, float* x);
+    void record_sssp_phase(medusa_graph* g, const Frontier& f, uint32_t wmean, uint32_t thr, double delta,
+                           double ms) const {
+        if (!o.trace) return;
+        TraceRec r;
+        to_sssp_record(f, wmean, thr, delta, ms, r.f);
+        g->trace.push_back(r);
     }
 
     // Appends one trace record when tracing is on.
     void record(medusa_graph* g, const Frontier& f, double action, double kernel_ms) const {
         if (!o.trace) return;
         TraceRec r;
-        r.f[0] = f.step;
-        r.f[1] = (double)f.n_f;
-        r.f[2] = (double)f.m_f;
-        r.f[3] = (double)f.n_u;
-        r.f[4] = (double)f.m_u;
-        r.f[5] = f.n;
-        r.f[6] = (double)f.m;
-        r.f[7] = action;
-        r.f[8] = kernel_ms;
+        to_record(f, action, kernel_ms, r.f);
         g->trace.push_back(r);
     }
 };

@@ -43,10 +43,13 @@ def lib():
             "orc_mbfs": [_u32, _u32p, _u32p, _u32p, _u32, _u8p, _int],
             "orc_sssp_dijkstra": [_u32, _u32p, _u32p, _u32p, _u32, _u32p],
             "orc_sssp_bellman_ford": [_u32, _u32p, _u32p, _u32p, _u32, _u32p],
-            "orc_sssp_bsp": [_u32, _u64, _u32p, _u32p, _u32p, _u32, _u32p, _int],
+            "orc_sssp_bsp": [_u32, _u64, _u32p, _u32p, _u32p, _u32, _u32p, _int, _int],
+            "orc_sssp_rounds": [_u32, _u32p, _u32p, _u32p, _u32, _u32p],
             "orc_pagerank_f64": [_u32, _u64, _u32p, _u32p, _f64, _u32, _f64p, _int],
             "orc_pagerank_f32": [_u32, _u64, _u32p, _u32p, _f32, _u32, _f32p, _int],
             "orc_pagerank_ref": [_u32, _u64, _u32p, _u32p, _u32, _f32p, _int],
+            "orc_mlp_loss_grad": [_f32p, _f32p, _f32p, _u32, _f64p, _f64p],
+            "orc_mlp_forward": [_f32p, _f32p, _u32, _f64p],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -120,13 +123,21 @@ def sssp_bellman_ford(row_ptr, col, w, src):
     return dist
 
 
-def sssp_bsp(row_ptr, col, w, src, threads=0):
-    """Reference CPU path; returns (dist, supersteps)."""
+def sssp_bsp(row_ptr, col, w, src, threads=0, max_steps=0):
+    """Reference CPU path; returns (dist, supersteps).  max_steps > 0 stops early."""
     row_ptr, col, n, m = _csr(row_ptr, col)
     w = np.ascontiguousarray(w, dtype=np.uint32)
     dist = np.zeros(n, np.uint32)
-    steps = lib().orc_sssp_bsp(n, m, row_ptr, col, w, src, dist, threads)
+    steps = lib().orc_sssp_bsp(n, m, row_ptr, col, w, src, dist, threads, max_steps)
     return dist, steps
+
+
+def sssp_rounds(row_ptr, col, w, src, dist):
+    """Supersteps the reference CPU path needs, derived from final distances."""
+    row_ptr, col, n, m = _csr(row_ptr, col)
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    dist = np.ascontiguousarray(dist, dtype=np.uint32)
+    return lib().orc_sssp_rounds(n, row_ptr, col, w, src, dist)
 
 
 def pagerank_f64(row_ptr, col, d=0.85, steps=30, threads=0):
@@ -147,4 +158,26 @@ def pagerank_ref(row_ptr, col, steps=100, threads=0):
     row_ptr, col, n, m = _csr(row_ptr, col)
     out = np.zeros(n, np.float32)
     lib().orc_pagerank_ref(n, m, row_ptr, col, steps, out, threads)
+    return out
+
+
+MLP_PARAMS = 1633
+
+
+def mlp_loss_grad(params, X, y):
+    """FP64 loss and gradient of the scheduler MLP (mean squared error)."""
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = np.zeros(1, np.float64)
+    lib().orc_mlp_loss_grad(params, X, y, len(y), g, loss)
+    return float(loss[0]), g
+
+
+def mlp_forward(params, X):
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(len(X), np.float64)
+    lib().orc_mlp_forward(params, X, len(X), out)
     return out

@@ -267,7 +267,7 @@ int orc_sssp_bellman_ford(uint32_t n, const uint32_t* row_ptr, const uint32_t* c
 // uint32 contract: an unreached source sends the init value (no wrap-around).
 // Returns the number of supersteps.
 int orc_sssp_bsp(uint32_t n, uint64_t m, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w,
-                 uint32_t src, uint32_t* dist, int threads) {
+                 uint32_t src, uint32_t* dist, int threads, int max_steps) {
     set_threads(threads);
     std::vector<uint32_t> cptr, csrc, slot, esrc(m), msg(m);
     build_csc(n, m, row_ptr, col, cptr, csrc, slot);
@@ -296,9 +296,33 @@ int orc_sssp_bsp(uint32_t n, uint64_t m, const uint32_t* row_ptr, const uint32_t
             }
         }
         ++step;
-        if (!cont) break;
+        if (!cont || (max_steps > 0 && step >= max_steps)) break;
     }
     return step;
+}
+
+// Supersteps the reference CPU path needs for SSSP, from final distances: the
+// synchronous relaxation settles a vertex after as many supersteps as the fewest arcs on
+// any of its shortest paths, plus one final superstep that changes nothing.
+int orc_sssp_rounds(uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t src,
+                    const uint32_t* dist) {
+    std::vector<uint32_t> order;
+    for (uint32_t v = 0; v < n; ++v)
+        if (dist[v] != kUnreached) order.push_back(v);
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return dist[a] < dist[b]; });
+    std::vector<uint32_t> hops(n, 0xFFFFFFFFu);
+    hops[src] = 0;
+    uint32_t worst = 0;
+    // zero-weight arcs are impossible (weights >= 1), so distance order is a topological order
+    for (uint32_t u : order) {
+        if (hops[u] == 0xFFFFFFFFu) continue;
+        worst = std::max(worst, hops[u]);
+        for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) {
+            const uint32_t v = col[e];
+            if ((uint64_t)dist[u] + w[e] == dist[v]) hops[v] = std::min(hops[v], hops[u] + 1);
+        }
+    }
+    return (int)worst + 1;
 }
 
 // FP64 PageRank power iteration with the engine's formula:
@@ -410,3 +434,86 @@ int orc_pagerank_ref(uint32_t n, uint64_t m, const uint32_t* row_ptr, const uint
 }
 
 }  // extern "C"
+
+// ---- scheduler MLP in FP64 ---------------------------------------------------------
+// Same network as the engine's: x[16] -> 32 -> 32 -> 1, leaky ReLU (slope 1/64), parameters
+// W1[32][16], b1[32], W2[32][32], b2[32], W3[32], b3; loss = mean squared error.  Written
+// from the chain rule in double precision; the GPU's FP32 gradients are checked against it.
+extern "C" int orc_mlp_loss_grad(const float* params, const float* X, const float* y, uint32_t n, double* grad,
+                                 double* loss) {
+    constexpr int I = 16, H = 32;
+    const float* W1 = params;
+    const float* b1 = W1 + H * I;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H * H;
+    const float* W3 = b2 + H;
+    const double b3 = W3[H];
+    const double slope = 1.0 / 64.0;
+    const int P = H * I + H + H * H + H + H + 1;
+    for (int p = 0; p < P; ++p) grad[p] = 0.0;
+    double L = 0.0;
+    for (uint32_t s = 0; s < n; ++s) {
+        const float* x = X + (size_t)s * I;
+        double z1[H], a1[H], z2[H], a2[H];
+        for (int i = 0; i < H; ++i) {
+            double z = b1[i];
+            for (int k = 0; k < I; ++k) z += (double)W1[i * I + k] * x[k];
+            z1[i] = z;
+            a1[i] = z > 0 ? z : z * slope;
+        }
+        double o = b3;
+        for (int i = 0; i < H; ++i) {
+            double z = b2[i];
+            for (int k = 0; k < H; ++k) z += (double)W2[i * H + k] * a1[k];
+            z2[i] = z;
+            a2[i] = z > 0 ? z : z * slope;
+            o += (double)W3[i] * a2[i];
+        }
+        const double e = o - (double)y[s];
+        L += e * e;
+        const double g = 2.0 * e / n;
+        double d2[H];
+        for (int i = 0; i < H; ++i) {
+            grad[H * I + H + H * H + H + i] += g * a2[i];
+            d2[i] = g * W3[i] * (z2[i] > 0 ? 1.0 : slope);
+            grad[H * I + H + H * H + i] += d2[i];
+            for (int k = 0; k < H; ++k) grad[H * I + H + i * H + k] += d2[i] * a1[k];
+        }
+        grad[P - 1] += g;
+        for (int k = 0; k < H; ++k) {
+            double acc = 0;
+            for (int i = 0; i < H; ++i) acc += d2[i] * W2[i * H + k];
+            const double d1 = acc * (z1[k] > 0 ? 1.0 : slope);
+            grad[H * I + k] += d1;
+            for (int j = 0; j < I; ++j) grad[k * I + j] += d1 * x[j];
+        }
+    }
+    *loss = L / n;
+    return 0;
+}
+
+extern "C" int orc_mlp_forward(const float* params, const float* X, uint32_t n, double* out) {
+    constexpr int I = 16, H = 32;
+    const float* W1 = params;
+    const float* b1 = W1 + H * I;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H * H;
+    const float* W3 = b2 + H;
+    for (uint32_t s = 0; s < n; ++s) {
+        const float* x = X + (size_t)s * I;
+        double a1[H];
+        for (int i = 0; i < H; ++i) {
+            double z = b1[i];
+            for (int k = 0; k < I; ++k) z += (double)W1[i * I + k] * x[k];
+            a1[i] = z > 0 ? z : z / 64.0;
+        }
+        double o = W3[H];
+        for (int i = 0; i < H; ++i) {
+            double z = b2[i];
+            for (int k = 0; k < H; ++k) z += (double)W2[i * H + k] * a1[k];
+            o += (double)W3[i] * (z > 0 ? z : z / 64.0);
+        }
+        out[s] = o;
+    }
+    return 0;
+}

@@ -1,0 +1,171 @@
+"""Learned per-superstep scheduler: the MLP's GPU forward/backward against the FP64
+oracle, training, and what the trained policy changes in BFS (direction) and SSSP
+(bucket width) — results bit-exact, runtime only."""
+import numpy as np
+import pytest
+
+import medusa_amd as md
+from medusa_amd import workloads as wl
+from conftest import random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[case] {request.node.nodeid}", flush=True)
+
+
+def test_gpu_gradients_match_fp64_oracle(oracle):
+    rng = np.random.default_rng(0)
+    pol = md.Policy(md.POLICY_DIRECTION, seed=1)
+    X = rng.random((300, md.POLICY_INPUTS)).astype(np.float32)
+    y = rng.standard_normal(300).astype(np.float32)
+    g, loss = pol.gradients(X, y)
+    loss64, g64 = oracle.mlp_loss_grad(pol.params, X, y)
+    assert abs(loss - loss64) <= 1e-5 * max(1.0, loss64)
+    scale = np.abs(g64).max()
+    assert np.abs(g - g64).max() <= 1e-4 * scale + 1e-6, np.abs(g - g64).max() / scale
+    out = pol.predict(X)
+    np.testing.assert_allclose(out, oracle.mlp_forward(pol.params, X), rtol=1e-5, atol=1e-5)
+
+
+def test_training_reduces_loss(oracle):
+    rng = np.random.default_rng(1)
+    X = rng.random((2000, md.POLICY_INPUTS)).astype(np.float32)
+    y = (2 * X[:, 0] - X[:, 3] + np.sin(3 * X[:, 5]) + 0.5).astype(np.float32)
+    pol = md.Policy(md.POLICY_DIRECTION, seed=2)
+    loss0, _ = oracle.mlp_loss_grad(pol.params, X, y)
+    loss = pol.train(X, y, iters=1500, lr=3e-3)
+    loss64, _ = oracle.mlp_loss_grad(pol.params, X, y)
+    assert abs(loss - loss64) <= 1e-4 * max(1e-3, loss64)
+    assert loss64 < loss0 / 20, (loss0, loss64)
+    # the trained weights are deterministic: a second run from the same seed matches
+    pol2 = md.Policy(md.POLICY_DIRECTION, seed=2)
+    pol2.train(X, y, iters=1500, lr=3e-3)
+    assert np.array_equal(pol.params, pol2.params)
+
+
+def _direction_samples(G, src):
+    """Per-superstep (record, push µs, pull µs) from a push-only and a pull-only traced run;
+    BFS visits the same frontier sequence in either direction."""
+    _, _ = G.bfs(src, direction=md.DIR_PUSH, timing=1, trace=1)
+    tp = G.trace()
+    _, _ = G.bfs(src, direction=md.DIR_PULL, timing=1, trace=1)
+    tl = G.trace()
+    assert len(tp) == len(tl)
+    assert np.array_equal(tp[:, :7], tl[:, :7])
+    return tp, tp[:, 8] * 1e3, tl[:, 8] * 1e3
+
+
+def _rows(records, action):
+    return np.stack([md.policy_features(md.POLICY_DIRECTION, r, action) for r in records])
+
+
+def _train_direction_policy():
+    Xs, ys = [], []
+    graphs = [wl.rmat_graph(scale=s, edge_factor=16, seed=10 + s) for s in (14, 15, 16, 17)]
+    graphs += [wl.er_graph(2 ** 15, 2 ** 19, seed=21), wl.chung_lu_graph(200_000, 420_000, seed=22)]
+    for rp, col in graphs:
+        with md.Graph(rp, col) as G:
+            for src in (wl.max_outdeg_vertex(rp), 3):
+                rec, push_us, pull_us = _direction_samples(G, src)
+                for a, t in ((0, push_us), (1, pull_us)):
+                    Xs.append(_rows(rec, a))
+                    ys.append(np.log2(1.0 + t))
+    X = np.concatenate(Xs).astype(np.float32)
+    y = np.concatenate(ys).astype(np.float32)
+    pol = md.Policy(md.POLICY_DIRECTION, seed=7)
+    pol.train(X, y, iters=3000, lr=3e-3)
+    return pol
+
+
+@pytest.fixture(scope="module")
+def direction_policy():
+    return _train_direction_policy()
+
+
+@pytest.mark.parametrize("which", ["rmat18", "chung_lu"])
+def test_learned_direction_bfs(oracle, direction_policy, which):
+    if which == "rmat18":
+        rp, col = wl.rmat_graph(scale=18, edge_factor=16, seed=99)
+    else:
+        rp, col = wl.chung_lu_graph(600_000, 1_250_000, seed=98)
+    src = wl.max_outdeg_vertex(rp)
+    want = oracle.bfs_queue(rp, col, src)
+    pol = direction_policy
+    with md.Graph(rp, col) as G:
+        rec, push_us, pull_us = _direction_samples(G, src)
+        # the heuristic's and the policy's choices on the same supersteps (host-driven, traced)
+        lv_h, _ = G.bfs(src, trace=1)
+        act_h = G.trace()[:, 7].astype(int)
+        lv_l, _ = G.bfs(src, trace=1, policy=pol)
+        act_l = G.trace()[:, 7].astype(int)
+        assert np.array_equal(lv_h, want) and np.array_equal(lv_l, want)
+        # device-driven runs (heuristic and learned) give the same levels
+        assert np.array_equal(G.bfs(src)[0], want)
+        lv_d, st_d = G.bfs(src, policy=pol)
+        assert np.array_equal(lv_d, want)
+        cost = np.stack([push_us, pull_us], 1)
+        steps = np.arange(len(rec))
+        c_heur = cost[steps, act_h].sum()
+        c_learn = cost[steps, act_l].sum()
+        c_best = cost.min(1).sum()
+        print(f"[{which}] modelled superstep time: heuristic {c_heur:.1f} us, learned {c_learn:.1f} us, "
+              f"best {c_best:.1f} us; learned/heuristic speedup {c_heur / c_learn:.2f}x", flush=True)
+        assert c_learn <= 1.10 * c_heur
+
+
+def _delta_samples(G, src, deltas):
+    Xs, ys = [], []
+    for d in deltas:
+        G.sssp(src, delta=d, trace=1)
+        tr = G.trace()
+        for r in tr:
+            if r[7] <= 0 or r[8] <= 0:
+                continue
+            Xs.append(md.policy_features(md.POLICY_DELTA, r, r[7]))
+            ys.append(np.log2(r[8] * 1e3 / r[7] + 1e-3))  # time per unit of threshold advanced
+    return Xs, ys
+
+
+def test_learned_delta_sssp(oracle):
+    Xs, ys = [], []
+    for rows, cols, seed in ((200, 200, 31), (400, 300, 32), (500, 500, 33)):
+        rp, col, w = wl.lattice_graph(rows, cols, seed=seed)
+        with md.Graph(rp, col, w) as G:
+            x, y = _delta_samples(G, wl.lattice_source(rows, cols), [250, 500, 1000, 2000, 4000, 8000, 16000])
+            Xs += x
+            ys += y
+    pol = md.Policy(md.POLICY_DELTA, seed=3)
+    pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
+    rp, col, w = wl.lattice_graph(700, 700, seed=34)
+    src = wl.lattice_source(700, 700)
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        d_plain, st_plain = G.sssp(src)
+        d_learn, st_learn = G.sssp(src, policy=pol)
+        d_host, _ = G.sssp(src, policy=pol, trace=1)
+        assert np.array_equal(d_plain, want)
+        assert np.array_equal(d_learn, want)
+        assert np.array_equal(d_host, want)
+        print(f"[sssp 700x700] plain: {st_plain.supersteps} supersteps {st_plain.device_ms:.2f} ms; "
+              f"learned: {st_learn.supersteps} supersteps {st_learn.device_ms:.2f} ms", flush=True)
+
+
+def test_device_and_host_drivers_agree(oracle):
+    rp, col, _ = random_graph(20000, 160000, 12)
+    w = np.random.default_rng(12).integers(1, 1001, len(col)).astype(np.uint32)
+    with md.Graph(rp, col, w) as G:
+        for d in (md.DIR_AUTO, md.DIR_PUSH, md.DIR_PULL):
+            a, sa = G.bfs(5, direction=d)
+            b, sb = G.bfs(5, direction=d, timing=1)
+            assert np.array_equal(a, b)
+            assert (sa.supersteps, sa.push_steps, sa.pull_steps, sa.edges_traversed) == \
+                   (sb.supersteps, sb.push_steps, sb.pull_steps, sb.edges_traversed)
+        want = oracle.sssp_dijkstra(rp, col, w, 5)
+        for delta in (0, 1, 300, 0xFFFFFFFF):
+            a, sa = G.sssp(5, delta=delta)
+            b, sb = G.sssp(5, delta=delta, timing=1)
+            assert np.array_equal(a, want) and np.array_equal(b, want)
+            assert sa.edges_traversed == sb.edges_traversed and sa.reached == sb.reached

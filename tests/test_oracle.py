@@ -314,3 +314,53 @@ def test_planted_mistake_is_caught(oracle):
         r = 0.15 / n + 0.85 * s
     r_bad = r
     assert not np.allclose(r_bad, r_dense, rtol=1e-6)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_sssp_rounds_predicts_reference_path(oracle, seed):
+    """The superstep count derived from distances equals the count the reference CPU path runs."""
+    rp, col, _ = random_graph(500, 2500, seed)
+    w = np.random.default_rng(seed).integers(1, 50, len(col)).astype(np.uint32)
+    d, steps = oracle.sssp_bsp(rp, col, w, 3)
+    assert oracle.sssp_rounds(rp, col, w, 3, d) == steps
+    d2, s2 = oracle.sssp_bsp(rp, col, w, 3, max_steps=2)
+    assert s2 == min(2, steps)
+
+
+def test_mlp_oracle_gradient_matches_finite_differences(oracle):
+    """The FP64 MLP gradient (chain rule) against central differences of the FP64 loss."""
+    rng = np.random.default_rng(3)
+    P = oracle.MLP_PARAMS
+    params = (rng.standard_normal(P) * 0.3).astype(np.float32)
+    X = rng.random((40, 16)).astype(np.float32)
+    y = rng.standard_normal(40).astype(np.float32)
+    loss, g = oracle.mlp_loss_grad(params, X, y)
+    assert loss > 0
+    idx = np.concatenate([rng.choice(P, 60, replace=False), [P - 1, 0, 512]])
+    checked = 0
+    for p in idx:
+        h = np.float32(1e-3 * max(1.0, abs(float(params[p]))))
+        up, dn = params.copy(), params.copy()
+        up[p] = params[p] + h
+        dn[p] = params[p] - h
+        lu, _ = oracle.mlp_loss_grad(up, X, y)
+        ld, _ = oracle.mlp_loss_grad(dn, X, y)
+        fd = (lu - ld) / (float(up[p]) - float(dn[p]))
+        # a kink of the leaky ReLU inside [p-h, p+h] makes the difference quotient inexact
+        if abs(fd - g[p]) <= 1e-6 + 1e-4 * abs(g[p]):
+            checked += 1
+    assert checked >= len(idx) - 3, (checked, len(idx))
+
+
+def test_mlp_oracle_forward_consistent_with_loss(oracle):
+    rng = np.random.default_rng(4)
+    params = (rng.standard_normal(oracle.MLP_PARAMS) * 0.2).astype(np.float32)
+    X = rng.random((25, 16)).astype(np.float32)
+    y = rng.standard_normal(25).astype(np.float32)
+    out = oracle.mlp_forward(params, X)
+    loss, _ = oracle.mlp_loss_grad(params, X, y)
+    assert abs(loss - np.mean((out - y.astype(np.float64)) ** 2)) < 1e-12
+    # a zero network predicts b3
+    z = np.zeros(oracle.MLP_PARAMS, np.float32)
+    z[-1] = 0.5
+    assert np.all(oracle.mlp_forward(z, X) == 0.5)
