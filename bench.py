@@ -32,7 +32,7 @@ def parse():
     p.add_argument("--scale", type=int, default=20)
     p.add_argument("--workload", default="pagerank_rmat20",
                    choices=["pagerank_rmat20", "bfs_er", "bfs_rmat20", "sssp_lattice", "mbfs_rmat20",
-                            "chung_lu"],
+                            "chung_lu", "chung_lu_batch", "sssp_rmat20"],
                    help="pagerank_rmat20 is the headline line; the others measure the remaining configurations")
     p.add_argument("--profile", action="store_true",
                    help="short run for hardware counters: 3 PageRank runs (non-graph launches) and one "
@@ -262,13 +262,21 @@ def run_workload(a, md, wl, torch, np):
                          f"{steps_cpu} supersteps"}
         _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
         G.close()
-    elif w == "sssp_lattice":
-        rp, col, wt = wl.lattice_graph()
-        src = wl.lattice_source()
-        n, m = len(rp) - 1, len(col)
-        cfg = {"workload": "sssp_lattice", "rows": 1400, "cols": 1400, "vertices": n, "arcs": m, "source": src,
-               "graph": "lattice p_keep 0.7 + 2% diagonals, both directions, weights U[1,1000], seed 3",
-               "dtype": "uint32"}
+    elif w in ("sssp_lattice", "sssp_rmat20"):
+        if w == "sssp_lattice":
+            rp, col, wt = wl.lattice_graph()
+            src = wl.lattice_source()
+            n, m = len(rp) - 1, len(col)
+            cfg = {"workload": "sssp_lattice", "rows": 1400, "cols": 1400, "vertices": n, "arcs": m, "source": src,
+                   "graph": "lattice p_keep 0.7 + 2% diagonals, both directions, weights U[1,1000], seed 3",
+                   "dtype": "uint32"}
+        else:
+            rp, col = wl.rmat_graph()
+            wt = wl.weighted(rp, col, seed=6)
+            src = wl.max_outdeg_vertex(rp)
+            n, m = len(rp) - 1, len(col)
+            cfg = {"workload": "sssp_rmat20", "vertices": n, "arcs": m, "source": src,
+                   "graph": "RMAT scale 20, edge factor 16, seed 2; weights U[1,1000] seed 6", "dtype": "uint32"}
         G = md.Graph(rp, col, wt)
         out = torch.empty(n, dtype=torch.int32, device="cuda")
         wall, res = _timed(lambda: G.sssp(src, out=out)[1], a.steps, a.warmup, torch)
@@ -324,6 +332,83 @@ def run_workload(a, md, wl, torch, np):
         tc = time.perf_counter() - c0
         cpu = {"value": round(cpu_edges / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
                "sample": f"reference CPU path restated (all-edge sweep BFS, OpenMP) for {ks} of the 64 sources"}
+        _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
+        G.close()
+    elif w == "chung_lu_batch":
+        graphs, sizes = wl.chung_lu_batch()
+        G = md.Graph.packed(graphs)
+        begin = G.batch()
+        n, m = G.n, G.m
+        local = [wl.max_outdeg_vertex(grp) for grp, _ in graphs]
+        sources = np.array([begin[k] + s for k, s in enumerate(local)], np.uint32)
+        cfg = {"workload": "chung_lu_batch", "graphs": len(graphs), "vertices": n, "arcs": m,
+               "graph": "128 Chung-Lu graphs, n log-uniform in [2^10, 2^18], edge factor 8, Zipf(2.0), seed 5; "
+                        "one source per graph (its max out-degree vertex)", "dtype": "int32"}
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
+        # learned direction policy, trained on the GPU from this batch's own push-only and
+        # pull-only per-superstep traces (the plain engine's logs)
+        Xs, ys = [], []
+        for a_, d in ((0, md.DIR_PUSH), (1, md.DIR_PULL)):
+            G.bfs_multi(sources, direction=d, timing=1, trace=1)
+            tr = G.trace()
+            Xs.append(np.stack([md.policy_features(md.POLICY_DIRECTION, r, a_) for r in tr]))
+            ys.append(np.log2(1.0 + tr[:, 8] * 1e3))
+        pol = md.Policy(md.POLICY_DIRECTION, seed=7)
+        pol.train(np.concatenate(Xs).astype(np.float32), np.concatenate(ys).astype(np.float32), iters=2000,
+                  lr=3e-3)
+        wall, res = _timed(lambda: G.bfs_multi(sources, out=out, policy=pol)[1], a.steps, a.warmup, torch)
+        m_r = res[-1].edges_traversed
+        wh, rh = _timed(lambda: G.bfs_multi(sources, out=out)[1], a.steps, 1, torch)
+        lv_batch = out.cpu().numpy()
+        # plain engine: one handle and one BFS per graph
+        plain = [md.Graph(grp, gcol) for grp, gcol in graphs]
+        pouts = [torch.empty(len(grp) - 1, dtype=torch.int32, device="cuda") for grp, _ in graphs]
+        for H, s_, o_ in zip(plain, local, pouts):
+            H.bfs(s_, out=o_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_edges = sum(H.bfs(s_, out=o_)[1].edges_traversed for H, s_, o_ in zip(plain, local, pouts))
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        same = all(np.array_equal(o_.cpu().numpy(), lv_batch[begin[k]:begin[k + 1]]) for k, o_ in enumerate(pouts))
+        pr_out = torch.empty(n, dtype=torch.float32, device="cuda")
+        wpr, _ = _timed(lambda: G.pagerank(0.85, 20, out=pr_out, use_graph=1), a.steps, 1, torch)
+        pr_outs = [torch.empty(len(grp) - 1, dtype=torch.float32, device="cuda") for grp, _ in graphs]
+        for H, o_ in zip(plain, pr_outs):
+            H.pagerank(0.85, 20, out=o_, use_graph=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for H, o_ in zip(plain, pr_outs):
+            H.pagerank(0.85, 20, out=o_, use_graph=1)
+        torch.cuda.synchronize()
+        tpr = time.perf_counter() - t0
+        pr_b = pr_out.cpu().numpy()
+        pr_same = all(np.array_equal(o_.cpu().numpy().view(np.uint32), pr_b[begin[k]:begin[k + 1]].view(np.uint32))
+                      for k, o_ in enumerate(pr_outs))
+        for H in plain:
+            H.close()
+        extra = {"supersteps": res[-1].supersteps, "push_steps": res[-1].push_steps,
+                 "pull_steps": res[-1].pull_steps, "edges_reached": m_r,
+                 "gteps_batched_heuristic": round(a.steps * m_r / wh / 1e9, 3),
+                 "gteps_plain_per_graph": round(plain_edges / tp / 1e9, 3),
+                 "learned_batched_over_plain": round((a.steps * m_r / wall) / (plain_edges / tp), 2),
+                 "levels_identical_to_plain": bool(same),
+                 "pagerank20_batched_gteps": round(a.steps * 20 * m / wpr / 1e9, 3),
+                 "pagerank20_plain_per_graph_gteps": round(20 * m / tpr / 1e9, 3),
+                 "pagerank20_batched_over_plain": round((a.steps / wpr) / (1.0 / tpr), 2),
+                 "pagerank_bits_identical_to_plain": bool(pr_same)}
+        roof = _roofline([G.bfs_multi(sources, out=out, timing=1)[1] for _ in range(3)],
+                         "k_bfs_push / k_bfs_pull")
+        ks = 16
+        c0 = time.perf_counter()
+        cpu_edges = 0
+        for (grp, gcol), s_ in list(zip(graphs, local))[:ks]:
+            lv_cpu, _ = oracle.bfs_sweep(grp, gcol, int(s_), threads)
+            cpu_edges += int(np.diff(grp.astype(np.int64))[lv_cpu >= 0].sum())
+        tc = time.perf_counter() - c0
+        cpu = {"value": round(cpu_edges / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+               "sample": f"reference CPU path restated (all-edge sweep BFS, OpenMP) on the first {ks} of the "
+                         f"128 graphs"}
         _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
         G.close()
 

@@ -63,7 +63,8 @@ typedef struct medusa_opts {
     uint32_t trace;        /* nonzero: keep a per-superstep trace, read back with medusa_trace_* */
     const medusa_policy* policy; /* learned scheduler; NULL = plain engine */
     uint32_t use_graph;    /* nonzero: run the fixed superstep sequence as one captured hipGraph (PageRank) */
-    uint32_t reserved[7];
+    uint32_t grid;         /* workgroups of the BFS/MBFS frontier-expansion launch; 0 = default (or tuned) */
+    uint32_t reserved[6];
 } medusa_opts;
 
 /* Per-run statistics.  Reference: printf of wall time / steps (Algorithm/main.cu:316,355,
@@ -102,6 +103,17 @@ int medusa_graph_create(uint32_t n, uint64_t m, const uint32_t* row_ptr, const u
 /* Replaces ResetGraphData's re-upload (MultiPublicAPI.h:310-326): frees everything. */
 int medusa_graph_destroy(medusa_graph* g);
 int medusa_graph_info(const medusa_graph* g, uint32_t* n, uint64_t* m, uint32_t* weighted);
+/* Declares the handle a packed batch of `count` independent graphs: graph k owns the
+ * vertex range [graph_begin[k], graph_begin[k+1]) (graph_begin[0] = 0, graph_begin[count]
+ * = n, every range non-empty) and no edge may leave its range.  PageRank then normalises
+ * each graph by its own vertex count and dangling mass (ranks bit-identical to a run on
+ * each graph alone); BFS from one source per graph (medusa_bfs_multi) gives each graph's
+ * own levels; all graphs share every launch.  No reference counterpart (the reference runs
+ * one graph per process; its GraphPartition ranges, MultipleGPU/PartitionManager.cu:116-534,
+ * split one graph across GPUs instead).  count = 1 with {0, n} restores a plain graph. */
+int medusa_graph_set_batch(medusa_graph* g, uint32_t count, const uint32_t* graph_begin);
+/* Number of packed graphs and up to `capacity` entries of graph_begin (count+1 in all). */
+int medusa_graph_batch(const medusa_graph* g, uint32_t* count, uint32_t* graph_begin, uint32_t capacity);
 /* Device addresses of the CSR/CSC arrays (for header-only functor kernels compiled by the
  * caller).  Replaces the __constant__ D_VertexArray/D_EdgeArray symbols
  * (Algorithm/DeviceDataStructure.h:7-24).  Any pointer argument may be NULL. */
@@ -116,6 +128,12 @@ int medusa_graph_device_arrays(const medusa_graph* g, const uint32_t** row_ptr, 
  * (MultipleGPU/MultiUtilities.h:48-69).  levels[n]: int32, -1 unreached. */
 int medusa_bfs(medusa_graph* g, uint32_t source, int32_t* levels, const medusa_opts* opts,
                medusa_stats* stats);
+/* BFS from `count` sources at once (duplicates allowed): every source starts at level 0
+ * and levels[v] is the distance from the nearest one.  Same loop as medusa_bfs (which is
+ * the count = 1 case); on a packed batch with one source per graph it runs every graph's
+ * BFS in the same supersteps. */
+int medusa_bfs_multi(medusa_graph* g, const uint32_t* sources, uint32_t count, int32_t* levels,
+                     const medusa_opts* opts, medusa_stats* stats);
 
 /* SSSP: replaces SSSP/UserOperators.h:200-264 (InitMessageBuffer(FLT_MAX), ssspEdgeProcAA
  * scatter, combineAllDevice = CUDPP segmented MIN scan, UpdateVertex) and its loop
@@ -128,7 +146,8 @@ int medusa_sssp(medusa_graph* g, uint32_t source, uint32_t* dist, const medusa_o
  * UpdateVertex) and the fixed-step loop MPG/main.cu:144-169.  Formula (FP32, Kahan-
  * compensated combine in a fixed order):
  *   r_0 = 1/n;  r_{t+1}(v) = (1-d)/n + d*( sum_{u->v} r_t(u)/outdeg(u) + D_t/n ),
- *   D_t = sum of r_t over vertices with outdeg 0.   rank[n] float. */
+ *   D_t = sum of r_t over vertices with outdeg 0.   rank[n] float.
+ * On a packed batch (medusa_graph_set_batch) n and D_t are those of each vertex's graph. */
 int medusa_pagerank(medusa_graph* g, float damping, uint32_t steps, float* rank,
                     const medusa_opts* opts, medusa_stats* stats);
 
@@ -138,6 +157,29 @@ int medusa_pagerank(medusa_graph* g, float damping, uint32_t steps, float* rank,
  * reference is multi-GPU BFS, HowToBuild:13); equals k independent medusa_bfs runs. */
 int medusa_mbfs(medusa_graph* g, const uint32_t* sources, uint32_t k, uint8_t* levels,
                 const medusa_opts* opts, medusa_stats* stats);
+
+/* ---- autotuner ------------------------------------------------------------
+ * Chooses, per graph, the launch shape of the frontier engines: the out-degree above
+ * which a frontier vertex is split over whole workgroups (opts.large_degree) and the
+ * workgroups of the expansion launch (opts.grid).  Every candidate is run `reps` times
+ * from `sources` (BFS: any count, as medusa_bfs_multi; MBFS: at most 64) and timed with
+ * hipEvents; the fastest is kept in the handle and used by every later run whose opts
+ * leave large_degree / grid at 0.  reps = 0 forgets the tuned shape.  Results never
+ * depend on the shape, only run time.  Replaces the reference's fixed launch shape
+ * (MultipleGPU/MultiAPIHoster.h:649-754) and hybrid threshold (MultiPublicAPI.h:82). */
+#define MEDUSA_TUNE_BFS  1
+#define MEDUSA_TUNE_MBFS 2
+typedef struct medusa_tuning {
+    uint32_t large_degree;  /* chosen hub threshold */
+    uint32_t grid;          /* chosen expansion workgroups */
+    uint32_t candidates;    /* shapes timed */
+    uint32_t reserved0;
+    double   best_ms;       /* best run time of the chosen shape */
+    double   default_ms;    /* best run time of the engine's default shape */
+    uint32_t reserved[4];
+} medusa_tuning;
+int medusa_autotune(medusa_graph* g, uint32_t algo, const uint32_t* sources, uint32_t count, uint32_t reps,
+                    medusa_tuning* result);
 
 /* ---- per-superstep trace (feeds the learned scheduler) --------------------
  * Reference analogue: frontier counters of Algorithm/CPUFunctorHoster.cpp:13-15,21-30

@@ -32,8 +32,10 @@ EXPORTED = [
     "medusa_mbfs", "medusa_trace_count", "medusa_trace_read", "medusa_gen_erdos_renyi", "medusa_gen_rmat",
     "medusa_gen_lattice", "medusa_gen_chung_lu", "medusa_gen_weights", "medusa_csr_from_arcs",
     "medusa_policy_create", "medusa_policy_destroy", "medusa_policy_params", "medusa_policy_train",
-    "medusa_policy_predict", "medusa_policy_gradients", "medusa_policy_features",
+    "medusa_policy_predict", "medusa_policy_gradients", "medusa_policy_features", "medusa_graph_set_batch",
+    "medusa_graph_batch", "medusa_bfs_multi", "medusa_autotune",
 ]
+TUNE_BFS, TUNE_MBFS = 1, 2
 POLICY_INPUTS = 16
 POLICY_PARAMS = 1633
 POLICY_DIRECTION, POLICY_DELTA = 1, 2
@@ -55,7 +57,20 @@ class Opts(ctypes.Structure):
         ("trace", ctypes.c_uint32),
         ("policy", ctypes.c_void_p),
         ("use_graph", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 7),
+        ("grid", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 6),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("large_degree", ctypes.c_uint32),
+        ("grid", ctypes.c_uint32),
+        ("candidates", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("best_ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 
@@ -129,6 +144,10 @@ def lib():
             "medusa_policy_predict": ([_vp, _vp, _u32, _vp], ctypes.c_int),
             "medusa_policy_gradients": ([_vp, _vp, _vp, _u32, _vp, ctypes.POINTER(_f32)], ctypes.c_int),
             "medusa_policy_features": ([_u32, _vp, _f64, _vp], ctypes.c_int),
+            "medusa_graph_set_batch": ([_vp, _u32, _vp], ctypes.c_int),
+            "medusa_graph_batch": ([_vp, ctypes.POINTER(_u32), _vp, _u32], ctypes.c_int),
+            "medusa_bfs_multi": ([_vp, _vp, _u32, _vp, ctypes.POINTER(Opts), ctypes.POINTER(Stats)], ctypes.c_int),
+            "medusa_autotune": ([_vp, _u32, _vp, _u32, _u32, ctypes.POINTER(Tuning)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -219,6 +238,31 @@ def csr_from_arcs(n, src, dst, w=None):
     return row_ptr, col[:m], (wo[:m] if wo is not None else None)
 
 
+def pack_graphs(graphs):
+    """Disjoint union of CSR graphs [(row_ptr, col[, weight]), ...] (host only): returns
+    (row_ptr, col, weight-or-None, graph_begin) with graph k's vertex v at graph_begin[k] + v."""
+    rps, cols, ws, begin = [], [], [], [0]
+    e0 = 0
+    for gr in graphs:
+        rp, col = _u32arr(gr[0]), _u32arr(gr[1])
+        n = len(rp) - 1
+        rps.append(rp[:-1].astype(np.uint64) + e0)
+        cols.append(col + np.uint32(begin[-1]))
+        ws.append(_u32arr(gr[2]) if len(gr) > 2 and gr[2] is not None else None)
+        e0 += len(col)
+        begin.append(begin[-1] + n)
+    if begin[-1] >= 2 ** 32 or e0 >= 2 ** 32:
+        raise MedusaError(-1, "packed batch exceeds 32-bit ids")
+    row_ptr = np.concatenate(rps + [np.array([e0], np.uint64)]).astype(np.uint32)
+    col = np.concatenate(cols).astype(np.uint32) if cols else np.zeros(0, np.uint32)
+    w = None
+    if any(x is not None for x in ws):
+        if not all(x is not None for x in ws):
+            raise MedusaError(-1, "pack_graphs: either every graph or none has weights")
+        w = np.concatenate(ws).astype(np.uint32)
+    return row_ptr, col, w, np.array(begin, np.uint32)
+
+
 def _opts(**kw):
     o = Opts()
     for k, v in kw.items():
@@ -241,6 +285,26 @@ class Graph:
         _check(lib().medusa_graph_create(self.n, self.m, _ptr(row_ptr), _ptr(col) if self.m else None, _ptr(w),
                                          ctypes.byref(h)))
         self.handle = h
+
+    @classmethod
+    def packed(cls, graphs):
+        """One handle holding a batch of independent graphs (see pack_graphs)."""
+        row_ptr, col, w, begin = pack_graphs(graphs)
+        G = cls(row_ptr, col, w)
+        G.set_batch(begin)
+        return G
+
+    def set_batch(self, graph_begin):
+        b = _u32arr(graph_begin)
+        _check(lib().medusa_graph_set_batch(self.handle, len(b) - 1, _ptr(b)))
+
+    def batch(self):
+        """graph_begin (count+1 entries) of the packed graphs; [0, n] for a plain graph."""
+        c = _u32(0)
+        _check(lib().medusa_graph_batch(self.handle, ctypes.byref(c), None, 0))
+        out = np.zeros(c.value + 1, np.uint32)
+        _check(lib().medusa_graph_batch(self.handle, ctypes.byref(c), _ptr(out), c.value + 1))
+        return out
 
     @classmethod
     def from_arcs(cls, n, src, dst, w=None):
@@ -282,6 +346,25 @@ class Graph:
         o = _opts(**opts)
         _check(lib().medusa_bfs(self.handle, source, _ptr(out), ctypes.byref(o), ctypes.byref(st)))
         return out, st
+
+    def bfs_multi(self, sources, out=None, **opts):
+        """BFS from several sources at once (all at level 0); returns (levels, Stats)."""
+        sources = _u32arr(sources)
+        if out is None:
+            out = np.empty(self.n, np.int32)
+        st = Stats()
+        o = _opts(**opts)
+        _check(lib().medusa_bfs_multi(self.handle, _ptr(sources), len(sources), _ptr(out), ctypes.byref(o),
+                                      ctypes.byref(st)))
+        return out, st
+
+    def autotune(self, algo, sources, reps=3):
+        """Times the launch-shape candidates for TUNE_BFS / TUNE_MBFS from `sources` and keeps
+        the fastest in the handle; reps=0 forgets it.  Returns the Tuning record."""
+        sources = _u32arr(np.atleast_1d(sources))
+        t = Tuning()
+        _check(lib().medusa_autotune(self.handle, algo, _ptr(sources), len(sources), reps, ctypes.byref(t)))
+        return t
 
     def sssp(self, source, out=None, **opts):
         if out is None:

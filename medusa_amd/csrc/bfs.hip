@@ -8,7 +8,9 @@
 // MultipleGPU/MultiUtilities.h:48-69).  Levels are int32, -1 = unreached
 // (Algorithm/Message.h:9-10).
 //
-// Here a superstep only touches the frontier:
+// Any number of sources may be given (medusa_bfs_multi): all start at level 0, which on a
+// packed batch of disjoint graphs with one source per graph gives every graph's own BFS
+// levels in one run.  Here a superstep only touches the frontier:
 //   push:  frontier queue -> wave-cooperative expansion (lanes share the out-edges
 //          of 64 frontier vertices through an in-wave prefix scan), hubs above
 //          `large` out-degree get split over whole workgroups;
@@ -21,10 +23,10 @@
 // writer stores the same value, so the race is benign (as in the reference).
 //
 // Two drivers run the same kernels:
-//   device-driven (default): the last workgroup of each compaction decides the next
-//     superstep (direction by the heuristic or the learned policy, queue sizes, the
+//   device-driven (default): a one-workgroup kernel after each compaction decides the
+//     next superstep (direction by the heuristic or the learned policy, queue sizes, the
 //     termination vote) into a control block in HBM; kBfsBatch supersteps of
-//     [expand, compact+decide] are captured once as a hipGraph and replayed until the
+//     [expand, compact, decide] are captured once as a hipGraph and replayed until the
 //     control block says done, so the host waits once per batch, not per superstep;
 //   host-driven (timing or trace requested): the host reads the counters after every
 //     superstep, times the expansion with hipEvents and logs the trace record.
@@ -40,7 +42,6 @@ constexpr uint32_t kLargeDefault = 2048;
 constexpr uint32_t kLargeSplit = 16;  // workgroups per hub in the large-vertex push
 constexpr uint32_t kBfsBatch = 8;     // supersteps per captured graph
 constexpr uint32_t kExpandGrid = 2048;
-constexpr uint32_t kLargeGroups = 256;  // workgroups of the expand kernel given to hubs when pushing
 
 // Control block of the device-driven loop.
 struct BfsCtl {
@@ -50,21 +51,30 @@ struct BfsCtl {
     uint32_t n_small, n_large;
     uint32_t supersteps, push_steps, pull_steps;
     uint32_t mode;      // 0 heuristic, 1 push, 2 pull, 3 learned policy
-    uint32_t spare;
+    uint32_t hub_wgs;   // push: workgroups of the expansion given to the hub queue
     uint32_t pad[2];
     unsigned long long n, m, n_f, m_f, n_u, m_u, reached, reached_edges, kernel_bytes, alg_bytes;
 };
 
-__global__ void k_bfs_init(uint32_t n, int32_t* __restrict__ level, uint32_t src, uint32_t* __restrict__ q_small,
-                           uint32_t* __restrict__ q_large, uint32_t src_class, unsigned long long* __restrict__ bitmap,
-                           uint32_t nwords) {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    for (uint32_t v = tid; v < n; v += stride) level[v] = (v == src) ? 0 : -1;
-    for (uint32_t w = tid; w < nwords; w += stride) bitmap[w] = (w == src / 64) ? (1ull << (src % 64)) : 0ull;
-    if (tid == 0) {
-        if (src_class == 1) q_small[0] = src;
-        if (src_class == 2) q_large[0] = src;
+// Level reset (with `single` at level 0 when it is < n) and zeroed frontier counters.
+__global__ void k_bfs_reset(uint32_t n, int32_t* __restrict__ level, uint32_t single,
+                            FrontierCounters* __restrict__ ctr) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        level[v] = v == single ? 0 : -1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctr->n_small = 0;
+        ctr->n_large = 0;
+        ctr->n_new = 0;
+        ctr->pad = 0;
+        ctr->deg_sum = 0;
+        ctr->probes = 0;
+        ctr->deg_large = 0;
     }
+}
+
+__global__ void k_bfs_mark(const uint32_t* __restrict__ src, uint32_t count, int32_t* __restrict__ level) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        level[src[i]] = 0;
 }
 
 __global__ void k_zero_counters(FrontierCounters* c) {
@@ -74,38 +84,45 @@ __global__ void k_zero_counters(FrontierCounters* c) {
     c->pad = 0;
     c->deg_sum = 0;
     c->probes = 0;
+    c->deg_large = 0;
 }
 
-// Counters and control block as if a compaction had produced the frontier {src}.
-__global__ void k_bfs_seed_ctl(BfsCtl* ctl, FrontierCounters* c, uint32_t n, unsigned long long m, uint32_t deg_src,
-                               uint32_t src_class, uint32_t mode) {
-    c->n_small = src_class == 1;
-    c->n_large = src_class == 2;
-    c->n_new = 1;
-    c->pad = 0;
-    c->deg_sum = deg_src;
-    c->probes = 0;
-    BfsCtl z{};
-    z.step = -1;
-    z.mode = mode;
-    z.n = n;
-    z.m = m;
-    z.n_u = n;
-    z.m_u = m;
-    *ctl = z;
+// Workgroups of a push expansion given to the hub queue: in proportion to its share of
+// the frontier's out-edges, at least one for each non-empty queue.
+__host__ __device__ inline uint32_t bfs_hub_wgs(uint32_t grid, uint32_t n_small, uint32_t n_large,
+                                                unsigned long long deg_large, unsigned long long deg_sum) {
+    if (!n_large) return 0;
+    if (!n_small) return grid;
+    unsigned long long h = deg_sum ? (unsigned long long)grid * deg_large / deg_sum : 1ull;
+    if (h < 1) h = 1;
+    if (h > grid - 1) h = grid - 1;
+    return (uint32_t)h;
+}
+
+// Host-driven superstep: the host's decision written into the control block.
+__global__ void k_bfs_set_ctl(BfsCtl* ctl, int32_t step, uint32_t pull, uint32_t n_small, uint32_t n_large,
+                              uint32_t hub_wgs) {
+    ctl->step = step;
+    ctl->pull = pull;
+    ctl->done = 0;
+    ctl->n_small = n_small;
+    ctl->n_large = n_large;
+    ctl->hub_wgs = hub_wgs;
 }
 
 // ---- superstep bodies (shared by both drivers) -----------------------------------
 
-// Wave-cooperative push over a queue of vertices with 1 <= outdeg <= large; `wave0`
-// and `nwaves` select which waves of the launch take part.
+// Wave-cooperative push over a queue of vertices with 1 <= outdeg <= large.  Of the
+// `nwaves` waves taking part (`wave0` is this wave's index among them), wave w holds queue
+// entries w, w + nwaves, w + 2*nwaves, ... one per lane, so a short queue of high-degree
+// vertices spreads over many waves instead of sharing one.
 __device__ __forceinline__ void push_small(const uint32_t* __restrict__ q, uint32_t qn,
                                            const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                                            int32_t* __restrict__ level, int32_t next, uint32_t wave0, uint32_t nwaves,
                                            uint32_t (*s_beg)[kWave], uint32_t (*s_off)[kWave]) {
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    for (uint32_t base = wave0 * kWave; base < qn; base += nwaves * kWave) {
-        const uint32_t i = base + lane;
+    for (uint64_t r = 0; wave0 + r * kWave * nwaves < qn; ++r) {  // uniform across the wave
+        const uint64_t i = wave0 + (r * kWave + lane) * nwaves;
         uint32_t beg = 0, deg = 0;
         if (i < qn) {
             const uint32_t u = q[i];
@@ -130,14 +147,17 @@ __device__ __forceinline__ void push_small(const uint32_t* __restrict__ q, uint3
     }
 }
 
-// Hubs: vertex i of the queue is split into kLargeSplit contiguous edge ranges.
+// Hubs: each vertex of the queue is split into `split` contiguous edge ranges, one per
+// workgroup item; the `nwg` workgroups taking part stride over the items.
 __device__ __forceinline__ void push_large(const uint32_t* __restrict__ q, uint32_t qn,
                                            const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                                            int32_t* __restrict__ level, int32_t next, uint32_t wg0, uint32_t nwg) {
-    for (uint32_t w = wg0; w < qn * kLargeSplit; w += nwg) {
-        const uint32_t u = q[w / kLargeSplit], part = w % kLargeSplit;
+    uint32_t split = nwg / qn;
+    split = split < kLargeSplit ? kLargeSplit : (split > 4096u ? 4096u : split);
+    for (uint64_t w = wg0; w < (uint64_t)qn * split; w += nwg) {
+        const uint32_t u = q[w / split], part = (uint32_t)(w % split);
         const uint32_t b = row_ptr[u], deg = row_ptr[u + 1] - b;
-        const uint32_t per = (deg + kLargeSplit - 1) / kLargeSplit;
+        const uint32_t per = (deg + split - 1) / split;
         const uint32_t lo = b + min(deg, part * per), hi = b + min(deg, (part + 1) * per);
         for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
             const uint32_t v = col[k];
@@ -179,13 +199,13 @@ __device__ __forceinline__ void compact(uint32_t n, const int32_t* __restrict__ 
                                         unsigned long long* __restrict__ bitmap, uint32_t nwords,
                                         FrontierCounters* __restrict__ ctr) {
     __shared__ uint32_t s_cnt[4][3];
-    __shared__ unsigned long long s_deg[4];
+    __shared__ unsigned long long s_deg[4], s_dlg[4];
     __shared__ uint32_t s_base[2];
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * kCompactTile < n; tile += gridDim.x) {
         unsigned long long msm[kCompactK], mlg[kCompactK];
         uint32_t csm = 0, clg = 0, cnew = 0;
-        unsigned long long dsum = 0;
+        unsigned long long dsum = 0, dlg = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kCompactK; ++k) {
             const uint32_t chunk = tile * (4 * kCompactK) + k * 4 + wv;
@@ -201,28 +221,33 @@ __device__ __forceinline__ void compact(uint32_t n, const int32_t* __restrict__ 
             csm += __popcll(msm[k]);
             clg += __popcll(mlg[k]);
             dsum += deg;
+            dlg += deg > large ? deg : 0u;
         }
         dsum = wave_sum_u64(dsum);
+        dlg = wave_sum_u64(dlg);
         if (lane == 0) {
             s_cnt[wv][0] = csm;
             s_cnt[wv][1] = clg;
             s_cnt[wv][2] = cnew;
             s_deg[wv] = dsum;
+            s_dlg[wv] = dlg;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t tsm = 0, tlg = 0, tnew = 0;
-            unsigned long long tdeg = 0;
+            unsigned long long tdeg = 0, tdlg = 0;
             for (int w = 0; w < 4; ++w) {
                 tsm += s_cnt[w][0];
                 tlg += s_cnt[w][1];
                 tnew += s_cnt[w][2];
                 tdeg += s_deg[w];
+                tdlg += s_dlg[w];
             }
             s_base[0] = tsm ? atomicAdd(&ctr->n_small, tsm) : 0u;
             s_base[1] = tlg ? atomicAdd(&ctr->n_large, tlg) : 0u;
             if (tnew) atomicAdd(&ctr->n_new, tnew);
             if (tdeg) atomicAdd(&ctr->deg_sum, tdeg);
+            if (tdlg) atomicAdd(&ctr->deg_large, tdlg);
         }
         __syncthreads();
         uint32_t osm = s_base[0], olg = s_base[1];
@@ -242,30 +267,7 @@ __device__ __forceinline__ void compact(uint32_t n, const int32_t* __restrict__ 
     }
 }
 
-// ---- host-driven kernels ------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_bfs_push(const uint32_t* __restrict__ q, uint32_t qn,
-                                                  const uint32_t* __restrict__ row_ptr,
-                                                  const uint32_t* __restrict__ col, int32_t* __restrict__ level,
-                                                  int32_t next) {
-    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
-    push_small(q, qn, row_ptr, col, level, next, blockIdx.x * 4 + threadIdx.x / kWave, gridDim.x * 4, s_beg, s_off);
-}
-
-__global__ __launch_bounds__(256) void k_bfs_push_large(const uint32_t* __restrict__ q, uint32_t qn,
-                                                        const uint32_t* __restrict__ row_ptr,
-                                                        const uint32_t* __restrict__ col,
-                                                        int32_t* __restrict__ level, int32_t next) {
-    push_large(q, qn, row_ptr, col, level, next, blockIdx.x, gridDim.x);
-}
-
-__global__ __launch_bounds__(256) void k_bfs_pull(uint32_t n, const uint32_t* __restrict__ cptr,
-                                                  const uint32_t* __restrict__ csrc,
-                                                  const unsigned long long* __restrict__ bitmap,
-                                                  int32_t* __restrict__ level, int32_t next,
-                                                  FrontierCounters* __restrict__ ctr) {
-    pull_all(n, cptr, csrc, bitmap, level, next, ctr);
-}
+// ---- kernels (both drivers) ---------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
                                                      const uint32_t* __restrict__ row_ptr, uint32_t large,
@@ -275,10 +277,9 @@ __global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* 
     compact(n, level, want, row_ptr, large, q_small, q_large, bitmap, nwords, ctr);
 }
 
-// ---- device-driven kernels -----------------------------------------------------------
-
-// Expansion of the current superstep: pull over every vertex, or push where the first
-// kLargeGroups workgroups take the hub queue and the rest the wave-cooperative queue.
+// Expansion of the current superstep as the control block describes it: pull over every
+// vertex, or push where the first hub_wgs workgroups take the hub queue and the rest the
+// wave-cooperative queue.
 __global__ __launch_bounds__(256) void k_bfs_expand_dev(const BfsCtl* __restrict__ ctl,
                                                         const uint32_t* __restrict__ q_small,
                                                         const uint32_t* __restrict__ q_large,
@@ -296,16 +297,16 @@ __global__ __launch_bounds__(256) void k_bfs_expand_dev(const BfsCtl* __restrict
         pull_all(n, cptr, csrc, bitmap, level, next, ctr);
         return;
     }
-    if (blockIdx.x < kLargeGroups) {
-        const uint32_t nl = ctl->n_large;
-        if (nl) push_large(q_large, nl, row_ptr, col, level, next, blockIdx.x, kLargeGroups);
+    const uint32_t hub = ctl->hub_wgs;
+    if (blockIdx.x < hub) {
+        push_large(q_large, ctl->n_large, row_ptr, col, level, next, blockIdx.x, hub);
         return;
     }
-    const uint32_t wg = blockIdx.x - kLargeGroups, nwg = gridDim.x - kLargeGroups;
+    const uint32_t wg = blockIdx.x - hub, nwg = gridDim.x - hub;
     push_small(q_small, ctl->n_small, row_ptr, col, level, next, wg * 4 + threadIdx.x / kWave, nwg * 4, s_beg, s_off);
 }
 
-// Compaction of the current superstep (device-driven).
+// Compaction of the current superstep.
 __global__ __launch_bounds__(256) void k_bfs_compact_dev(const BfsCtl* __restrict__ ctl, uint32_t n,
                                                          const int32_t* __restrict__ level,
                                                          const uint32_t* __restrict__ row_ptr, uint32_t large,
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) void k_bfs_compact_dev(const BfsCtl* __restric
 // so the decision needs no fence.  Decides the next superstep into the control block.
 __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, uint32_t n, uint32_t nwords,
                                                     FrontierCounters* __restrict__ ctr,
-                                                    const float* __restrict__ policy) {
+                                                    const float* __restrict__ policy, uint32_t grid) {
     __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
     __shared__ float s_cost[2];
     if (ctl->done) return;
@@ -329,6 +330,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
     const uint32_t n_large = ctr->n_large;
     const unsigned long long deg_sum = ctr->deg_sum;
     const unsigned long long probes = ctr->probes;
+    const unsigned long long deg_large = ctr->deg_large;
     BfsCtl c = *ctl;
     // byte model of the superstep just finished (same as the host-driven driver)
     const unsigned long long kb = c.pull ? 4ull * n + 4ull * (n + 1) + 8ull * probes + 4ull * n_new
@@ -347,6 +349,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
         c.reached_edges += deg_sum;
         c.n_small = n_small;
         c.n_large = n_large;
+        c.hub_wgs = bfs_hub_wgs(grid, n_small, n_large, deg_large, deg_sum);
     }
     bool pull = c.pull != 0;
     if (!c.done) {
@@ -391,15 +394,22 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
         ctr->n_new = 0;
         ctr->deg_sum = 0;
         ctr->probes = 0;
+        ctr->deg_large = 0;
     }
 }
 
 // First decision (superstep 0) from the seeded counters; one workgroup.
 __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ ctl, FrontierCounters* __restrict__ ctr,
-                                                          const float* __restrict__ policy) {
+                                                          const float* __restrict__ policy, uint32_t n,
+                                                          unsigned long long m, uint32_t mode, uint32_t grid) {
     __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
     __shared__ float s_cost[2];
-    BfsCtl c = *ctl;
+    BfsCtl c{};
+    c.mode = mode;
+    c.n = n;
+    c.m = m;
+    c.n_u = n;
+    c.m_u = m;
     c.step = 0;
     c.n_f = ctr->n_new;
     c.m_f = ctr->deg_sum;
@@ -409,6 +419,7 @@ __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ c
     c.reached_edges = c.m_f;
     c.n_small = ctr->n_small;
     c.n_large = ctr->n_large;
+    c.hub_wgs = bfs_hub_wgs(grid, c.n_small, c.n_large, ctr->deg_large, ctr->deg_sum);
     bool pull = false;
     if (c.mode == 2) {
         pull = true;
@@ -442,7 +453,17 @@ __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ c
         ctr->n_new = 0;
         ctr->deg_sum = 0;
         ctr->probes = 0;
+        ctr->deg_large = 0;
     }
+}
+
+// Degree threshold of the hub queue and workgroups of the expansion: the caller's value,
+// else the handle's autotuned one, else the default.
+static uint32_t bfs_large(const medusa_graph* g, const medusa_opts& o) {
+    return o.large_degree ? o.large_degree : (g->tuned.bfs_large ? g->tuned.bfs_large : kLargeDefault);
+}
+static uint32_t bfs_grid(const medusa_graph* g, const medusa_opts& o) {
+    return o.grid ? o.grid : (g->tuned.bfs_grid ? g->tuned.bfs_grid : kExpandGrid);
 }
 
 static uint32_t bfs_mode(const medusa_opts& o) {
@@ -452,15 +473,34 @@ static uint32_t bfs_mode(const medusa_opts& o) {
     return 0;
 }
 
+// Seeding: every source at level 0 (one source by value, several from g->q_aux), then the
+// compaction of level 0 builds the first frontier's bitmap, queues and counters exactly
+// as a superstep would.
+static void bfs_seed(medusa_graph* g, uint32_t nsrc, uint32_t single, uint32_t large) {
+    const uint32_t n = g->n, nwords = (n + 63) / 64;
+    hipStream_t s = g->stream;
+    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
+    hipLaunchKernelGGL(k_bfs_reset, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->level.p,
+                       nsrc == 1 ? single : 0xFFFFFFFFu, ctr);
+    MDS_LAUNCHED();
+    if (nsrc > 1) {
+        hipLaunchKernelGGL(k_bfs_mark, dim3(grid_for(nsrc, 256, 1024)), dim3(256), 0, s, (const uint32_t*)g->q_aux.p,
+                           nsrc, g->level.p);
+        MDS_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_bfs_compact, dim3(grid_for(n, kCompactTile, 4096)), dim3(256), 0, s, n,
+                       (const int32_t*)g->level.p, 0, g->row_ptr.p, large, g->q_small.p, g->q_large.p, g->bitmap.p,
+                       nwords, ctr);
+    MDS_LAUNCHED();
+}
+
 // Device-driven run: seed, first decision, then replays of a captured batch of supersteps.
-static void bfs_run_device(medusa_graph* g, uint32_t src, int32_t* out, const medusa_opts& o, medusa_stats& st) {
+static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_t* out, const medusa_opts& o, medusa_stats& st) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const uint32_t nwords = (n + 63) / 64;
-    const uint32_t large = o.large_degree ? o.large_degree : kLargeDefault;
+    const uint32_t large = bfs_large(g, o), grid = bfs_grid(g, o);
     hipStream_t s = g->stream;
-    const uint32_t deg_src = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
-    const uint32_t src_class = deg_src == 0 ? 0u : (deg_src <= large ? 1u : 2u);
     const uint32_t mode = bfs_mode(o);
     const float* policy = mode == 3 ? policy_device_params(o.policy) : nullptr;
     g->bfs_ctl.ensure(1);
@@ -470,18 +510,19 @@ static void bfs_run_device(medusa_graph* g, uint32_t src, int32_t* out, const me
 
     // capture (once per graph, threshold and policy buffer) the batch of supersteps
     GraphCache& gc = g->bfs_graph;
-    const uintptr_t key[4] = {(uintptr_t)g->level.p, (uintptr_t)policy, large, (uintptr_t)g->bitmap.p};
+    const uintptr_t key[4] = {(uintptr_t)g->level.p, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
+                              (uintptr_t)g->bitmap.p};
     if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (uint32_t k = 0; k < kBfsBatch; ++k) {
-            hipLaunchKernelGGL(k_bfs_expand_dev, dim3(kExpandGrid), dim3(256), 0, s, (const BfsCtl*)ctl,
+            hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)ctl,
                                (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
                                g->cptr.p, g->csrc.p, (const unsigned long long*)g->bitmap.p, g->level.p, n, ctr);
             hipLaunchKernelGGL(k_bfs_compact_dev, dim3(grid_for(n, kCompactTile, 4096)), dim3(256), 0, s,
                                (const BfsCtl*)ctl, n, (const int32_t*)g->level.p, g->row_ptr.p, large, g->q_small.p,
                                g->q_large.p, g->bitmap.p, nwords, ctr);
-            hipLaunchKernelGGL(k_bfs_decide, dim3(1), dim3(256), 0, s, ctl, n, nwords, ctr, policy);
+            hipLaunchKernelGGL(k_bfs_decide, dim3(1), dim3(256), 0, s, ctl, n, nwords, ctr, policy, grid);
         }
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
@@ -492,13 +533,9 @@ static void bfs_run_device(medusa_graph* g, uint32_t src, int32_t* out, const me
     }
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->level.p, src, g->q_small.p,
-                       g->q_large.p, src_class, g->bitmap.p, nwords);
-    MDS_LAUNCHED();
-    hipLaunchKernelGGL(k_bfs_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, deg_src, src_class,
-                       mode);
-    MDS_LAUNCHED();
-    hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, ctl, ctr, policy);
+    bfs_seed(g, nsrc, single, large);
+    hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, ctl, ctr, policy, n, (unsigned long long)m,
+                       mode, grid);
     MDS_LAUNCHED();
     BfsCtl h{};
     for (uint32_t batch = 0;; ++batch) {
@@ -520,37 +557,37 @@ static void bfs_run_device(medusa_graph* g, uint32_t src, int32_t* out, const me
     st.edges_traversed = h.reached_edges;
     st.reached = (uint32_t)h.reached;
     st.kernel_bytes = h.kernel_bytes;
-    st.alg_bytes = h.alg_bytes + 4ull * n + nwords * 8ull;
+    st.alg_bytes = h.alg_bytes + 8ull * n + nwords * 8ull + 8ull * nsrc;
     copy_out(g, out, g->level.p, (size_t)n * 4);
 }
 
-// Host-driven run: counters read back after every superstep (timing and tracing).
-static void bfs_run_host(medusa_graph* g, uint32_t src, int32_t* out, const medusa_opts& o, medusa_stats& st) {
+// Host-driven run: the same expansion and compaction kernels, with the host deciding each
+// superstep from counters read back after it (timing and tracing).
+static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_t* out, const medusa_opts& o,
+                         medusa_stats& st) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const uint32_t nwords = (n + 63) / 64;
-    const uint32_t large = o.large_degree ? o.large_degree : kLargeDefault;
+    const uint32_t large = bfs_large(g, o), grid = bfs_grid(g, o);
     hipStream_t s = g->stream;
-    const uint32_t deg_src = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
-    const uint32_t src_class = deg_src == 0 ? 0u : (deg_src <= large ? 1u : 2u);
     const bool timing = o.timing != 0;
     FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
+    g->bfs_ctl.ensure(1);
+    BfsCtl* ctl = reinterpret_cast<BfsCtl*>(g->bfs_ctl.p);
     g->trace.clear();
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    hipLaunchKernelGGL(k_bfs_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->level.p, src, g->q_small.p,
-                       g->q_large.p, src_class, g->bitmap.p, nwords);
-    MDS_LAUNCHED();
+    bfs_seed(g, nsrc, single, large);
+    FrontierCounters c = read_counters<FrontierCounters>(g);
 
     Frontier f{};
     f.n = n;
     f.m = m;
-    f.n_f = 1;
-    f.m_f = deg_src;
-    f.n_u = n - 1;
-    f.m_u = m - deg_src;
-    uint32_t n_small = src_class == 1, n_large = src_class == 2;
-    uint64_t reached_edges = deg_src, reached = 1, bytes = 4ull * n + nwords * 8ull;
+    f.n_f = c.n_new;
+    f.m_f = c.deg_sum;
+    f.n_u = n - c.n_new;
+    f.m_u = m - c.deg_sum;
+    uint64_t reached_edges = c.deg_sum, reached = c.n_new, bytes = 8ull * n + nwords * 8ull + 8ull * nsrc;
     bool pull = false;
     Sched sched(o, Sched::kBfs);
     if (timing) ensure_events(g, 2);
@@ -559,33 +596,23 @@ static void bfs_run_host(medusa_graph* g, uint32_t src, int32_t* out, const medu
         if (o.direction == MEDUSA_DIR_PUSH) pull = false;
         else if (o.direction == MEDUSA_DIR_PULL) pull = true;
         else pull = sched.bfs_pull(f, pull);
-        if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
+        const uint32_t hub = bfs_hub_wgs(grid, c.n_small, c.n_large, c.deg_large, c.deg_sum);
+        hipLaunchKernelGGL(k_bfs_set_ctl, dim3(1), dim3(1), 0, s, ctl, (int32_t)t, pull ? 1u : 0u, c.n_small,
+                           c.n_large, hub);
+        MDS_LAUNCHED();
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, ctr);
         MDS_LAUNCHED();
-        if (!pull) {
-            if (n_small) {
-                const uint32_t waves = (n_small + kWave - 1) / kWave;
-                hipLaunchKernelGGL(k_bfs_push, dim3(grid_for(waves, 4, 16384)), dim3(256), 0, s, g->q_small.p, n_small,
-                                   g->row_ptr.p, g->col.p, g->level.p, (int32_t)(t + 1));
-                MDS_LAUNCHED();
-            }
-            if (n_large) {
-                hipLaunchKernelGGL(k_bfs_push_large, dim3(grid_for((uint64_t)n_large * kLargeSplit, 1, 16384)),
-                                   dim3(256), 0, s, g->q_large.p, n_large, g->row_ptr.p, g->col.p, g->level.p,
-                                   (int32_t)(t + 1));
-                MDS_LAUNCHED();
-            }
-        } else {
-            hipLaunchKernelGGL(k_bfs_pull, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, g->cptr.p, g->csrc.p,
-                               (const unsigned long long*)g->bitmap.p, g->level.p, (int32_t)(t + 1), ctr);
-            MDS_LAUNCHED();
-        }
+        if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
+        hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)ctl,
+                           (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
+                           g->cptr.p, g->csrc.p, (const unsigned long long*)g->bitmap.p, g->level.p, n, ctr);
+        MDS_LAUNCHED();
         if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
         hipLaunchKernelGGL(k_bfs_compact, dim3(grid_for(n, kCompactTile, 4096)), dim3(256), 0, s, n,
                            (const int32_t*)g->level.p, (int32_t)(t + 1), g->row_ptr.p, large, g->q_small.p,
                            g->q_large.p, g->bitmap.p, nwords, ctr);
         MDS_LAUNCHED();
-        const FrontierCounters c = read_counters<FrontierCounters>(g);
+        c = read_counters<FrontierCounters>(g);
         double kms = 0;
         if (timing) {
             float ms = 0;
@@ -613,8 +640,6 @@ static void bfs_run_host(medusa_graph* g, uint32_t src, int32_t* out, const medu
         f.m_f = c.deg_sum;
         f.n_u -= c.n_new;
         f.m_u -= c.deg_sum;
-        n_small = c.n_small;
-        n_large = c.n_large;
         reached += c.n_new;
         reached_edges += c.deg_sum;
     }
@@ -629,27 +654,41 @@ static void bfs_run_host(medusa_graph* g, uint32_t src, int32_t* out, const medu
     copy_out(g, out, g->level.p, (size_t)n * 4);
 }
 
+static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, int32_t* levels,
+                      const medusa_opts* opts, medusa_stats* stats) {
+    enter(g);
+    MDS_CHECK(levels, MEDUSA_ERR_ARG, "bfs: null output");
+    MDS_CHECK(sources && count >= 1, MEDUSA_ERR_ARG, "bfs: need at least one source");
+    const uint32_t n = g->n;
+    for (uint32_t i = 0; i < count; ++i)
+        MDS_CHECK(sources[i] < n, MEDUSA_ERR_ARG, "bfs: source %u out of range (n=%u)", sources[i], n);
+    medusa_opts o{};
+    if (opts) o = *opts;
+    MDS_CHECK(o.direction >= 0 && o.direction <= 2, MEDUSA_ERR_ARG, "bfs: bad direction %d", o.direction);
+    MDS_CHECK(o.grid == 0 || (o.grid >= 2 && o.grid <= 65535), MEDUSA_ERR_ARG, "bfs: grid must be 0 or 2..65535");
+    g->level.ensure(n);
+    g->q_small.ensure(n);
+    g->q_large.ensure(n);
+    g->q_aux.ensure(std::max(n, count));
+    g->bitmap.ensure((n + 63) / 64);
+    if (count > 1)
+        MDS_HIP(hipMemcpyAsync(g->q_aux.p, sources, (size_t)count * 4, hipMemcpyHostToDevice, g->stream));
+    medusa_stats st{};
+    if (o.timing || o.trace) bfs_run_host(g, count, sources[0], levels, o, st);
+    else bfs_run_device(g, count, sources[0], levels, o, st);
+    if (stats) *stats = st;
+}
+
 }  // namespace medusa
 
 using namespace medusa;
 
 extern "C" int medusa_bfs(medusa_graph* g, uint32_t source, int32_t* levels, const medusa_opts* opts,
                           medusa_stats* stats) {
-    return guard([&] {
-        enter(g);
-        MDS_CHECK(levels, MEDUSA_ERR_ARG, "bfs: null output");
-        MDS_CHECK(source < g->n, MEDUSA_ERR_ARG, "bfs: source %u out of range (n=%u)", source, g->n);
-        medusa_opts o{};
-        if (opts) o = *opts;
-        MDS_CHECK(o.direction >= 0 && o.direction <= 2, MEDUSA_ERR_ARG, "bfs: bad direction %d", o.direction);
-        const uint32_t n = g->n;
-        g->level.ensure(n);
-        g->q_small.ensure(n);
-        g->q_large.ensure(n);
-        g->bitmap.ensure((n + 63) / 64);
-        medusa_stats st{};
-        if (o.timing || o.trace) bfs_run_host(g, source, levels, o, st);
-        else bfs_run_device(g, source, levels, o, st);
-        if (stats) *stats = st;
-    });
+    return guard([&] { bfs_entry(g, &source, 1, levels, opts, stats); });
+}
+
+extern "C" int medusa_bfs_multi(medusa_graph* g, const uint32_t* sources, uint32_t count, int32_t* levels,
+                                const medusa_opts* opts, medusa_stats* stats) {
+    return guard([&] { bfs_entry(g, sources, count, levels, opts, stats); });
 }

@@ -91,8 +91,9 @@ struct FrontierCounters {
     uint32_t n_large;    // frontier vertices with outdeg > threshold (workgroup queue)
     uint32_t n_new;      // vertices that joined the frontier (any outdeg)
     uint32_t pad;
-    unsigned long long deg_sum;  // sum of their out-degrees
-    unsigned long long probes;   // in-edges examined by a pull superstep
+    unsigned long long deg_sum;    // sum of their out-degrees
+    unsigned long long probes;     // in-edges examined by a pull superstep
+    unsigned long long deg_large;  // out-degree sum of the workgroup (hub) queue
 };
 
 // One contiguous run of PageRank vertices handled with the same lane mapping.
@@ -103,7 +104,17 @@ struct PrRun {
     uint32_t wg_begin;    // first workgroup of this run in the launch
     uint32_t deg;         // kPrLane: the in-degree shared by every vertex of the run (1..64)
     uint32_t ell_stride;  // kPrLane: column stride of the run's ELL slab (count rounded up to 64)
+    uint32_t part;        // packed graph the run belongs to (0 for a single graph)
+    uint32_t pad;
     uint64_t ell_base;    // kPrLane: offset of the slab in the ELL array
+};
+// A canonical block of the dangling-mass sum: up to 4096 consecutive internal ids.
+struct PrBlock {
+    uint32_t begin, len;
+};
+// Per packed graph: its dangling blocks and vertex count.
+struct PrPart {
+    uint32_t first_block, nblocks, n, pad;
 };
 constexpr uint32_t kPrLane = 0;  // one lane per vertex, in-degree 1..64, ELL slab
 constexpr uint32_t kPrWave = 1;  // one wave per vertex, in-degree > 64
@@ -111,15 +122,21 @@ constexpr uint32_t kPrZero = 2;  // no in-edges
 constexpr uint32_t kPrLaneMax = 64;
 
 // In-degree-sorted pull layout for PageRank (built on first use).
-// Internal id order: non-dangling vertices first, then dangling (outdeg 0);
-// within each, in-degree descending, ties by original id.  Every run of equal
-// in-degree (<= 64) is one contiguous id range whose in-edge sources are stored
-// as an ELL slab, column-major and 64-aligned, so the k-th source of 64
+// Internal ids keep every packed graph's id range (a single graph is one range);
+// inside a range: non-dangling vertices first, then dangling (outdeg 0); within
+// each, in-degree descending, ties by original id.  Every run of equal in-degree
+// (<= 64) is one contiguous id range inside one graph whose in-edge sources are
+// stored as an ELL slab, column-major and 64-aligned, so the k-th source of 64
 // consecutive vertices is one 256-byte load; writes of rank/contrib are
-// contiguous, and the dangling vertices form the tail [dangle_begin, n).
+// contiguous, and each graph's dangling vertices form the tail of its range.
 struct PrLayout {
-    uint32_t dangle_begin = 0;
+    uint32_t n_dangling = 0;  // over all packed graphs
     std::vector<PrRun> runs;
+    std::vector<PrBlock> blocks;
+    std::vector<PrPart> parts;
+    DBuf<PrBlock> d_blocks;
+    DBuf<PrPart> d_parts;
+    DBuf<uint32_t> d_part_begin;  // G+1 range starts (internal ids = original ids at range level)
     uint32_t total_wgs = 0;
     DBuf<PrRun> d_runs;
     DBuf<uint32_t> ell;     // ELL slabs of the kPrLane runs
@@ -133,7 +150,6 @@ struct PrLayout {
     // work
     DBuf<float> rank, rank_prev, contrib0, contrib1, msg, base, unperm;
     DBuf<float> partials;   // (s, c) pairs of the dangling-sum blocks
-    DBuf<uint32_t> counter;
     DBuf<double> resid;
     // captured superstep sequence, reused while (steps, variant, damping) stay the same
     hipGraphExec_t exec = nullptr;
@@ -181,6 +197,13 @@ struct medusa_graph {
     std::vector<uint32_t> h_row_ptr, h_col, h_w, h_cptr, h_csrc, h_cw, h_slot;
     // device CSR / CSC
     medusa::DBuf<uint32_t> row_ptr, col, w, cptr, csrc, cw, slot;
+    // packed batch of independent graphs: graph k owns vertices [parts[k], parts[k+1])
+    // and no edge leaves its range; a plain graph is the single range {0, n}
+    std::vector<uint32_t> parts;
+    // launch parameters chosen by medusa_autotune (0 = engine default)
+    struct {
+        uint32_t bfs_large = 0, bfs_grid = 0, mbfs_large = 0, mbfs_grid = 0;
+    } tuned;
     // traversal work buffers
     medusa::DBuf<int32_t> level;
     medusa::DBuf<uint32_t> dist, q_small, q_large, q_aux, stamp;

@@ -63,11 +63,14 @@ void copy_out(medusa_graph* g, void* dst, const void* src, size_t bytes) {
     MDS_HIP(hipStreamSynchronize(g->stream));
 }
 
-// Internal id order for PageRank: (dangling last, in-degree descending, original id).
+// Internal id order for PageRank: inside every packed graph's id range,
+// (dangling last, in-degree descending, original id).
 void build_pr_layout(medusa_graph* g) {
     if (g->pr) return;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
+    const std::vector<uint32_t>& pb = g->parts;
+    const uint32_t G = (uint32_t)pb.size() - 1;
     auto L = std::make_unique<PrLayout>();
     std::vector<uint32_t> perm(n), iperm(n), indeg(n), outdeg(n);
     for (uint32_t v = 0; v < n; ++v) {
@@ -75,12 +78,13 @@ void build_pr_layout(medusa_graph* g) {
         outdeg[v] = g->h_row_ptr[v + 1] - g->h_row_ptr[v];
     }
     std::iota(perm.begin(), perm.end(), 0u);
-    std::sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
-        const bool da = outdeg[a] == 0, db = outdeg[b] == 0;
-        if (da != db) return db;  // non-dangling first
-        if (indeg[a] != indeg[b]) return indeg[a] > indeg[b];
-        return a < b;
-    });
+    for (uint32_t k = 0; k < G; ++k)
+        std::sort(perm.begin() + pb[k], perm.begin() + pb[k + 1], [&](uint32_t a, uint32_t b) {
+            const bool da = outdeg[a] == 0, db = outdeg[b] == 0;
+            if (da != db) return db;  // non-dangling first
+            if (indeg[a] != indeg[b]) return indeg[a] > indeg[b];
+            return a < b;
+        });
     for (uint32_t p = 0; p < n; ++p) iperm[perm[p]] = p;
     std::vector<uint32_t> cptr(n + 1, 0), crow(m), od(n), optr(n + 1, 0), oslot(m);
     for (uint32_t p = 0; p < n; ++p) {
@@ -108,13 +112,19 @@ void build_pr_layout(medusa_graph* g) {
             }
         }
     });
-    uint32_t dangle_begin = n;
-    for (uint32_t p = 0; p < n; ++p)
-        if (od[p] == 0) {
-            dangle_begin = p;
-            break;
+    // dangling tail of every graph, cut into canonical blocks of 4096
+    L->n_dangling = 0;
+    for (uint32_t k = 0; k < G; ++k) {
+        uint32_t db = pb[k];
+        while (db < pb[k + 1] && od[db] != 0) ++db;
+        PrPart P{(uint32_t)L->blocks.size(), 0, pb[k + 1] - pb[k], 0};
+        for (uint32_t b = db; b < pb[k + 1]; b += 4096) {
+            L->blocks.push_back({b, std::min(4096u, pb[k + 1] - b)});
+            ++P.nblocks;
         }
-    L->dangle_begin = dangle_begin;
+        L->n_dangling += pb[k + 1] - db;
+        L->parts.push_back(P);
+    }
     // runs: equal in-degree for 1..64 (lane per vertex), >64 (wave per vertex), 0
     auto kind_of = [](uint32_t deg) -> uint32_t {
         if (deg == 0) return kPrZero;
@@ -125,12 +135,14 @@ void build_pr_layout(medusa_graph* g) {
     L->runs.clear();
     uint32_t wg = 0;
     uint64_t ell_size = 0;
+    uint32_t part = 0;
     for (uint32_t p = 0; p < n;) {
+        while (p >= pb[part + 1]) ++part;
         const uint32_t deg = cptr[p + 1] - cptr[p];
         const uint32_t kind = kind_of(deg);
         const bool dang = od[p] == 0;
         uint32_t q = p + 1;
-        while (q < n && kind_of(cptr[q + 1] - cptr[q]) == kind && (od[q] == 0) == dang &&
+        while (q < pb[part + 1] && kind_of(cptr[q + 1] - cptr[q]) == kind && (od[q] == 0) == dang &&
                (kind != kPrLane || cptr[q + 1] - cptr[q] == deg))
             ++q;
         PrRun r{};
@@ -138,6 +150,7 @@ void build_pr_layout(medusa_graph* g) {
         r.count = q - p;
         r.kind = kind;
         r.wg_begin = wg;
+        r.part = part;
         if (kind == kPrLane) {
             r.deg = deg;
             r.ell_stride = (r.count + 63u) & ~63u;
@@ -161,6 +174,9 @@ void build_pr_layout(medusa_graph* g) {
     hipStream_t s = g->stream;
     L->ell.upload(ell.data(), ell.size(), s);
     L->d_runs.upload(L->runs.data(), L->runs.size(), s);
+    if (!L->blocks.empty()) L->d_blocks.upload(L->blocks.data(), L->blocks.size(), s);
+    L->d_parts.upload(L->parts.data(), L->parts.size(), s);
+    L->d_part_begin.upload(pb.data(), pb.size(), s);
     L->perm.upload(perm.data(), n, s);
     L->iperm.upload(iperm.data(), n, s);
     L->cptr.upload(cptr.data(), (size_t)n + 1, s);
@@ -218,6 +234,7 @@ extern "C" int medusa_graph_create(uint32_t n, uint64_t m, const uint32_t* row_p
         MDS_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
         MDS_HIP(hipEventCreate(&g->ev_begin));
         MDS_HIP(hipEventCreate(&g->ev_end));
+        g->parts = {0u, n};
         g->h_row_ptr.assign(row_ptr, row_ptr + n + 1);
         g->h_col.assign(col, col + m);
         if (weight) g->h_w.assign(weight, weight + m);
@@ -251,6 +268,35 @@ extern "C" int medusa_graph_create(uint32_t n, uint64_t m, const uint32_t* row_p
         g->h_ctr_raw.ensure(kCounterWords);
         MDS_HIP(hipStreamSynchronize(s));
         *out = g.release();
+    });
+}
+
+extern "C" int medusa_graph_set_batch(medusa_graph* g, uint32_t count, const uint32_t* graph_begin) {
+    return guard([&] {
+        enter(g);
+        MDS_CHECK(count >= 1 && graph_begin, MEDUSA_ERR_ARG, "set_batch: need count >= 1 and graph_begin");
+        MDS_CHECK(graph_begin[0] == 0 && graph_begin[count] == g->n, MEDUSA_ERR_ARG,
+                  "set_batch: graph_begin must start at 0 and end at n=%u", g->n);
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t b = graph_begin[k], e = graph_begin[k + 1];
+            MDS_CHECK(b < e, MEDUSA_ERR_ARG, "set_batch: graph %u is empty or out of order", k);
+            for (uint64_t x = g->h_row_ptr[b]; x < g->h_row_ptr[e]; ++x)
+                MDS_CHECK(g->h_col[x] >= b && g->h_col[x] < e, MEDUSA_ERR_ARG,
+                          "set_batch: edge %llu leaves graph %u", (unsigned long long)x, k);
+        }
+        g->parts.assign(graph_begin, graph_begin + count + 1);
+        g->pr.reset();  // the PageRank layout depends on the graph ranges
+    });
+}
+
+extern "C" int medusa_graph_batch(const medusa_graph* g, uint32_t* count, uint32_t* graph_begin,
+                                  uint32_t capacity) {
+    return guard([&] {
+        MDS_CHECK(g && count, MEDUSA_ERR_ARG, "null argument");
+        *count = (uint32_t)g->parts.size() - 1;
+        MDS_CHECK(graph_begin || capacity == 0, MEDUSA_ERR_ARG, "null graph_begin");
+        const size_t k = std::min<size_t>(capacity, g->parts.size());
+        std::copy(g->parts.begin(), g->parts.begin() + k, graph_begin);
     });
 }
 

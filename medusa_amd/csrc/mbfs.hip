@@ -7,36 +7,63 @@
 //
 // Superstep:
 //   push: active vertices (frontier mask != 0) OR their mask into next[v] of every
-//         out-neighbour that still lacks some of those bits (64-bit atomicOr);
+//         out-neighbour that still lacks some of those bits (64-bit atomicOr); vertices
+//         up to `large` out-degree go through the wave-cooperative queue, hubs above it
+//         are split over whole workgroups (the threshold is autotunable);
 //   pull: every vertex not yet visited by all sources ORs the frontier masks of its
 //         in-neighbours, stopping early once every missing bit has been found;
 //   update: new = next & ~visited; visited |= new; frontier = new; level[s][v] =
-//         step+1 for each new bit s; the active queue and counters are rebuilt with
-//         ballot + in-workgroup prefix scan.
+//         step+1 for each new bit s; the two active queues and the counters are rebuilt
+//         with ballot + in-workgroup prefix scan.  Seeding is the same update at level 0
+//         after the sources' bits are placed in next[].
+//
+// Drivers as for BFS: device-driven (a one-workgroup decide kernel after each update
+// writes the next superstep into a control block; kMbfsBatch supersteps are captured as
+// one hipGraph and replayed until done) and host-driven (timing or tracing) over the
+// same kernels.
 
 #include <algorithm>
+#include <cstring>
 
 #include "engine.h"
 #include "sched.h"
 
 namespace medusa {
 
+constexpr uint32_t kMbfsLargeDefault = 1024;
+constexpr uint32_t kMbfsGridDefault = 2048;
+constexpr uint32_t kMbfsBatch = 8;
+constexpr uint32_t kMbfsSplit = 8;  // minimum workgroup items per hub
+
 struct MbfsCounters {
-    uint32_t n_active;
+    uint32_t n_small;    // active vertices with 1 <= outdeg <= large
+    uint32_t n_large;    // active vertices with outdeg > large
+    uint32_t n_active;   // all active vertices
     uint32_t saturated;  // a level >= 255 was clamped to 0xFF
     unsigned long long active_deg;
     unsigned long long new_bits;
     unsigned long long reached_edges;  // sum over new (source, vertex) pairs of outdeg(vertex)
     unsigned long long probes;
+    unsigned long long deg_large;      // out-degree sum of the hub queue
+};
+
+struct MbfsCtl {
+    int32_t step;
+    uint32_t pull, done, n_small, n_large, hub_wgs, mode, saturated;
+    uint32_t supersteps, push_steps, pull_steps, n_active;
+    unsigned long long n, m, n_f, m_f, n_u, m_u, reached, reached_edges, kernel_bytes, alg_bytes, full;
 };
 
 __global__ void k_mbfs_zero(MbfsCounters* c) {
+    c->n_small = 0;
+    c->n_large = 0;
     c->n_active = 0;
     c->saturated = 0;
     c->active_deg = 0;
     c->new_bits = 0;
     c->reached_edges = 0;
     c->probes = 0;
+    c->deg_large = 0;
 }
 
 __global__ void k_mbfs_fill(uint32_t n, uint64_t level_words, uint32_t* __restrict__ level32,
@@ -51,43 +78,37 @@ __global__ void k_mbfs_fill(uint32_t n, uint64_t level_words, uint32_t* __restri
     }
 }
 
-// Seeds the k sources (one thread: k <= 64) and the first active queue.
-__global__ void k_mbfs_seed(uint32_t n, const uint32_t* __restrict__ srcs, uint32_t k, uint8_t* __restrict__ level,
-                            unsigned long long* __restrict__ visited, unsigned long long* __restrict__ frontier,
-                            const uint32_t* __restrict__ row_ptr, uint32_t* __restrict__ active,
-                            MbfsCounters* __restrict__ ctr) {
+// Places the k source bits (k <= 64, one thread) into next[]; the level-0 update does the rest.
+__global__ void k_mbfs_seed(const uint32_t* __restrict__ srcs, uint32_t k, unsigned long long* __restrict__ next) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    uint32_t na = 0;
-    unsigned long long deg = 0, re = 0;
-    for (uint32_t s = 0; s < k; ++s) {
-        const uint32_t v = srcs[s];
-        if (frontier[v] == 0) {
-            active[na++] = v;
-            deg += row_ptr[v + 1] - row_ptr[v];
-        }
-        frontier[v] |= 1ull << s;
-        visited[v] |= 1ull << s;
-        level[(size_t)s * n + v] = 0;
-        re += row_ptr[v + 1] - row_ptr[v];
-    }
-    ctr->n_active = na;
-    ctr->active_deg = deg;
-    ctr->new_bits = k;
-    ctr->reached_edges = re;
+    for (uint32_t s = 0; s < k; ++s) next[srcs[s]] |= 1ull << s;
 }
 
-__global__ __launch_bounds__(256) void k_mbfs_push(const uint32_t* __restrict__ q, uint32_t qn,
-                                                   const uint32_t* __restrict__ row_ptr,
-                                                   const uint32_t* __restrict__ col,
-                                                   const unsigned long long* __restrict__ frontier,
-                                                   const unsigned long long* __restrict__ visited,
-                                                   unsigned long long* __restrict__ next) {
+// Workgroups of a push expansion given to the hub queue (as for BFS).
+__host__ __device__ inline uint32_t mbfs_hub_wgs(uint32_t grid, uint32_t n_small, uint32_t n_large,
+                                                 unsigned long long deg_large, unsigned long long deg_sum) {
+    if (!n_large) return 0;
+    if (!n_small) return grid;
+    unsigned long long h = deg_sum ? (unsigned long long)grid * deg_large / deg_sum : 1ull;
+    if (h < 1) h = 1;
+    if (h > grid - 1) h = grid - 1;
+    return (uint32_t)h;
+}
+
+// ---- superstep bodies ---------------------------------------------------------------
+
+// Wave-cooperative push; wave w of the nwaves taking part holds queue entries
+// w, w + nwaves, ... one per lane.
+__device__ __forceinline__ void mbfs_push_small(const uint32_t* __restrict__ q, uint32_t qn,
+                                                const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                                const unsigned long long* __restrict__ frontier,
+                                                const unsigned long long* __restrict__ visited,
+                                                unsigned long long* __restrict__ next, uint32_t wave0, uint32_t nwaves) {
     __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
     __shared__ unsigned long long s_mask[4][kWave];
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const uint32_t total_waves = gridDim.x * 4;
-    for (uint32_t base = (blockIdx.x * 4 + wv) * kWave; base < qn; base += total_waves * kWave) {
-        const uint32_t i = base + lane;
+    for (uint64_t r = 0; wave0 + r * kWave * nwaves < qn; ++r) {  // uniform across the wave
+        const uint64_t i = wave0 + (r * kWave + lane) * nwaves;
         uint32_t beg = 0, deg = 0;
         unsigned long long fm = 0;
         if (i < qn) {
@@ -115,13 +136,35 @@ __global__ __launch_bounds__(256) void k_mbfs_push(const uint32_t* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void k_mbfs_pull(uint32_t n, unsigned long long full,
-                                                   const uint32_t* __restrict__ cptr,
-                                                   const uint32_t* __restrict__ csrc,
-                                                   const unsigned long long* __restrict__ frontier,
-                                                   const unsigned long long* __restrict__ visited,
-                                                   unsigned long long* __restrict__ next,
-                                                   MbfsCounters* __restrict__ ctr) {
+// Hubs: each queue vertex split into `split` edge ranges, one per workgroup item.
+__device__ __forceinline__ void mbfs_push_large(const uint32_t* __restrict__ q, uint32_t qn,
+                                                const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                                                const unsigned long long* __restrict__ frontier,
+                                                const unsigned long long* __restrict__ visited,
+                                                unsigned long long* __restrict__ next, uint32_t wg0, uint32_t nwg) {
+    uint32_t split = nwg / qn;
+    split = split < kMbfsSplit ? kMbfsSplit : (split > 4096u ? 4096u : split);
+    for (uint64_t w = wg0; w < (uint64_t)qn * split; w += nwg) {
+        const uint32_t u = q[w / split], part = (uint32_t)(w % split);
+        const uint32_t b = row_ptr[u], deg = row_ptr[u + 1] - b;
+        const unsigned long long fm = frontier[u];
+        const uint32_t per = (deg + split - 1) / split;
+        const uint32_t lo = b + min(deg, part * per), hi = b + min(deg, (part + 1) * per);
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
+            const uint32_t v = col[k];
+            const unsigned long long add = fm & ~visited[v];
+            if (add && (next[v] & add) != add) atomicOr(&next[v], add);
+        }
+    }
+}
+
+// Bottom-up: every vertex not yet visited by all sources ORs its in-neighbours' frontier
+// masks, stopping once every missing bit is found.
+__device__ __forceinline__ void mbfs_pull_all(uint32_t n, unsigned long long full, const uint32_t* __restrict__ cptr,
+                                              const uint32_t* __restrict__ csrc,
+                                              const unsigned long long* __restrict__ frontier,
+                                              const unsigned long long* __restrict__ visited,
+                                              unsigned long long* __restrict__ next, MbfsCounters* __restrict__ ctr) {
     unsigned long long probes = 0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         const unsigned long long vis = visited[v];
@@ -140,23 +183,51 @@ __global__ __launch_bounds__(256) void k_mbfs_pull(uint32_t n, unsigned long lon
     if ((threadIdx.x & (kWave - 1)) == 0 && probes) atomicAdd(&ctr->probes, probes);
 }
 
+__global__ __launch_bounds__(256) void k_mbfs_expand(const MbfsCtl* __restrict__ ctl,
+                                                     const uint32_t* __restrict__ q_small,
+                                                     const uint32_t* __restrict__ q_large,
+                                                     const uint32_t* __restrict__ row_ptr,
+                                                     const uint32_t* __restrict__ col, const uint32_t* __restrict__ cptr,
+                                                     const uint32_t* __restrict__ csrc,
+                                                     const unsigned long long* __restrict__ frontier,
+                                                     const unsigned long long* __restrict__ visited,
+                                                     unsigned long long* __restrict__ next, uint32_t n,
+                                                     MbfsCounters* __restrict__ ctr) {
+    if (ctl->done) return;
+    if (ctl->pull) {
+        mbfs_pull_all(n, ctl->full, cptr, csrc, frontier, visited, next, ctr);
+        return;
+    }
+    const uint32_t hub = ctl->hub_wgs;
+    if (blockIdx.x < hub) {
+        mbfs_push_large(q_large, ctl->n_large, row_ptr, col, frontier, visited, next, blockIdx.x, hub);
+        return;
+    }
+    const uint32_t wg = blockIdx.x - hub, nwg = gridDim.x - hub;
+    mbfs_push_small(q_small, ctl->n_small, row_ptr, col, frontier, visited, next, wg * 4 + threadIdx.x / kWave,
+                    nwg * 4);
+}
+
 constexpr uint32_t kMbfsTile = 4 * kWave * 4;  // vertices per workgroup tile (4 chunks per wave)
 
-__global__ __launch_bounds__(256) void k_mbfs_update(uint32_t n, uint32_t k, int32_t level_val,
-                                                     const uint32_t* __restrict__ row_ptr,
+// new = next & ~visited for every vertex: levels, masks, the two active queues, counters.
+__global__ __launch_bounds__(256) void k_mbfs_update(const MbfsCtl* __restrict__ ctl, int32_t level_fixed,
+                                                     uint32_t n, uint32_t large, const uint32_t* __restrict__ row_ptr,
                                                      unsigned long long* __restrict__ visited,
                                                      unsigned long long* __restrict__ frontier,
                                                      unsigned long long* __restrict__ next,
-                                                     uint8_t* __restrict__ level, uint32_t* __restrict__ active,
-                                                     MbfsCounters* __restrict__ ctr) {
-    __shared__ uint32_t s_cnt[4];
-    __shared__ uint32_t s_base;
+                                                     uint8_t* __restrict__ level, uint32_t* __restrict__ q_small,
+                                                     uint32_t* __restrict__ q_large, MbfsCounters* __restrict__ ctr) {
+    __shared__ uint32_t s_cnt[4][3];
+    __shared__ uint32_t s_base[2];
+    if (ctl && ctl->done) return;
+    const int32_t level_val = ctl ? ctl->step + 1 : level_fixed;
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const uint8_t lv = level_val >= 255 ? (uint8_t)0xFF : (uint8_t)level_val;
     for (uint32_t tile = blockIdx.x; (uint64_t)tile * kMbfsTile < n; tile += gridDim.x) {
-        unsigned long long amask[4];
-        uint32_t cnt = 0;
-        unsigned long long adeg = 0, bits = 0, redges = 0;
+        unsigned long long msm[4], mlg[4];
+        uint32_t csm = 0, clg = 0, cact = 0;
+        unsigned long long adeg = 0, bits = 0, redges = 0, dlg = 0;
 #pragma unroll
         for (uint32_t c = 0; c < 4; ++c) {
             const uint32_t v = (tile * 16 + c * 4 + wv) * kWave + lane;
@@ -171,10 +242,15 @@ __global__ __launch_bounds__(256) void k_mbfs_update(uint32_t n, uint32_t k, int
                 if (nx) visited[v] |= nx;
                 if (fr != nx) frontier[v] = nx;
             }
-            amask[c] = __ballot(nx != 0);
-            if (amask[c]) {
+            const unsigned long long amask = __ballot(nx != 0);
+            msm[c] = 0;
+            mlg[c] = 0;
+            if (amask) {
                 const uint32_t deg = nx ? row_ptr[v + 1] - row_ptr[v] : 0u;
+                msm[c] = __ballot(nx != 0 && deg > 0 && deg <= large);
+                mlg[c] = __ballot(nx != 0 && deg > large);
                 adeg += deg;
+                dlg += deg > large ? deg : 0u;
                 bits += __popcll(nx);
                 redges += (unsigned long long)__popcll(nx) * deg;
                 // level bytes: row s gets this wave's 64 consecutive vertices
@@ -186,43 +262,170 @@ __global__ __launch_bounds__(256) void k_mbfs_update(uint32_t n, uint32_t k, int
                     any &= any - 1;
                     if ((nx >> s) & 1ull) level[(size_t)s * n + v] = lv;
                 }
-                cnt += __popcll(amask[c]);
+                cact += __popcll(amask);
+                csm += __popcll(msm[c]);
+                clg += __popcll(mlg[c]);
             }
         }
-        if (lane == 0) s_cnt[wv] = cnt;
+        if (lane == 0) {
+            s_cnt[wv][0] = csm;
+            s_cnt[wv][1] = clg;
+            s_cnt[wv][2] = cact;
+        }
         adeg = wave_sum_u64(adeg);
         bits = wave_sum_u64(bits);
         redges = wave_sum_u64(redges);
+        dlg = wave_sum_u64(dlg);
         if (lane == 0 && (adeg | bits)) {
             atomicAdd(&ctr->active_deg, adeg);
             atomicAdd(&ctr->new_bits, bits);
             atomicAdd(&ctr->reached_edges, redges);
+            if (dlg) atomicAdd(&ctr->deg_large, dlg);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            const uint32_t t = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-            s_base = t ? atomicAdd(&ctr->n_active, t) : 0u;
-            if (level_val >= 255 && t) ctr->saturated = 1;
+            uint32_t tsm = 0, tlg = 0, tact = 0;
+            for (int w = 0; w < 4; ++w) {
+                tsm += s_cnt[w][0];
+                tlg += s_cnt[w][1];
+                tact += s_cnt[w][2];
+            }
+            s_base[0] = tsm ? atomicAdd(&ctr->n_small, tsm) : 0u;
+            s_base[1] = tlg ? atomicAdd(&ctr->n_large, tlg) : 0u;
+            if (tact) atomicAdd(&ctr->n_active, tact);
+            if (level_val >= 255 && tact) ctr->saturated = 1;
         }
         __syncthreads();
-        uint32_t o = s_base;
-        for (uint32_t w = 0; w < wv; ++w) o += s_cnt[w];
+        uint32_t osm = s_base[0], olg = s_base[1];
+        for (uint32_t w = 0; w < wv; ++w) {
+            osm += s_cnt[w][0];
+            olg += s_cnt[w][1];
+        }
 #pragma unroll
         for (uint32_t c = 0; c < 4; ++c) {
             const uint32_t v = (tile * 16 + c * 4 + wv) * kWave + lane;
-            if ((amask[c] >> lane) & 1ull) active[o + lanes_below(amask[c])] = v;
-            o += __popcll(amask[c]);
+            if ((msm[c] >> lane) & 1ull) q_small[osm + lanes_below(msm[c])] = v;
+            if ((mlg[c] >> lane) & 1ull) q_large[olg + lanes_below(mlg[c])] = v;
+            osm += __popcll(msm[c]);
+            olg += __popcll(mlg[c]);
         }
         __syncthreads();
     }
+}
+
+// Direction of the next superstep: fixed, learned (two MLP evaluations by the block), or the
+// direction-optimising heuristic.  All threads of the block call it.
+__device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restrict__ policy, float* s_x, float* s_a1,
+                               float* s_a2, float* s_cost) {
+    if (c.mode == 1) return false;
+    if (c.mode == 2) return true;
+    if (c.mode == 3 && policy) {
+        const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
+                                                 (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
+        for (int a = 0; a < 2; ++a) {
+            if (threadIdx.x == 0) {
+                float x[MEDUSA_POLICY_INPUTS];
+                direction_row(rec, (double)a, x);
+                for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
+            }
+            __syncthreads();
+            const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
+            if (threadIdx.x == 0) s_cost[a] = cost;
+            __syncthreads();
+        }
+        return s_cost[1] < s_cost[0];
+    }
+    if (!pull) return (double)c.m_f > (double)c.m_u / Sched::kAlpha;
+    return !((double)c.n_f < (double)c.n / Sched::kBeta);
+}
+
+// One workgroup after an update (or the seeding update when `first`): folds the counters
+// into the control block and decides the next superstep.
+__global__ __launch_bounds__(256) void k_mbfs_decide(MbfsCtl* __restrict__ ctl, MbfsCounters* __restrict__ ctr,
+                                                     const float* __restrict__ policy, uint32_t grid, uint32_t first) {
+    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
+    __shared__ float s_cost[2];
+    if (ctl->done) return;
+    const MbfsCounters k = *ctr;
+    MbfsCtl c = *ctl;
+    const unsigned long long n = c.n;
+    if (!first) {
+        // byte model of the superstep just finished: push = queue + offsets + mask per active
+        // vertex, col + visited + next probe per edge; pull = offsets + visited per vertex,
+        // source + frontier mask per probe; update = 24n
+        const unsigned long long kb = c.pull ? 4ull * (n + 1) + 8ull * n + 12ull * k.probes
+                                             : 20ull * c.n_f + 20ull * c.m_f;
+        c.kernel_bytes += kb;
+        c.alg_bytes += kb + 24ull * n + k.new_bits;
+        c.supersteps += 1;
+        if (c.pull) c.pull_steps += 1; else c.push_steps += 1;
+        c.step += 1;
+    }
+    c.saturated |= k.saturated;
+    c.reached += k.new_bits;
+    c.reached_edges += k.reached_edges;
+    c.n_u -= k.new_bits;
+    c.m_u -= k.reached_edges;
+    c.n_f = k.n_active;
+    c.m_f = k.active_deg;
+    c.n_small = k.n_small;
+    c.n_large = k.n_large;
+    c.n_active = k.n_active;
+    c.hub_wgs = mbfs_hub_wgs(grid, k.n_small, k.n_large, k.deg_large, k.active_deg);
+    if (k.n_active == 0) c.done = 1;
+    const bool pull = c.done ? false : mbfs_direction(c, first ? false : c.pull != 0, policy, s_x, s_a1, s_a2, s_cost);
+    c.pull = pull ? 1u : 0u;
+    __syncthreads();  // every wave has read the control block and counters
+    if (threadIdx.x == 0) {
+        *ctl = c;
+        ctr->n_small = 0;
+        ctr->n_large = 0;
+        ctr->n_active = 0;
+        ctr->saturated = 0;
+        ctr->active_deg = 0;
+        ctr->new_bits = 0;
+        ctr->reached_edges = 0;
+        ctr->probes = 0;
+        ctr->deg_large = 0;
+    }
+}
+
+__global__ void k_mbfs_init_ctl(MbfsCtl* ctl, uint32_t n, unsigned long long m, uint32_t k, uint32_t mode) {
+    MbfsCtl z{};
+    z.mode = mode;
+    z.n = n;
+    z.m = m;
+    z.n_u = (unsigned long long)n * k;
+    z.m_u = m * k;
+    z.full = k == 64 ? ~0ull : ((1ull << k) - 1);
+    *ctl = z;
+}
+
+// Host-driven superstep: the host's decision written into the control block.
+__global__ void k_mbfs_set_ctl(MbfsCtl* ctl, int32_t step, uint32_t pull, uint32_t n_small, uint32_t n_large,
+                               uint32_t hub_wgs) {
+    ctl->step = step;
+    ctl->pull = pull;
+    ctl->done = 0;
+    ctl->n_small = n_small;
+    ctl->n_large = n_large;
+    ctl->hub_wgs = hub_wgs;
+}
+
+static uint32_t mbfs_large(const medusa_graph* g, const medusa_opts& o) {
+    return o.large_degree ? o.large_degree : (g->tuned.mbfs_large ? g->tuned.mbfs_large : kMbfsLargeDefault);
+}
+static uint32_t mbfs_grid(const medusa_graph* g, const medusa_opts& o) {
+    return o.grid ? o.grid : (g->tuned.mbfs_grid ? g->tuned.mbfs_grid : kMbfsGridDefault);
 }
 
 static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t* out, const medusa_opts& o,
                      medusa_stats& st) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    const unsigned long long full = k == 64 ? ~0ull : ((1ull << k) - 1);
+    const uint32_t large = mbfs_large(g, o), grid = mbfs_grid(g, o);
     g->q_small.ensure(n);
+    g->q_large.ensure(n);
     g->q_aux.ensure(64);
     DBuf<unsigned long long>& masks = g->bitmap;
     masks.ensure(3 * (size_t)n);
@@ -234,12 +437,53 @@ static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t*
     lvbuf.ensure((level_bytes + 3) / 4);
     uint8_t* level = reinterpret_cast<uint8_t*>(lvbuf.p);
     MbfsCounters* C = dev_counters<MbfsCounters>(g);
+    static_assert(sizeof(MbfsCtl) <= sizeof(CtlStorage), "control block storage");
+    g->mbfs_ctl.ensure(1);
+    MbfsCtl* ctl = reinterpret_cast<MbfsCtl*>(g->mbfs_ctl.p);
     hipStream_t s = g->stream;
+    const bool host_driven = o.timing || o.trace;
+    uint32_t mode = 0;
+    if (o.direction == MEDUSA_DIR_PUSH) mode = 1;
+    else if (o.direction == MEDUSA_DIR_PULL) mode = 2;
+    else if (o.policy && policy_kind(o.policy) == MEDUSA_POLICY_DIRECTION) mode = 3;
+    const float* policy = (mode == 3 && !host_driven) ? policy_device_params(o.policy) : nullptr;
     MDS_HIP(hipMemcpyAsync(g->q_aux.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
     g->trace.clear();
-    Sched sched(o, Sched::kMbfs);
-    const bool timing = o.timing != 0;
-    if (timing) ensure_events(g, 2);
+
+    auto expand = [&]() {
+        hipLaunchKernelGGL(k_mbfs_expand, dim3(grid), dim3(256), 0, s, (const MbfsCtl*)ctl,
+                           (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
+                           g->cptr.p, g->csrc.p, (const unsigned long long*)frontier,
+                           (const unsigned long long*)visited, next, n, C);
+        MDS_LAUNCHED();
+    };
+    auto update = [&](const MbfsCtl* c, int32_t lv) {
+        hipLaunchKernelGGL(k_mbfs_update, dim3(grid_for(n, kMbfsTile, 4096)), dim3(256), 0, s, c, lv, n, large,
+                           g->row_ptr.p, visited, frontier, next, level, g->q_small.p, g->q_large.p, C);
+        MDS_LAUNCHED();
+    };
+
+    // capture (once per graph, launch shape and policy buffer) a batch of supersteps
+    GraphCache& gc = g->mbfs_graph;
+    if (!host_driven) {
+        const uintptr_t key[4] = {(uintptr_t)masks.p, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
+                                  (uintptr_t)lvbuf.p ^ (uintptr_t)n};
+        if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
+            gc.reset();
+            MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for (uint32_t b = 0; b < kMbfsBatch; ++b) {
+                expand();
+                update(ctl, 0);
+                hipLaunchKernelGGL(k_mbfs_decide, dim3(1), dim3(256), 0, s, ctl, C, policy, grid, 0u);
+            }
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            MDS_HIP(e);
+            std::copy(key, key + 4, gc.key);
+        }
+    }
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_mbfs_fill, dim3(grid_for(std::max<uint64_t>(n, (level_bytes + 3) / 4), 256, 8192)),
@@ -247,78 +491,94 @@ static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t*
     MDS_LAUNCHED();
     hipLaunchKernelGGL(k_mbfs_zero, dim3(1), dim3(1), 0, s, C);
     MDS_LAUNCHED();
-    hipLaunchKernelGGL(k_mbfs_seed, dim3(1), dim3(1), 0, s, n, (const uint32_t*)g->q_aux.p, k, level, visited,
-                       frontier, g->row_ptr.p, g->q_small.p, C);
+    hipLaunchKernelGGL(k_mbfs_seed, dim3(1), dim3(1), 0, s, (const uint32_t*)g->q_aux.p, k, next);
     MDS_LAUNCHED();
-    MbfsCounters c = read_counters<MbfsCounters>(g);
-    uint64_t reached_edges = c.reached_edges, reached = c.new_bits;
-    Frontier f{};
-    f.n = n;
-    f.m = m;
-    f.n_u = (uint64_t)n * k - c.new_bits;
-    f.m_u = (uint64_t)m * k - c.reached_edges;
-    bool pull = false;
-    uint32_t saturated = 0;
-    for (uint32_t t = 0;; ++t) {
-        f.step = t;
-        f.n_f = c.n_active;
-        f.m_f = c.active_deg;
-        if (f.n_f == 0) break;
-        if (o.direction == MEDUSA_DIR_PUSH) pull = false;
-        else if (o.direction == MEDUSA_DIR_PULL) pull = true;
-        else pull = sched.bfs_pull(f, pull);
-        if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
-        hipLaunchKernelGGL(k_mbfs_zero, dim3(1), dim3(1), 0, s, C);
+    update(nullptr, 0);
+    hipLaunchKernelGGL(k_mbfs_init_ctl, dim3(1), dim3(1), 0, s, ctl, n, (unsigned long long)m, k, mode);
+    MDS_LAUNCHED();
+
+    MbfsCtl h{};
+    if (!host_driven) {
+        hipLaunchKernelGGL(k_mbfs_decide, dim3(1), dim3(256), 0, s, ctl, C, policy, grid, 1u);
         MDS_LAUNCHED();
-        if (!pull) {
-            const uint32_t waves = (c.n_active + kWave - 1) / kWave;
-            hipLaunchKernelGGL(k_mbfs_push, dim3(grid_for(waves, 4, 16384)), dim3(256), 0, s,
-                               (const uint32_t*)g->q_small.p, c.n_active, g->row_ptr.p, g->col.p,
-                               (const unsigned long long*)frontier, (const unsigned long long*)visited, next);
-        } else {
-            hipLaunchKernelGGL(k_mbfs_pull, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, n, full, g->cptr.p,
-                               g->csrc.p, (const unsigned long long*)frontier, (const unsigned long long*)visited,
-                               next, C);
+        for (uint32_t batch = 0;; ++batch) {
+            MDS_HIP(hipGraphLaunch(gc.exec, s));
+            MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(MbfsCtl), hipMemcpyDeviceToHost, s));
+            MDS_HIP(hipStreamSynchronize(s));
+            std::memcpy(&h, g->h_ctr_raw.p, sizeof(MbfsCtl));
+            if (h.done) break;
+            MDS_CHECK(batch < (n + kMbfsBatch) / kMbfsBatch + 2, MEDUSA_ERR_INTERNAL,
+                      "mbfs: superstep loop did not end");
         }
-        MDS_LAUNCHED();
-        if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
-        hipLaunchKernelGGL(k_mbfs_update, dim3(grid_for(n, kMbfsTile, 4096)), dim3(256), 0, s, n, k, (int32_t)(t + 1),
-                           g->row_ptr.p, visited, frontier, next, level, g->q_small.p, C);
-        MDS_LAUNCHED();
-        const MbfsCounters prev = c;
-        c = read_counters<MbfsCounters>(g);
-        double kms = 0;
-        if (timing) {
-            float ms = 0;
-            MDS_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
-            kms = ms;
-            st.kernel_ms += ms;
-            ++st.kernel_launches;
+    } else {
+        Sched sched(o, Sched::kMbfs);
+        const bool timing = o.timing != 0;
+        if (timing) ensure_events(g, 2);
+        MbfsCounters c = read_counters<MbfsCounters>(g);
+        Frontier f{};
+        f.n = n;
+        f.m = m;
+        h.n_u = (uint64_t)n * k - c.new_bits;
+        h.m_u = (uint64_t)m * k - c.reached_edges;
+        h.reached = c.new_bits;
+        h.reached_edges = c.reached_edges;
+        h.saturated = c.saturated;
+        bool pull = false;
+        for (uint32_t t = 0; c.n_active; ++t) {
+            f.step = t;
+            f.n_f = c.n_active;
+            f.m_f = c.active_deg;
+            f.n_u = h.n_u;
+            f.m_u = h.m_u;
+            if (o.direction == MEDUSA_DIR_PUSH) pull = false;
+            else if (o.direction == MEDUSA_DIR_PULL) pull = true;
+            else pull = sched.bfs_pull(f, pull);
+            hipLaunchKernelGGL(k_mbfs_set_ctl, dim3(1), dim3(1), 0, s, ctl, (int32_t)t, pull ? 1u : 0u, c.n_small,
+                               c.n_large, mbfs_hub_wgs(grid, c.n_small, c.n_large, c.deg_large, c.active_deg));
+            MDS_LAUNCHED();
+            hipLaunchKernelGGL(k_mbfs_zero, dim3(1), dim3(1), 0, s, C);
+            MDS_LAUNCHED();
+            if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
+            expand();
+            if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
+            update(nullptr, (int32_t)(t + 1));
+            const MbfsCounters prev = c;
+            c = read_counters<MbfsCounters>(g);
+            double kms = 0;
+            if (timing) {
+                float ms = 0;
+                MDS_HIP(hipEventElapsedTime(&ms, g->events[0], g->events[1]));
+                kms = ms;
+                st.kernel_ms += ms;
+                ++st.kernel_launches;
+            }
+            const uint64_t kb = pull ? 4ull * (n + 1) + 8ull * n + 12ull * c.probes
+                                     : 20ull * prev.n_active + 20ull * prev.active_deg;
+            h.kernel_bytes += kb;
+            h.alg_bytes += kb + 24ull * n + c.new_bits;
+            if (pull) ++h.pull_steps; else ++h.push_steps;
+            sched.record(g, f, pull ? 1.0 : 0.0, kms);
+            ++h.supersteps;
+            h.saturated |= c.saturated;
+            h.reached += c.new_bits;
+            h.reached_edges += c.reached_edges;
+            h.n_u -= c.new_bits;
+            h.m_u -= c.reached_edges;
         }
-        // byte model: push = queue+offsets+mask per active vertex, col+visited+next probe per
-        // edge; pull = offsets+visited per vertex, source+frontier mask per probe; update = 24n
-        const uint64_t kb = pull ? 4ull * (n + 1) + 8ull * n + 12ull * c.probes
-                                 : 20ull * prev.n_active + 20ull * prev.active_deg;
-        st.kernel_bytes += kb;
-        st.alg_bytes += kb + 24ull * n + c.new_bits;
-        if (pull) ++st.pull_steps; else ++st.push_steps;
-        sched.record(g, f, pull ? 1.0 : 0.0, kms);
-        ++st.supersteps;
-        saturated |= c.saturated;
-        reached += c.new_bits;
-        reached_edges += c.reached_edges;
-        f.n_u -= c.new_bits;
-        f.m_u -= c.reached_edges;
     }
     MDS_HIP(hipEventRecord(g->ev_end, s));
     MDS_HIP(hipStreamSynchronize(s));
     float ms = 0;
     MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
     st.device_ms = ms;
-    st.edges_traversed = reached_edges;
-    st.reached = (uint32_t)std::min<uint64_t>(reached, 0xFFFFFFFFull);
-    st.alg_bytes += level_bytes;
-    st.reserved[0] = saturated;
+    st.supersteps = h.supersteps;
+    st.push_steps = h.push_steps;
+    st.pull_steps = h.pull_steps;
+    st.kernel_bytes = h.kernel_bytes;
+    st.alg_bytes = h.alg_bytes + level_bytes + 24ull * n;
+    st.edges_traversed = h.reached_edges;
+    st.reached = (uint32_t)std::min<uint64_t>(h.reached, 0xFFFFFFFFull);
+    st.reserved[0] = h.saturated;
     copy_out(g, out, level, level_bytes);
 }
 
@@ -337,6 +597,7 @@ extern "C" int medusa_mbfs(medusa_graph* g, const uint32_t* sources, uint32_t k,
         medusa_opts o{};
         if (opts) o = *opts;
         MDS_CHECK(o.direction >= 0 && o.direction <= 2, MEDUSA_ERR_ARG, "mbfs: bad direction %d", o.direction);
+        MDS_CHECK(o.grid == 0 || (o.grid >= 2 && o.grid <= 65535), MEDUSA_ERR_ARG, "mbfs: grid must be 0 or 2..65535");
         medusa_stats st{};
         mbfs_run(g, sources, k, levels, o, st);
         if (stats) *stats = st;

@@ -10,6 +10,10 @@
 //     S_t(v) = sum over in-edges u->v of c_t(u),  c_t(u) = r_t(u)/outdeg(u)
 //     D_t = sum of r_t(u) over outdeg(u) == 0.
 //
+// A packed batch of disjoint graphs runs in the same launches: graph g uses its own
+// vertex count n_g for r_0, (1-d)/n_g and D_t/n_g, and D_t sums its own dangling vertices,
+// so every graph's ranks are bit-identical to a run on that graph alone.
+//
 // Canonical combine order (identical for every kernel variant and lane mapping,
 // so push/pull and every binning give bit-identical ranks):
 //   a segment of values x_0..x_{L-1} (in-edges in ascending original source id,
@@ -19,6 +23,7 @@
 //   32,16,8,4,2,1 (lane i absorbs lane i+off); block results are merged left to
 //   right with kahan_combine; the value is s - c.
 // Scalars: d_f = (float)d, tele = (float)((1-d)/n), base_t = fmaf(d_f, D_t/(float)n, tele),
+// r_0 = (float)(1/n),
 // r_{t+1}(v) = fmaf(d_f, S_t(v), base_t), c = r/(float)outdeg (IEEE division).
 
 #include <cmath>
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs,
     const PrRun R = runs[lo];
     const uint32_t local = wg - R.wg_begin;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const float base = *base_p;
+    const float base = base_p[R.part];
 
     auto finish = [&](uint32_t p, float S) {
         const float r = fmaf(d, S, base);
@@ -181,62 +186,77 @@ __global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs,
     finish(p, lane_vertex_sum<kFromMsg>(R.deg, ell + R.ell_base + i, R.ell_stride, vals, seg));
 }
 
-// Dangling mass D = canonical sum of rank[dbegin + i], i < nd; the last
-// workgroup to finish merges the block partials in order and writes
-// base = fmaf(d, D / nf, tele).
-__global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ rank, uint32_t dbegin, uint32_t nd,
-                                                   float* __restrict__ partials, uint32_t* __restrict__ counter,
-                                                   float d, float nf, float tele, float* __restrict__ base_out) {
+// Dangling mass, step 1: one wave per canonical block of a graph's dangling tail
+// writes the block's (s, c) pair.
+__global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ rank, const PrBlock* __restrict__ blocks,
+                                                   uint32_t nblk, float* __restrict__ partials) {
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t nblk = (nd + kBlock - 1) / kBlock;
     const uint32_t b = blockIdx.x * 4 + wave;
-    if (b < nblk) {
-        const uint32_t lo = b * kBlock, lim = min(nd - lo, kBlock);
-        const float* x = rank + dbegin + lo;
-        float s = 0.f, c = 0.f;
-        for (uint32_t r0 = 0; r0 * kWave < lim; r0 += 16) {
-            float v[16];
+    if (b >= nblk) return;  // uniform across the wave
+    const PrBlock B = blocks[b];
+    const float* x = rank + B.begin;
+    float s = 0.f, c = 0.f;
+    for (uint32_t r0 = 0; r0 * kWave < B.len; r0 += 16) {
+        float v[16];
 #pragma unroll
-            for (uint32_t j = 0; j < 16; ++j) {
-                const uint32_t k = lane + (r0 + j) * kWave;
-                v[j] = k < lim ? x[k] : 0.f;
-            }
+        for (uint32_t j = 0; j < 16; ++j) {
+            const uint32_t k = lane + (r0 + j) * kWave;
+            v[j] = k < B.len ? x[k] : 0.f;
+        }
 #pragma unroll
-            for (uint32_t j = 0; j < 16; ++j)
-                if (lane + (r0 + j) * kWave < lim) kahan_add(s, c, v[j]);
-        }
-        group_tree(s, c, kWave);
-        if (lane == 0) {
-            __hip_atomic_store(&partials[2 * b], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&partials[2 * b + 1], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (uint32_t j = 0; j < 16; ++j)
+            if (lane + (r0 + j) * kWave < B.len) kahan_add(s, c, v[j]);
     }
-    __threadfence();
-    __syncthreads();
-    __shared__ uint32_t s_last;
-    if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1u : 0u;
-    __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
-    __threadfence();
+    group_tree(s, c, kWave);
+    if (lane == 0) {
+        partials[2 * b] = s;
+        partials[2 * b + 1] = c;
+    }
+}
+
+// Dangling mass, step 2 (after the block kernel, so its partials are visible): one wave
+// per graph merges the graph's block pairs left to right and writes
+// base = fmaf(d, D / (float)n_g, (float)((1 - d) / n_g)).
+__global__ __launch_bounds__(256) void k_pr_base(const float* __restrict__ partials, const PrPart* __restrict__ parts,
+                                                 uint32_t nparts, float d, float* __restrict__ base_out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t w = blockIdx.x * 4 + threadIdx.x / kWave;
+    if (w >= nparts) return;  // uniform across the wave
+    const PrPart P = parts[w];
     float as = 0.f, ac = 0.f;
-    for (uint32_t i = 0; i < nblk; ++i) {
-        const float s = __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const float c = __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (i == 0) {
-            as = s;
-            ac = c;
-        } else {
-            kahan_combine(as, ac, s, c);
+    for (uint32_t c0 = 0; c0 < P.nblocks; c0 += kWave) {
+        const uint32_t i = c0 + lane;
+        const float ps = i < P.nblocks ? partials[2 * (P.first_block + i)] : 0.f;
+        const float pc = i < P.nblocks ? partials[2 * (P.first_block + i) + 1] : 0.f;
+        const uint32_t lim = min(P.nblocks - c0, (uint32_t)kWave);
+        for (uint32_t j = 0; j < lim; ++j) {
+            const float s2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ps), j));
+            const float c2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pc), j));
+            if (c0 + j == 0) {
+                as = s2;
+                ac = c2;
+            } else {
+                kahan_combine(as, ac, s2, c2);
+            }
         }
     }
     const float D = as - ac;
-    *base_out = fmaf(d, D / nf, tele);
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float tele = (float)((1.0 - (double)d) / (double)P.n);
+    if (lane == 0) base_out[w] = fmaf(d, D / (float)P.n, tele);
 }
 
-__global__ void k_pr_init(uint32_t n, float r0, const uint32_t* __restrict__ outdeg, float* __restrict__ rank,
-                          float* __restrict__ contrib) {
+// r_0 = (float)(1 / n_g) for every vertex of graph g (graph found by binary search
+// over the range starts, which internal ids keep).
+__global__ void k_pr_init(uint32_t n, const uint32_t* __restrict__ part_begin, uint32_t nparts,
+                          const uint32_t* __restrict__ outdeg, float* __restrict__ rank, float* __restrict__ contrib) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nparts;  // last graph with begin <= p
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (part_begin[mid] <= p) lo = mid;
+            else hi = mid;
+        }
+        const float r0 = (float)(1.0 / (double)(part_begin[lo + 1] - part_begin[lo]));
         rank[p] = r0;
         const uint32_t od = outdeg[p];
         if (od) contrib[p] = r0 / (float)od;
@@ -286,37 +306,42 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     PrLayout& L = *g->pr;
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    const uint32_t nd = n - L.dangle_begin;
-    const uint32_t nblk = (nd + kBlock - 1) / kBlock;
+    const uint32_t nd = L.n_dangling;
+    const uint32_t nblk = (uint32_t)L.blocks.size();
+    const uint32_t nparts = (uint32_t)L.parts.size();
     const bool push = o.variant == MEDUSA_PR_PUSH;
     MDS_CHECK(o.variant == MEDUSA_PR_PULL || push, MEDUSA_ERR_ARG, "pagerank: unknown variant %d", o.variant);
     L.rank.ensure(n);
     L.contrib0.ensure(n);
     L.contrib1.ensure(n);
-    L.base.ensure(2);
+    L.base.ensure(nparts);
     L.partials.ensure(2 * (size_t)std::max(nblk, 1u));
-    L.counter.ensure(1);
     L.resid.ensure(1);
     L.rank_prev.ensure(n);
     L.unperm.ensure(n);
     if (push) L.msg.ensure(m);
     hipStream_t s = g->stream;
-    MDS_HIP(hipMemsetAsync(L.counter.p, 0, sizeof(uint32_t), s));
     MDS_HIP(hipMemsetAsync(L.contrib0.p, 0, (size_t)n * 4, s));
     MDS_HIP(hipMemsetAsync(L.contrib1.p, 0, (size_t)n * 4, s));
 
     const float d_f = damping;
-    const double dd = (double)damping;
-    const float nf = (float)n;
-    const float tele = (float)((1.0 - dd) / (double)n);
-    const float r0 = (float)(1.0 / (double)n);
-    const uint32_t dangle_wgs = std::max(1u, (nblk + 3) / 4);
+    const uint32_t dangle_wgs = (nblk + 3) / 4, base_wgs = (nparts + 3) / 4;
+    // base of the next superstep from the ranks in `r`
+    auto enqueue_base = [&](const float* r) {
+        if (nblk) {
+            hipLaunchKernelGGL(k_pr_dangle, dim3(dangle_wgs), dim3(256), 0, s, r, (const PrBlock*)L.d_blocks.p, nblk,
+                               L.partials.p);
+            MDS_LAUNCHED();
+        }
+        hipLaunchKernelGGL(k_pr_base, dim3(base_wgs), dim3(256), 0, s, (const float*)L.partials.p,
+                           (const PrPart*)L.d_parts.p, nparts, d_f, L.base.p);
+        MDS_LAUNCHED();
+    };
 
-    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, 256)), dim3(256), 0, s, n, r0, L.outdeg.p, L.rank.p, L.contrib0.p);
+    hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const uint32_t*)L.d_part_begin.p,
+                       nparts, L.outdeg.p, L.rank.p, L.contrib0.p);
     MDS_LAUNCHED();
-    hipLaunchKernelGGL(k_pr_dangle, dim3(dangle_wgs), dim3(256), 0, s, L.rank.p, L.dangle_begin, nd, L.partials.p,
-                       L.counter.p, d_f, nf, tele, L.base.p);
-    MDS_LAUNCHED();
+    enqueue_base(L.rank.p);
 
     const bool timing = o.timing != 0;
     if (timing) ensure_events(g, 2 * (size_t)steps);
@@ -344,11 +369,7 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
                                    L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             MDS_LAUNCHED();
             if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t + 1], s));
-            if (t + 1 < steps) {
-                hipLaunchKernelGGL(k_pr_dangle, dim3(dangle_wgs), dim3(256), 0, s, (const float*)r_out, L.dangle_begin,
-                                   nd, L.partials.p, L.counter.p, d_f, nf, tele, L.base.p);
-                MDS_LAUNCHED();
-            }
+            if (t + 1 < steps) enqueue_base(r_out);
         }
     };
 

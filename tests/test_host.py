@@ -141,3 +141,16 @@ def test_no_gpu_is_an_error_not_a_fallback():
     with pytest.raises(md.MedusaError) as e:
         md.Graph(np.array([0, 1, 1], np.uint32), np.array([1], np.uint32))
     assert e.value.code == -2
+
+
+def test_pack_graphs_host_logic():
+    a = (np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.array([5, 6], np.uint32))
+    b = (np.array([0, 0], np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    c = (np.array([0, 2, 2, 3], np.uint32), np.array([1, 2, 0], np.uint32), np.array([7, 8, 9], np.uint32))
+    rp, col, w, begin = md.pack_graphs([a, b, c])
+    assert list(begin) == [0, 2, 3, 6]
+    assert list(rp) == [0, 1, 2, 2, 4, 4, 5]
+    assert list(col) == [1, 0, 4, 5, 3]
+    assert list(w) == [5, 6, 7, 8, 9]
+    with pytest.raises(md.MedusaError):
+        md.pack_graphs([a, b[:2]])  # weights on some graphs only
