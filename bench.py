@@ -214,6 +214,31 @@ def _emit(a, metric_cfg, value, ms_per_step, roofline, cpu, extra):
     print(json.dumps(line), flush=True)
 
 
+def _train_delta_policy(md, wl, np, family):
+    """Delta-width cost model trained on the GPU from traced SSSP runs of smaller graphs of the
+    same family (lattices, or RMAT scale 14-16 for RMAT), over a sweep of fixed widths."""
+    if family == "sssp_lattice":
+        graphs = [wl.lattice_graph(r, c, seed=sd) + (wl.lattice_source(r, c),)
+                  for r, c, sd in ((200, 200, 31), (400, 300, 32), (500, 500, 33))]
+    else:
+        graphs = []
+        for sc in (14, 15, 16):
+            rp, col = wl.rmat_graph(scale=sc, edge_factor=16, seed=40 + sc)
+            graphs.append((rp, col, wl.weighted(rp, col, seed=sc), wl.max_outdeg_vertex(rp)))
+    Xs, ys = [], []
+    for rp, col, wt, src in graphs:
+        with md.Graph(rp, col, wt) as H:
+            for d in (250, 500, 1000, 2000, 4000, 8000, 16000):
+                H.sssp(src, delta=d, trace=1)
+                for r in H.trace():
+                    if r[7] > 0 and r[8] > 0:
+                        Xs.append(md.policy_features(md.POLICY_DELTA, r, r[7]))
+                        ys.append(np.log2(r[8] * 1e3 / r[7] + 1e-3))
+    pol = md.Policy(md.POLICY_DELTA, seed=3)
+    pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
+    return pol
+
+
 def run_workload(a, md, wl, torch, np):
     """The non-headline configurations: one JSON line each, same schema as the headline."""
     import oracle
@@ -254,12 +279,14 @@ def run_workload(a, md, wl, torch, np):
             wpr, _ = _timed(lambda: G.pagerank(0.85, 20, out=pout, use_graph=1), a.steps, 1, torch)
             extra["pagerank20_gteps"] = round(a.steps * 20 * m / wpr / 1e9, 3)
             extra["pagerank20_ms"] = round(wpr / a.steps * 1e3, 4)
-        c0 = time.perf_counter()
-        _, steps_cpu = oracle.bfs_sweep(rp, col, src, threads)
-        tc = time.perf_counter() - c0
-        cpu = {"value": round(m_r / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": f"reference CPU path restated (all-edge sweep per superstep, OpenMP), full run, "
-                         f"{steps_cpu} supersteps"}
+        cpu = None
+        if not a.no_cpu:
+            c0 = time.perf_counter()
+            _, steps_cpu = oracle.bfs_sweep(rp, col, src, threads)
+            tc = time.perf_counter() - c0
+            cpu = {"value": round(m_r / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+                   "sample": f"reference CPU path restated (all-edge sweep per superstep, OpenMP), full run, "
+                             f"{steps_cpu} supersteps"}
         _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
         G.close()
     elif w in ("sssp_lattice", "sssp_rmat20"):
@@ -279,24 +306,34 @@ def run_workload(a, md, wl, torch, np):
                    "graph": "RMAT scale 20, edge factor 16, seed 2; weights U[1,1000] seed 6", "dtype": "uint32"}
         G = md.Graph(rp, col, wt)
         out = torch.empty(n, dtype=torch.int32, device="cuda")
-        wall, res = _timed(lambda: G.sssp(src, out=out)[1], a.steps, a.warmup, torch)
+        pol = _train_delta_policy(md, wl, np, w)
+        wall, res = _timed(lambda: G.sssp(src, out=out, policy=pol)[1], a.steps, a.warmup, torch)
+        learned = out.cpu().numpy().copy()
         m_r = res[-1].edges_traversed
-        extra = {"supersteps": res[-1].supersteps, "phases": res[-1].reserved[1], "edges_reached": m_r}
+        wp, rp_ = _timed(lambda: G.sssp(src, out=out)[1], a.steps, 1, torch)
+        extra = {"supersteps": res[-1].supersteps, "phases": res[-1].reserved[1], "edges_reached": m_r,
+                 "policy": "delta width per threshold advance by the MLP, trained on the GPU from traced runs of "
+                           "smaller graphs of the same family (not this graph)",
+                 "gteps_plain": round(a.steps * m_r / wp / 1e9, 4), "supersteps_plain": rp_[-1].supersteps,
+                 "learned_over_plain": round(wp / wall, 3),
+                 "distances_identical_to_plain": bool(np.array_equal(learned, out.cpu().numpy()))}
         for name, d in (("bellman_ford", 0xFFFFFFFF), ("delta_1000", 1000), ("delta_8000", 8000)):
             wt_, r_ = _timed(lambda: G.sssp(src, out=out, delta=d)[1], max(1, a.steps // 2), 1, torch)
             extra[f"gteps_{name}"] = round(max(1, a.steps // 2) * m_r / wt_ / 1e9, 4)
             extra[f"supersteps_{name}"] = r_[-1].supersteps
         roof = _roofline([G.sssp(src, out=out, timing=1)[1] for _ in range(2)], "k_sssp_relax")
-        dist = out.cpu().numpy().view(np.uint32)
-        rounds = oracle.sssp_rounds(rp, col, wt, src, dist)
-        k = 3
-        c0 = time.perf_counter()
-        oracle.sssp_bsp(rp, col, wt, src, threads, max_steps=k)
-        tc = (time.perf_counter() - c0) / k * rounds
-        cpu = {"value": round(m_r / tc / 1e9, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": f"reference CPU path restated (message array, all-edge sweep per superstep, OpenMP): "
-                         f"{k} supersteps timed, scaled to the {rounds} supersteps the path needs (computed "
-                         f"from the final distances)"}
+        cpu = None
+        if not a.no_cpu:
+            dist = out.cpu().numpy().view(np.uint32)
+            rounds = oracle.sssp_rounds(rp, col, wt, src, dist)
+            k = 3
+            c0 = time.perf_counter()
+            oracle.sssp_bsp(rp, col, wt, src, threads, max_steps=k)
+            tc = (time.perf_counter() - c0) / k * rounds
+            cpu = {"value": round(m_r / tc / 1e9, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
+                   "sample": f"reference CPU path restated (message array, all-edge sweep per superstep, OpenMP): "
+                             f"{k} supersteps timed, scaled to the {rounds} supersteps the path needs (computed "
+                             f"from the final distances)"}
         _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
         G.close()
     elif w == "mbfs_rmat20":
@@ -322,16 +359,18 @@ def run_workload(a, md, wl, torch, np):
                  "plain_64_bfs_gteps": round(plain_edges / tp / 1e9, 3),
                  "batched_over_plain": round((a.steps * m_r / wall) / (plain_edges / tp), 3)}
         roof = _roofline([G.mbfs(srcs, out=out, timing=1)[1] for _ in range(3)], "k_mbfs_push / k_mbfs_pull")
-        ks = 2
-        c0 = time.perf_counter()
-        cpu_edges = 0
-        deg = np.diff(rp.astype(np.int64))
-        for s in srcs[:ks]:
-            lv_cpu, _ = oracle.bfs_sweep(rp, col, int(s), threads)
-            cpu_edges += int(deg[lv_cpu >= 0].sum())
-        tc = time.perf_counter() - c0
-        cpu = {"value": round(cpu_edges / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": f"reference CPU path restated (all-edge sweep BFS, OpenMP) for {ks} of the 64 sources"}
+        cpu = None
+        if not a.no_cpu:
+            ks = 2
+            c0 = time.perf_counter()
+            cpu_edges = 0
+            deg = np.diff(rp.astype(np.int64))
+            for s in srcs[:ks]:
+                lv_cpu, _ = oracle.bfs_sweep(rp, col, int(s), threads)
+                cpu_edges += int(deg[lv_cpu >= 0].sum())
+            tc = time.perf_counter() - c0
+            cpu = {"value": round(cpu_edges / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+                   "sample": f"reference CPU path restated (all-edge sweep BFS, OpenMP) for {ks} of the 64 sources"}
         _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
         G.close()
     elif w == "chung_lu_batch":
@@ -399,16 +438,18 @@ def run_workload(a, md, wl, torch, np):
                  "pagerank_bits_identical_to_plain": bool(pr_same)}
         roof = _roofline([G.bfs_multi(sources, out=out, timing=1)[1] for _ in range(3)],
                          "k_bfs_push / k_bfs_pull")
-        ks = 16
-        c0 = time.perf_counter()
-        cpu_edges = 0
-        for (grp, gcol), s_ in list(zip(graphs, local))[:ks]:
-            lv_cpu, _ = oracle.bfs_sweep(grp, gcol, int(s_), threads)
-            cpu_edges += int(np.diff(grp.astype(np.int64))[lv_cpu >= 0].sum())
-        tc = time.perf_counter() - c0
-        cpu = {"value": round(cpu_edges / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": f"reference CPU path restated (all-edge sweep BFS, OpenMP) on the first {ks} of the "
-                         f"128 graphs"}
+        cpu = None
+        if not a.no_cpu:
+            ks = 16
+            c0 = time.perf_counter()
+            cpu_edges = 0
+            for (grp, gcol), s_ in list(zip(graphs, local))[:ks]:
+                lv_cpu, _ = oracle.bfs_sweep(grp, gcol, int(s_), threads)
+                cpu_edges += int(np.diff(grp.astype(np.int64))[lv_cpu >= 0].sum())
+            tc = time.perf_counter() - c0
+            cpu = {"value": round(cpu_edges / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+                   "sample": f"reference CPU path restated (all-edge sweep BFS, OpenMP) on the first {ks} of the "
+                             f"128 graphs"}
         _emit(a, cfg, a.steps * m_r / wall / 1e9, wall / a.steps * 1e3, roof, cpu, extra)
         G.close()
 

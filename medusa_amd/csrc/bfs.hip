@@ -166,26 +166,63 @@ __device__ __forceinline__ void push_large(const uint32_t* __restrict__ q, uint3
     }
 }
 
-// Bottom-up: an unvisited vertex joins if any in-neighbour is in the frontier bitmap.
+constexpr uint32_t kPullLane = 32;  // in-degree up to which one lane scans a vertex alone
+
+// Bottom-up: an unvisited vertex joins if any in-neighbour is in the frontier bitmap.  A
+// lane scans a vertex of in-degree <= kPullLane alone; above that the whole wave scans it
+// 64 in-edges at a time and stops at the first hit (ballot), so one high in-degree vertex
+// does not hold its wave for hundreds of sequential probes.  `probes` counts in-edges
+// examined up to and including the first hit in either case.
 __device__ __forceinline__ void pull_all(uint32_t n, const uint32_t* __restrict__ cptr,
                                          const uint32_t* __restrict__ csrc,
                                          const unsigned long long* __restrict__ bitmap, int32_t* __restrict__ level,
                                          int32_t next, FrontierCounters* __restrict__ ctr) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     unsigned long long probes = 0;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        if (level[v] >= 0) continue;
-        const uint32_t b = cptr[v], e = cptr[v + 1];
-        for (uint32_t k = b; k < e; ++k) {
-            const uint32_t u = csrc[k];
-            ++probes;
-            if ((bitmap[u >> 6] >> (u & 63)) & 1ull) {
-                level[v] = next;
-                break;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t v = base + lane;  // uniform loop: every lane of the wave iterates together
+        uint32_t b = 0, e = 0;
+        bool todo = false;
+        if (v < n && level[v] < 0) {
+            b = cptr[v];
+            e = cptr[v + 1];
+            todo = b < e;
+        }
+        const bool big = todo && e - b > kPullLane;
+        if (todo && !big) {
+            for (uint32_t k = b; k < e; ++k) {
+                const uint32_t u = csrc[k];
+                ++probes;
+                if ((bitmap[u >> 6] >> (u & 63)) & 1ull) {
+                    level[v] = next;
+                    break;
+                }
+            }
+        }
+        for (unsigned long long bm = __ballot(big); bm; bm &= bm - 1) {
+            const uint32_t owner = __builtin_ctzll(bm);
+            const uint32_t vb = __shfl(b, owner, kWave), ve = __shfl(e, owner, kWave);
+            for (uint32_t k0 = vb; k0 < ve; k0 += kWave) {
+                const uint32_t k = k0 + lane;
+                bool hit = false;
+                if (k < ve) {
+                    const uint32_t u = csrc[k];
+                    hit = (bitmap[u >> 6] >> (u & 63)) & 1ull;
+                }
+                const unsigned long long hm = __ballot(hit);
+                if (hm) {
+                    if (lane == 0) {
+                        probes += __builtin_ctzll(hm) + 1;
+                        level[base + owner] = next;
+                    }
+                    break;
+                }
+                if (lane == 0) probes += min(ve - k0, (uint32_t)kWave);
             }
         }
     }
     probes = wave_sum_u64(probes);
-    if ((threadIdx.x & (kWave - 1)) == 0 && probes) atomicAdd(&ctr->probes, probes);
+    if (lane == 0 && probes) atomicAdd(&ctr->probes, probes);
 }
 
 constexpr uint32_t kCompactK = 8;                         // 64-vertex chunks per wave per tile

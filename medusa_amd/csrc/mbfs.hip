@@ -158,29 +158,61 @@ __device__ __forceinline__ void mbfs_push_large(const uint32_t* __restrict__ q, 
     }
 }
 
+constexpr uint32_t kMbfsPullLane = 32;  // in-degree up to which one lane scans a vertex alone
+
 // Bottom-up: every vertex not yet visited by all sources ORs its in-neighbours' frontier
-// masks, stopping once every missing bit is found.
+// masks, stopping once every missing bit is found.  Vertices above kMbfsPullLane in-degree
+// are scanned by the whole wave, 64 in-edges per step with an OR reduction (their probe
+// count then includes the rest of the last 64-edge step).
 __device__ __forceinline__ void mbfs_pull_all(uint32_t n, unsigned long long full, const uint32_t* __restrict__ cptr,
                                               const uint32_t* __restrict__ csrc,
                                               const unsigned long long* __restrict__ frontier,
                                               const unsigned long long* __restrict__ visited,
                                               unsigned long long* __restrict__ next, MbfsCounters* __restrict__ ctr) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     unsigned long long probes = 0;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        const unsigned long long vis = visited[v];
-        if (vis == full) continue;
-        unsigned long long acc = 0;
-        const uint32_t b = cptr[v], e = cptr[v + 1];
-        for (uint32_t k = b; k < e; ++k) {
-            acc |= frontier[csrc[k]];
-            ++probes;
-            if ((acc | vis) == full) break;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t v = base + lane;  // uniform loop: every lane of the wave iterates together
+        uint32_t b = 0, e = 0;
+        unsigned long long vis = full;
+        if (v < n) {
+            vis = visited[v];
+            if (vis != full) {
+                b = cptr[v];
+                e = cptr[v + 1];
+            }
         }
-        acc &= ~vis;
-        if (acc) next[v] = acc;
+        const bool big = e - b > kMbfsPullLane;
+        if (!big && b < e) {
+            unsigned long long acc = 0;
+            for (uint32_t k = b; k < e; ++k) {
+                acc |= frontier[csrc[k]];
+                ++probes;
+                if ((acc | vis) == full) break;
+            }
+            acc &= ~vis;
+            if (acc) next[v] = acc;
+        }
+        for (unsigned long long bm = __ballot(big); bm; bm &= bm - 1) {
+            const uint32_t owner = __builtin_ctzll(bm);
+            const uint32_t vb = __shfl(b, owner, kWave), ve = __shfl(e, owner, kWave);
+            const unsigned long long ovis = __shfl(vis, owner, kWave);
+            unsigned long long acc = 0;
+            for (uint32_t k0 = vb; k0 < ve; k0 += kWave) {
+                const uint32_t k = k0 + lane;
+                unsigned long long x = k < ve ? frontier[csrc[k]] : 0ull;
+#pragma unroll
+                for (int off = kWave / 2; off >= 1; off >>= 1) x |= __shfl_xor(x, off, kWave);
+                acc |= x;
+                if (lane == 0) probes += min(ve - k0, (uint32_t)kWave);
+                if ((acc | ovis) == full) break;
+            }
+            acc &= ~ovis;
+            if (lane == 0 && acc) next[base + owner] = acc;
+        }
     }
     probes = wave_sum_u64(probes);
-    if ((threadIdx.x & (kWave - 1)) == 0 && probes) atomicAdd(&ctr->probes, probes);
+    if (lane == 0 && probes) atomicAdd(&ctr->probes, probes);
 }
 
 __global__ __launch_bounds__(256) void k_mbfs_expand(const MbfsCtl* __restrict__ ctl,
