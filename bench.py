@@ -214,6 +214,25 @@ def _emit(a, metric_cfg, value, ms_per_step, roofline, cpu, extra):
     print(json.dumps(line), flush=True)
 
 
+def _train_direction_policy(md, wl, np):
+    """Push/pull cost model trained on the GPU from push-only and pull-only traced BFS runs
+    on smaller graphs (RMAT scale 14-17, Erdos-Renyi, Chung-Lu); never the measured graph."""
+    graphs = [wl.rmat_graph(scale=sc, edge_factor=16, seed=10 + sc) for sc in (14, 15, 16, 17)]
+    graphs += [wl.er_graph(2 ** 15, 2 ** 19, seed=21), wl.chung_lu_graph(200_000, 420_000, seed=22)]
+    Xs, ys = [], []
+    for rp, col in graphs:
+        with md.Graph(rp, col) as H:
+            for src in (wl.max_outdeg_vertex(rp), 3):
+                for act, d in ((0, md.DIR_PUSH), (1, md.DIR_PULL)):
+                    H.bfs(src, direction=d, timing=1, trace=1)
+                    tr = H.trace()
+                    Xs.append(np.stack([md.policy_features(md.POLICY_DIRECTION, r, act) for r in tr]))
+                    ys.append(np.log2(1.0 + tr[:, 8] * 1e3))
+    pol = md.Policy(md.POLICY_DIRECTION, seed=7)
+    pol.train(np.concatenate(Xs).astype(np.float32), np.concatenate(ys).astype(np.float32), iters=3000, lr=3e-3)
+    return pol
+
+
 def _train_delta_policy(md, wl, np, family):
     """Delta-width cost model trained on the GPU from traced SSSP runs of smaller graphs of the
     same family (lattices, or RMAT scale 14-16 for RMAT), over a sweep of fixed widths."""
@@ -273,6 +292,15 @@ def run_workload(a, md, wl, torch, np):
         for name, d in (("push_only", md.DIR_PUSH), ("pull_only", md.DIR_PULL)):
             wt, _ = _timed(lambda: G.bfs(src, out=out, direction=d), a.steps, 1, torch)
             extra[f"gteps_{name}"] = round(a.steps * m_r / wt / 1e9, 3)
+        plain_levels = out.cpu().numpy().copy()
+        pol = _train_direction_policy(md, wl, np)
+        wl_, rl_ = _timed(lambda: G.bfs(src, out=out, policy=pol)[1], a.steps, 1, torch)
+        extra.update({"gteps_learned": round(a.steps * m_r / wl_ / 1e9, 3),
+                      "learned_over_plain": round(wall / wl_, 3),
+                      "learned_push_pull_steps": [rl_[-1].push_steps, rl_[-1].pull_steps],
+                      "levels_identical_to_plain": bool(np.array_equal(plain_levels, out.cpu().numpy())),
+                      "policy": "direction per superstep by the MLP, trained on the GPU from push-only/pull-only "
+                                "traces of smaller RMAT, Erdos-Renyi and Chung-Lu graphs (not this graph)"})
         roof = _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)], "k_bfs_push / k_bfs_pull")
         if w == "chung_lu":
             pout = torch.empty(n, dtype=torch.float32, device="cuda")

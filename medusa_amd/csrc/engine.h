@@ -208,6 +208,7 @@ struct medusa_graph {
     medusa::DBuf<int32_t> level;
     medusa::DBuf<uint32_t> dist, q_small, q_large, q_aux, stamp;
     medusa::DBuf<unsigned long long> bitmap;
+    medusa::DBuf<unsigned long long> near_q;  // SSSP near queues of (slice << 32 | vertex) entries
     // per-superstep counters (device) and their pinned host mirror; each engine
     // views the 256 bytes as its own counter struct
     medusa::DBuf<unsigned long long> ctr_raw;

@@ -10,8 +10,8 @@
 // Architecture: x[16] -> 32 -> 32 -> 1, leaky ReLU (slope 1/64) on both hidden layers.
 // Parameters, in order: W1[32][16], b1[32], W2[32][32], b2[32], W3[32], b3.
 // Training: full-batch mean squared error, Adam.  Forward and backward run as HIP
-// kernels written here; every reduction over samples is a sequential loop in sample
-// order, so gradients are deterministic.
+// kernels written here; every reduction over samples has a fixed order (samples in
+// order within chunks of 256, chunks in order), so gradients are deterministic.
 
 #include <cmath>
 
@@ -91,38 +91,48 @@ __global__ __launch_bounds__(256) void k_mlp_delta(const float* __restrict__ P, 
     }
 }
 
-// One thread per parameter: its gradient summed over samples in order.  Needs the
-// activations A1 = f(Z1_pre), A2 = f(Z2_pre), which are recomputed from X and P.
-__global__ __launch_bounds__(256) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X, uint32_t n,
-                                                  const float* __restrict__ G, const float* __restrict__ D1,
-                                                  const float* __restrict__ D2, const float* __restrict__ A1,
-                                                  const float* __restrict__ A2, float* __restrict__ grad) {
+constexpr uint32_t kGradChunk = 256;  // samples per partial gradient
+
+// Partial gradients: thread p of column `chunk` sums parameter p's gradient over the
+// chunk's samples in order.  Needs the activations A1 = f(Z1_pre), A2 = f(Z2_pre).
+__global__ __launch_bounds__(256) void k_mlp_grad_part(const float* __restrict__ X, uint32_t n,
+                                                       const float* __restrict__ G, const float* __restrict__ D1,
+                                                       const float* __restrict__ D2, const float* __restrict__ A1,
+                                                       const float* __restrict__ A2, float* __restrict__ part) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
-    (void)P;
+    const uint32_t s0 = blockIdx.y * kGradChunk, s1 = min(n, s0 + kGradChunk);
     float acc = 0.f;
-    const uint32_t oW1 = 0, ob1 = kMlpH * kMlpIn, oW2 = ob1 + kMlpH, ob2 = oW2 + kMlpH * kMlpH, oW3 = ob2 + kMlpH,
-                   ob3 = oW3 + kMlpH;
+    constexpr uint32_t ob1 = kMlpH * kMlpIn, oW2 = ob1 + kMlpH, ob2 = oW2 + kMlpH * kMlpH, oW3 = ob2 + kMlpH,
+                       ob3 = oW3 + kMlpH;
     if (p < ob1) {
         const uint32_t i = p / kMlpIn, k = p % kMlpIn;
-        for (uint32_t s = 0; s < n; ++s) acc = fmaf(D1[(size_t)s * kMlpH + i], X[(size_t)s * kMlpIn + k], acc);
+        for (uint32_t s = s0; s < s1; ++s) acc = fmaf(D1[(size_t)s * kMlpH + i], X[(size_t)s * kMlpIn + k], acc);
     } else if (p < oW2) {
         const uint32_t i = p - ob1;
-        for (uint32_t s = 0; s < n; ++s) acc += D1[(size_t)s * kMlpH + i];
+        for (uint32_t s = s0; s < s1; ++s) acc += D1[(size_t)s * kMlpH + i];
     } else if (p < ob2) {
         const uint32_t i = (p - oW2) / kMlpH, k = (p - oW2) % kMlpH;
-        for (uint32_t s = 0; s < n; ++s) acc = fmaf(D2[(size_t)s * kMlpH + i], A1[(size_t)s * kMlpH + k], acc);
+        for (uint32_t s = s0; s < s1; ++s) acc = fmaf(D2[(size_t)s * kMlpH + i], A1[(size_t)s * kMlpH + k], acc);
     } else if (p < oW3) {
         const uint32_t i = p - ob2;
-        for (uint32_t s = 0; s < n; ++s) acc += D2[(size_t)s * kMlpH + i];
+        for (uint32_t s = s0; s < s1; ++s) acc += D2[(size_t)s * kMlpH + i];
     } else if (p < ob3) {
         const uint32_t i = p - oW3;
-        for (uint32_t s = 0; s < n; ++s) acc = fmaf(G[s], A2[(size_t)s * kMlpH + i], acc);
+        for (uint32_t s = s0; s < s1; ++s) acc = fmaf(G[s], A2[(size_t)s * kMlpH + i], acc);
     } else {
-        for (uint32_t s = 0; s < n; ++s) acc += G[s];
+        for (uint32_t s = s0; s < s1; ++s) acc += G[s];
     }
+    part[(size_t)blockIdx.y * kMlpParams + p] = acc;
+}
+
+// Gradient = partials summed over chunks in chunk order (deterministic).
+__global__ void k_mlp_grad_sum(const float* __restrict__ part, uint32_t nchunks, float* __restrict__ grad) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float acc = 0.f;
+    for (uint32_t c = 0; c < nchunks; ++c) acc += part[(size_t)c * kMlpParams + p];
     grad[p] = acc;
-    (void)oW1;
 }
 
 // Hidden activations from pre-activations (A = f(Z)), elementwise.
@@ -156,10 +166,11 @@ __global__ void k_mlp_loss(const float* __restrict__ out, const float* __restric
 
 // Device work buffers for a batch of n samples.
 struct MlpWork {
-    DBuf<float> X, y, out, Z1, Z2, A1, A2, grad, M, V, loss;
+    DBuf<float> X, y, out, Z1, Z2, A1, A2, grad, M, V, loss, part;
     uint32_t cap = 0;
     void ensure(uint32_t n) {
         if (n <= cap) return;
+        part.alloc((size_t)((n + kGradChunk - 1) / kGradChunk) * kMlpParams);
         X.alloc((size_t)n * kMlpIn);
         y.alloc(n);
         out.alloc(n);
@@ -204,14 +215,18 @@ static void mlp_gradients(medusa_policy* pol, uint32_t n, float* loss_out) {
     hipLaunchKernelGGL(k_mlp_delta, dim3(grid), dim3(256), 0, s, (const float*)pol->d_params.p, (const float*)w.y.p,
                        n, w.out.p, w.Z1.p, w.Z2.p);
     MDS_LAUNCHED();
-    hipLaunchKernelGGL(k_mlp_grad, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, (const float*)pol->d_params.p,
+    const uint32_t nchunks = (n + kGradChunk - 1) / kGradChunk;
+    hipLaunchKernelGGL(k_mlp_grad_part, dim3((kMlpParams + 255) / 256, nchunks), dim3(256), 0, s,
                        (const float*)w.X.p, n, (const float*)w.out.p, (const float*)w.Z1.p, (const float*)w.Z2.p,
-                       (const float*)w.A1.p, (const float*)w.A2.p, w.grad.p);
+                       (const float*)w.A1.p, (const float*)w.A2.p, w.part.p);
+    MDS_LAUNCHED();
+    hipLaunchKernelGGL(k_mlp_grad_sum, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, (const float*)w.part.p,
+                       nchunks, w.grad.p);
     MDS_LAUNCHED();
     if (loss_out) {
         MDS_HIP(hipMemcpyAsync(loss_out, w.loss.p, 4, hipMemcpyDeviceToHost, s));
+        MDS_HIP(hipStreamSynchronize(s));
     }
-    MDS_HIP(hipStreamSynchronize(s));
 }
 
 static void upload_batch(medusa_policy* pol, const float* X, const float* y, uint32_t n) {
@@ -311,10 +326,12 @@ extern "C" int medusa_policy_gradients(medusa_policy* p, const float* X, const f
                                        float* loss) {
     return guard([&] {
         MDS_CHECK(p && X && y && grad && n > 0, MEDUSA_ERR_ARG, "policy_gradients: bad arguments");
+        MDS_CHECK(n <= 65535u * kGradChunk, MEDUSA_ERR_ARG, "policy_gradients: at most %u samples", 65535u * kGradChunk);
         MDS_HIP(hipSetDevice(p->device));
         upload_batch(p, X, y, n);
         mlp_gradients(p, n, loss);
-        MDS_HIP(hipMemcpy(grad, p->w.grad.p, kMlpParams * 4, hipMemcpyDeviceToHost));
+        MDS_HIP(hipMemcpyAsync(grad, p->w.grad.p, kMlpParams * 4, hipMemcpyDeviceToHost, p->stream));
+        MDS_HIP(hipStreamSynchronize(p->stream));
     });
 }
 
@@ -336,6 +353,7 @@ extern "C" int medusa_policy_train(medusa_policy* p, const float* X, const float
                                    float lr, float* final_loss) {
     return guard([&] {
         MDS_CHECK(p && X && y && n > 0, MEDUSA_ERR_ARG, "policy_train: bad arguments");
+        MDS_CHECK(n <= 65535u * kGradChunk, MEDUSA_ERR_ARG, "policy_train: at most %u samples", 65535u * kGradChunk);
         MDS_HIP(hipSetDevice(p->device));
         upload_batch(p, X, y, n);
         MDS_HIP(hipMemsetAsync(p->w.M.p, 0, kMlpParams * 4, p->stream));

@@ -14,6 +14,11 @@
 // near-far form); delta = 0xFFFFFFFF degenerates to frontier Bellman-Ford.  Any
 // delta gives the same (shortest) distances; only the number of supersteps and
 // relaxations change.
+//
+// Near-queue entries are (vertex, slice) pairs: a vertex enters with one entry per
+// kSlice of its out-edges, so a hub's relaxation spreads over as many workgroups as it
+// has slices.  A workgroup relaxes 256 entries at a time through a block-wide prefix scan
+// of their degrees, and stages its queue appends in LDS (one atomic per flush).
 
 #include <algorithm>
 #include <chrono>
@@ -32,13 +37,21 @@ struct SsspCounters {
     unsigned long long relax;     // edges relaxed this superstep
 };
 
+constexpr uint32_t kSlice = 128;       // out-edges per near-queue entry
+constexpr uint32_t kStage = 2048;      // LDS staging entries per queue per workgroup
+constexpr uint32_t kStageSlices = 4;   // a vertex with more slices goes straight to HBM
+
+__host__ __device__ inline uint32_t slices_of(uint32_t deg) { return deg <= kSlice ? 1u : (deg + kSlice - 1) / kSlice; }
+__host__ inline size_t near_capacity(uint32_t n, uint64_t m) { return (size_t)n + m / kSlice + 64; }
+
 __global__ void k_sssp_init(uint32_t n, uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp, uint32_t src,
-                            uint32_t* __restrict__ near_q) {
+                            uint32_t src_slices, unsigned long long* __restrict__ near_q) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         dist[v] = v == src ? 0u : 0xFFFFFFFFu;
         stamp[v] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) near_q[0] = src;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < src_slices; k += gridDim.x * blockDim.x)
+        near_q[k] = ((unsigned long long)k << 32) | src;
 }
 
 __global__ void k_sssp_zero(SsspCounters* c, uint32_t keep_far) {
@@ -65,104 +78,227 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t* __r
     }
 }
 
-// One relaxation superstep over the near queue (wave-cooperative over out-edges).
-__global__ __launch_bounds__(256) void k_sssp_relax(const uint32_t* __restrict__ q, uint32_t qn,
-                                                    const uint32_t* __restrict__ row_ptr,
-                                                    const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
-                                                    uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
-                                                    uint32_t epoch, uint32_t thr, uint32_t* __restrict__ near_next,
-                                                    uint32_t near_cap, uint32_t* __restrict__ far_q, uint32_t far_cap,
-                                                    SsspCounters* __restrict__ ctr) {
-    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave], s_du[4][kWave];
+// ---- workgroup-staged queue appends ----------------------------------------------------
+// Appends from a workgroup collect in LDS and go to HBM in bulk, one atomic per queue per
+// flush, instead of one atomic per wave per 64 edges on a single counter (which serialises
+// on that address).
+
+struct Stage {
+    unsigned long long near[kStage];
+    uint32_t far[kStage];
+    uint32_t n_near, n_far, base_near, base_far, flush;
+};
+
+// Where a kernel's appends go.
+struct Sinks {
+    unsigned long long* near;
+    uint32_t* near_cnt;
+    uint32_t near_cap;
+    uint32_t* far;
+    uint32_t* far_cnt;
+    uint32_t far_cap;
+    const uint32_t* row_ptr;
+};
+
+__device__ __forceinline__ void stage_init(Stage& st) {
+    if (threadIdx.x == 0) {
+        st.n_near = 0;
+        st.n_far = 0;
+    }
+    __syncthreads();
+}
+
+// One thread's appends: v to the near queue (as its slices) and/or to the far queue.
+__device__ __forceinline__ void stage_put(Stage& st, const Sinks& k, bool to_near, bool to_far, uint32_t v,
+                                          unsigned long long& ndeg) {
+    if (to_near) {
+        const uint32_t deg = k.row_ptr[v + 1] - k.row_ptr[v];
+        ndeg += deg;
+        const uint32_t cnt = slices_of(deg);
+        if (cnt > kStageSlices) {
+            const uint32_t base = atomicAdd(k.near_cnt, cnt);
+            for (uint32_t j = 0; j < cnt; ++j)
+                if (base + j < k.near_cap) k.near[base + j] = ((unsigned long long)j << 32) | v;
+        } else {
+            const uint32_t pos = atomicAdd(&st.n_near, cnt);
+            for (uint32_t j = 0; j < cnt; ++j) st.near[pos + j] = ((unsigned long long)j << 32) | v;
+        }
+    }
+    if (to_far) st.far[atomicAdd(&st.n_far, 1u)] = v;
+}
+
+// Called by every thread of the block after each round of stage_put (at most one call per
+// thread per round): copies the staged entries to HBM when another round could overflow
+// the stage, or whenever anything is staged if `last`.
+__device__ __forceinline__ void stage_flush(Stage& st, const Sinks& k, bool last) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t nn = st.n_near, nf = st.n_far;
+        const bool go = last ? (nn | nf) != 0 : (nn > kStage - 256 * kStageSlices || nf > kStage - 256);
+        st.flush = go ? 1u : 0u;
+        if (go) {
+            st.base_near = nn ? atomicAdd(k.near_cnt, nn) : 0u;
+            st.base_far = nf ? atomicAdd(k.far_cnt, nf) : 0u;
+        }
+    }
+    __syncthreads();
+    if (!st.flush) return;  // uniform: written before the barrier above, rewritten only after the next one
+    const uint32_t nn = st.n_near, nf = st.n_far;
+    for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x)
+        if (st.base_near + i < k.near_cap) k.near[st.base_near + i] = st.near[i];
+    for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x)
+        if (st.base_far + i < k.far_cap) k.far[st.base_far + i] = st.far[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st.n_near = 0;
+        st.n_far = 0;
+    }
+    __syncthreads();
+}
+
+// Exclusive prefix sum over the 256 threads of the block; `total` = the block's sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_wave, uint32_t& total) {
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const uint32_t total_waves = gridDim.x * 4;
-    unsigned long long ndeg = 0, relax = 0;
-    for (uint32_t base = (blockIdx.x * 4 + wv) * kWave; base < qn; base += total_waves * kWave) {
-        const uint32_t i = base + lane;
+    const uint32_t incl = wave_incl_scan(x);
+    if (lane == kWave - 1) s_wave[wv] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t w = 0; w < wv; ++w) off += s_wave[w];
+    total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    return off + incl - x;
+}
+
+// Relaxes near entries q[0, qn) with the whole block: workgroup b takes the 256 entries of
+// chunks b, b + grid, ...; the block shares their out-edges through a prefix scan of the
+// entry degrees and every thread binary-searches the owner of its edge.  Returns (in every
+// thread) the edges relaxed by this block.
+__device__ __forceinline__ unsigned long long relax_block(const unsigned long long* __restrict__ q, uint32_t qn,
+                                                          const uint32_t* __restrict__ col,
+                                                          const uint32_t* __restrict__ w, uint32_t* dist,
+                                                          uint32_t* __restrict__ stamp, uint32_t epoch, uint32_t thr,
+                                                          Stage& st, const Sinks& k, unsigned long long& ndeg) {
+    __shared__ uint32_t s_beg[256], s_off[256], s_du[256], s_wave[4];
+    const uint32_t t = threadIdx.x;
+    unsigned long long relax = 0;
+    for (uint32_t chunk = blockIdx.x; (uint64_t)chunk * 256 < qn; chunk += gridDim.x) {  // uniform per block
+        const uint64_t i = (uint64_t)chunk * 256 + t;
         uint32_t beg = 0, deg = 0, du = 0;
         if (i < qn) {
-            const uint32_t u = q[i];
-            beg = row_ptr[u];
-            deg = row_ptr[u + 1] - beg;
+            const unsigned long long ent = q[i];
+            const uint32_t u = (uint32_t)ent, slice = (uint32_t)(ent >> 32);
+            const uint32_t b = k.row_ptr[u], e = k.row_ptr[u + 1];
+            beg = b + min(e - b, slice * kSlice);
+            deg = min(e - beg, kSlice);
             du = dist[u];
         }
-        const uint32_t incl = wave_incl_scan(deg);
-        const uint32_t total = __shfl(incl, kWave - 1, kWave);
-        s_beg[wv][lane] = beg;
-        s_off[wv][lane] = incl - deg;
-        s_du[wv][lane] = du;
-        __builtin_amdgcn_wave_barrier();
+        uint32_t total;
+        const uint32_t excl = block_excl_scan(deg, s_wave, total);
+        s_beg[t] = beg;
+        s_off[t] = excl;
+        s_du[t] = du;
+        __syncthreads();
         relax += total;
-        for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
-            const uint32_t e = e0 + lane;
+        for (uint32_t e0 = 0; e0 < total; e0 += 256) {  // uniform per block
+            const uint32_t e = e0 + t;
             bool to_near = false, to_far = false;
             uint32_t v = 0;
             if (e < total) {
                 uint32_t lo = 0;
 #pragma unroll
-                for (uint32_t step = kWave / 2; step; step >>= 1)
-                    if (s_off[wv][lo + step] <= e) lo += step;
-                const uint32_t k = s_beg[wv][lo] + (e - s_off[wv][lo]);
-                v = col[k];
-                const unsigned long long nd64 = (unsigned long long)s_du[wv][lo] + w[k];
+                for (uint32_t step = 128; step; step >>= 1)
+                    if (s_off[lo + step] <= e) lo += step;
+                const uint32_t x = s_beg[lo] + (e - s_off[lo]);
+                v = col[x];
+                const unsigned long long nd64 = (unsigned long long)s_du[lo] + w[x];
                 if (nd64 < dist[v]) {
                     const uint32_t nd = (uint32_t)nd64;
                     const uint32_t old = atomicMin(&dist[v], nd);
                     if (nd < old) {
-                        if (nd < thr) {
-                            to_near = atomicMax(&stamp[v], epoch) < epoch;
-                        } else {
-                            to_far = true;
-                        }
+                        if (nd < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
+                        else to_far = true;
                     }
                 }
             }
-            wave_append(to_near, v, near_next, &ctr->n_near, near_cap);
-            wave_append(to_far, v, far_q, &ctr->n_far, far_cap);
-            if (to_near) ndeg += row_ptr[v + 1] - row_ptr[v];
+            stage_put(st, k, to_near, to_far, v, ndeg);
+            stage_flush(st, k, false);
         }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();  // s_beg / s_off / s_du / s_wave are rewritten by the next chunk
     }
-    ndeg = wave_sum_u64(ndeg);
-    if (lane == 0) {
-        if (ndeg) atomicAdd(&ctr->near_deg, ndeg);
-        if (relax) atomicAdd(&ctr->relax, relax);
-    }
+    return relax;
 }
 
-// Splits the far queue at a new threshold: entries in [old_thr, thr) move to the
-// near queue (once, by stamp), entries >= thr stay far (compacted into far_out),
-// entries below old_thr were already settled through the near queue.
+// Splits far entries (or, when `rescan`, every vertex) at a new threshold: distances in
+// [old_thr, thr) move to the near queue (once, by stamp), distances >= thr stay far
+// (deduplicated by stamp); below old_thr they were settled through the near queue.
+// Returns the smallest kept far distance of this thread.
+__device__ __forceinline__ uint32_t split_block(const uint32_t* __restrict__ far_in, uint32_t nin, bool rescan,
+                                                const uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
+                                                uint32_t epoch, uint32_t old_thr, uint32_t thr, Stage& st,
+                                                const Sinks& k, unsigned long long& ndeg) {
+    uint32_t fmin = 0xFFFFFFFFu;
+    for (uint32_t i0 = blockIdx.x * 256; i0 < nin; i0 += gridDim.x * 256) {  // uniform per block
+        const uint32_t i = i0 + threadIdx.x;
+        bool to_near = false, keep = false;
+        uint32_t v = 0;
+        if (i < nin) {
+            v = rescan ? i : far_in[i];
+            const uint32_t d = dist[v];
+            if (d != 0xFFFFFFFFu && d >= old_thr) {
+                if (d < thr) {
+                    to_near = atomicMax(&stamp[v], epoch) < epoch;
+                } else {
+                    keep = atomicMax(&stamp[v], epoch) < epoch;
+                    if (keep) fmin = min(fmin, d);
+                }
+            }
+        }
+        stage_put(st, k, to_near, keep, v, ndeg);
+        stage_flush(st, k, false);
+    }
+    return fmin;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+    for (int off = kWave / 2; off >= 1; off >>= 1) x = min(x, (uint32_t)__shfl_xor(x, off, kWave));
+    return x;
+}
+
+// One relaxation superstep over the near queue (host-driven loop).
+__global__ __launch_bounds__(256) void k_sssp_relax(const unsigned long long* __restrict__ q, uint32_t qn,
+                                                    const uint32_t* __restrict__ row_ptr,
+                                                    const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
+                                                    uint32_t* dist, uint32_t* __restrict__ stamp, uint32_t epoch,
+                                                    uint32_t thr, unsigned long long* __restrict__ near_next,
+                                                    uint32_t near_cap, uint32_t* __restrict__ far_q, uint32_t far_cap,
+                                                    SsspCounters* __restrict__ ctr) {
+    __shared__ Stage st;
+    stage_init(st);
+    const Sinks k{near_next, &ctr->n_near, near_cap, far_q, &ctr->n_far, far_cap, row_ptr};
+    unsigned long long ndeg = 0;
+    const unsigned long long relax = relax_block(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg);
+    stage_flush(st, k, true);
+    ndeg = wave_sum_u64(ndeg);
+    if ((threadIdx.x & (kWave - 1)) == 0 && ndeg) atomicAdd(&ctr->near_deg, ndeg);
+    if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
+}
+
+// Split of the far queue at a new threshold (host-driven loop).
 __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__ far_in, uint32_t nfar,
                                                     const uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
                                                     uint32_t epoch, uint32_t old_thr, uint32_t thr,
                                                     const uint32_t* __restrict__ row_ptr,
-                                                    uint32_t* __restrict__ near_q, uint32_t near_cap,
+                                                    unsigned long long* __restrict__ near_q, uint32_t near_cap,
                                                     uint32_t* __restrict__ far_out, uint32_t far_cap,
                                                     SsspCounters* __restrict__ ctr) {
+    __shared__ Stage st;
+    stage_init(st);
+    const Sinks k{near_q, &ctr->n_near, near_cap, far_out, &ctr->n_far, far_cap, row_ptr};
     unsigned long long ndeg = 0;
-    uint32_t fmin = 0xFFFFFFFFu;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < nfar; i0 += stride) {  // uniform per wave
-        const uint32_t i = i0 + threadIdx.x;
-        bool to_near = false, keep = false;
-        uint32_t v = 0;
-        if (i < nfar) {
-            v = far_in[i];
-            const uint32_t d = dist[v];
-            if (d >= old_thr && d < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
-            else if (d >= thr) {
-                keep = atomicMax(&stamp[v], epoch) < epoch;  // drop duplicates
-                if (keep) fmin = min(fmin, d);
-            }
-        }
-        wave_append(to_near, v, near_q, &ctr->n_near, near_cap);
-        wave_append(keep, v, far_out, &ctr->n_far, far_cap);
-        if (to_near) ndeg += row_ptr[v + 1] - row_ptr[v];
-    }
+    uint32_t fmin = split_block(far_in, nfar, false, dist, stamp, epoch, old_thr, thr, st, k, ndeg);
+    stage_flush(st, k, true);
     ndeg = wave_sum_u64(ndeg);
-#pragma unroll
-    for (int off = kWave / 2; off >= 1; off >>= 1) fmin = min(fmin, (uint32_t)__shfl_xor(fmin, off, kWave));
+    fmin = wave_min_u32(fmin);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (ndeg) atomicAdd(&ctr->near_deg, ndeg);
         if (fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
@@ -183,18 +319,19 @@ __global__ __launch_bounds__(256) void k_sssp_refill(uint32_t n, const uint32_t*
 }
 
 // ---- device-driven loop ----------------------------------------------------------------
-// Two kernels per superstep, both ending with a decision by their last workgroup:
-//   relax+decide: relax the current near queue; when it comes out empty, the phase ends:
-//                 pick the next bucket width (default, fixed, or the learned policy on
-//                 the GPU), raise the threshold and mark a split as pending;
-//   split+decide: when pending, move far entries below the new threshold to the near
-//                 queue (a full rescan of the distances if the far queue overflowed);
-//                 an empty bucket raises the threshold past the smallest far distance.
-// kSsspBatch supersteps are captured once as a hipGraph and replayed until done.
+// Each slot of the captured loop is two kernels:
+//   work:   if a split is pending, move far entries below the new threshold to the near queue
+//           (a full rescan of the distances if the far queue overflowed); otherwise relax the
+//           current near queue;
+//   decide: one workgroup folds the counters into the control block; after a relaxation
+//           that left the near queue empty the phase ends: it picks the next bucket width
+//           (default, fixed, or the learned policy evaluated on the GPU), raises the
+//           threshold and marks a split pending; after a split, an empty bucket raises the
+//           threshold past the smallest far distance.
+// kSsspBatch slots are captured once as a hipGraph and replayed until done.
 
 constexpr uint32_t kSsspBatch = 32;
-constexpr uint32_t kSsspRelaxGrid = 512;
-constexpr uint32_t kSsspSplitGrid = 512;
+constexpr uint32_t kSsspGrid = 2048;
 
 struct SsspCtl {
     uint32_t done, epoch, thr, old_thr, delta, n_near, n_far, split_pending;
@@ -204,7 +341,7 @@ struct SsspCtl {
 };
 
 struct SsspDevCounters {
-    uint32_t n_near;   // near entries written by this kernel
+    uint32_t n_near;   // near entries written by this slot's work kernel
     uint32_t n_far;    // far queue length (relax appends after the current entries)
     uint32_t far_out;  // far entries kept by a split
     uint32_t far_min;  // smallest distance kept by a split
@@ -240,195 +377,104 @@ __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     return s >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
 }
 
-__global__ __launch_bounds__(256) void k_sssp_relax_dev(SsspCtl* __restrict__ ctl, uint32_t* __restrict__ near0,
-                                                        uint32_t* __restrict__ near1, uint32_t* __restrict__ far0,
-                                                        uint32_t* __restrict__ far1, uint32_t far_cap,
-                                                        const uint32_t* __restrict__ row_ptr,
-                                                        const uint32_t* __restrict__ col,
-                                                        const uint32_t* __restrict__ w, uint32_t* __restrict__ dist,
-                                                        uint32_t* __restrict__ stamp,
-                                                        SsspDevCounters* __restrict__ ctr) {
-    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave], s_du[4][kWave];
+__global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ ctl,
+                                                   unsigned long long* __restrict__ near0,
+                                                   unsigned long long* __restrict__ near1, uint32_t near_cap,
+                                                   uint32_t* __restrict__ far0, uint32_t* __restrict__ far1,
+                                                   uint32_t far_cap, const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
+                                                   uint32_t* dist, uint32_t* __restrict__ stamp,
+                                                   SsspDevCounters* __restrict__ ctr) {
+    __shared__ Stage st;
     if (ctl->done) return;
-    const uint32_t N = ctl->n_near, n = (uint32_t)ctl->n;
-    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    if (N > 0) {
-        const uint32_t* q = ctl->near_sel ? near1 : near0;
-        uint32_t* next = ctl->near_sel ? near0 : near1;
-        uint32_t* farq = ctl->far_sel ? far1 : far0;
-        const uint32_t epoch = ctl->epoch, thr = ctl->thr;
-        unsigned long long relax = 0;
-        for (uint32_t base = (blockIdx.x * 4 + wv) * kWave; base < N; base += gridDim.x * 4 * kWave) {
-            const uint32_t i = base + lane;
-            uint32_t beg = 0, deg = 0, du = 0;
-            if (i < N) {
-                const uint32_t u = q[i];
-                beg = row_ptr[u];
-                deg = row_ptr[u + 1] - beg;
-                du = dist[u];
-            }
-            const uint32_t incl = wave_incl_scan(deg);
-            const uint32_t total = __shfl(incl, kWave - 1, kWave);
-            s_beg[wv][lane] = beg;
-            s_off[wv][lane] = incl - deg;
-            s_du[wv][lane] = du;
-            __builtin_amdgcn_wave_barrier();
-            relax += total;
-            for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
-                const uint32_t e = e0 + lane;
-                bool to_near = false, to_far = false;
-                uint32_t v = 0;
-                if (e < total) {
-                    uint32_t lo = 0;
-#pragma unroll
-                    for (uint32_t step = kWave / 2; step; step >>= 1)
-                        if (s_off[wv][lo + step] <= e) lo += step;
-                    const uint32_t k = s_beg[wv][lo] + (e - s_off[wv][lo]);
-                    v = col[k];
-                    const unsigned long long nd64 = (unsigned long long)s_du[wv][lo] + w[k];
-                    if (nd64 < dist[v]) {
-                        const uint32_t nd = (uint32_t)nd64;
-                        const uint32_t old = atomicMin(&dist[v], nd);
-                        if (nd < old) {
-                            if (nd < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
-                            else to_far = true;
-                        }
-                    }
-                }
-                wave_append(to_near, v, next, &ctr->n_near, n);
-                wave_append(to_far, v, farq, &ctr->n_far, far_cap);
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-        relax = wave_sum_u64(relax);
-        if (lane == 0 && relax) atomicAdd(&ctr->relax, relax);
+    const uint32_t epoch = ctl->epoch, thr = ctl->thr;
+    unsigned long long ndeg = 0;
+    if (ctl->split_pending) {
+        const bool rescan = ctl->overflow != 0;
+        const uint32_t nin = rescan ? (uint32_t)ctl->n : ctl->n_far;
+        const Sinks k{ctl->near_sel ? near1 : near0, &ctr->n_near, near_cap, ctl->far_sel ? far0 : far1,
+                      &ctr->far_out, far_cap, row_ptr};
+        stage_init(st);
+        uint32_t fmin = split_block(ctl->far_sel ? far1 : far0, nin, rescan, dist, stamp, epoch, ctl->old_thr, thr,
+                                    st, k, ndeg);
+        stage_flush(st, k, true);
+        fmin = wave_min_u32(fmin);
+        if ((threadIdx.x & (kWave - 1)) == 0 && fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
+        return;
     }
+    const uint32_t N = ctl->n_near;
+    if (N == 0) return;
+    const Sinks k{ctl->near_sel ? near0 : near1, &ctr->n_near, near_cap, ctl->far_sel ? far1 : far0, &ctr->n_far,
+                  far_cap, row_ptr};
+    stage_init(st);
+    const unsigned long long relax =
+        relax_block(ctl->near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st, k, ndeg);
+    stage_flush(st, k, true);
+    if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
 }
 
-// One workgroup, after the relaxation kernel: reads its counters (visible at the kernel
-// boundary, so no fence is needed) and decides the next superstep.
-__global__ __launch_bounds__(256) void k_sssp_relax_decide(SsspCtl* __restrict__ ctl,
-                                                           SsspDevCounters* __restrict__ ctr, uint32_t far_cap,
-                                                           const float* __restrict__ policy) {
+// One workgroup after the work kernel (its counters are visible at the kernel boundary).
+__global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
+                                                     uint32_t far_cap, const float* __restrict__ policy) {
     __shared__ uint32_t s_pick;
     __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH], s_best;
     if (ctl->done) return;
-    const uint32_t N = ctl->n_near;
-    const uint32_t n_next = ctr->n_near;
-    uint32_t n_far = ctr->n_far;
-    const unsigned long long relaxed = ctr->relax;
+    const SsspDevCounters k = *ctr;
     SsspCtl c = *ctl;
-    if (N > 0) {
-        c.supersteps += 1;
-        c.near_sel ^= 1u;
-        c.relaxed += relaxed;
-        c.kernel_bytes += 16ull * N + 12ull * relaxed + 12ull * n_next;
-    }
-    if (n_far > far_cap) {
-        c.overflow = 1;
-        n_far = far_cap;
-    }
-    c.n_near = n_next;
-    c.n_far = n_far;
-    c.epoch += 1;
-    bool advance = false;
-    if (n_next == 0 && !c.split_pending) {
-        if (n_far == 0 && !c.overflow) c.done = 1;
-        else advance = true;
-    }
-    if (advance) {
-        const uint32_t d = sssp_pick_delta(c, n_far, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
-        c.old_thr = c.thr;
-        c.delta = d;
-        c.thr = sat_add(c.thr, d);
-        c.split_pending = 1;
-        c.phase += 1;
+    uint32_t n_far;
+    if (c.split_pending) {
+        // after a split
+        c.far_sel ^= 1u;
+        n_far = min(k.far_out, far_cap);
+        c.n_far = n_far;
+        c.overflow = 0;
+        c.epoch += 1;
+        c.n_near = k.n_near;
+        if (k.n_near > 0) {
+            c.split_pending = 0;
+        } else if (n_far == 0) {
+            c.done = 1;
+        } else {  // empty bucket: restart from the smallest far distance
+            const uint32_t d = sssp_pick_delta(c, n_far, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
+            c.old_thr = c.thr;
+            c.delta = d;
+            c.thr = sat_add(k.far_min > c.thr ? k.far_min : c.thr, d);
+        }
+    } else {
+        // after a relaxation (of c.n_near entries)
+        const uint32_t N = c.n_near;
+        n_far = k.n_far;
+        if (N > 0) {
+            c.supersteps += 1;
+            c.near_sel ^= 1u;
+            c.relaxed += k.relax;
+            c.kernel_bytes += 16ull * N + 12ull * k.relax + 12ull * k.n_near;
+        }
+        if (n_far > far_cap) {
+            c.overflow = 1;
+            n_far = far_cap;
+        }
+        c.n_near = k.n_near;
+        c.n_far = n_far;
+        c.epoch += 1;
+        if (k.n_near == 0) {
+            if (n_far == 0 && !c.overflow) {
+                c.done = 1;
+            } else {
+                const uint32_t d = sssp_pick_delta(c, n_far, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
+                c.old_thr = c.thr;
+                c.delta = d;
+                c.thr = sat_add(c.thr, d);
+                c.split_pending = 1;
+                c.phase += 1;
+            }
+        }
     }
     __syncthreads();  // every wave has read the control block and counters
     if (threadIdx.x == 0) {
         *ctl = c;
         ctr->n_near = 0;
         ctr->n_far = n_far;
-        ctr->relax = 0;
-        ctr->far_out = 0;
-        ctr->far_min = 0xFFFFFFFFu;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_sssp_split_dev(SsspCtl* __restrict__ ctl, uint32_t* __restrict__ near0,
-                                                        uint32_t* __restrict__ near1, uint32_t* __restrict__ far0,
-                                                        uint32_t* __restrict__ far1, uint32_t far_cap,
-                                                        const uint32_t* __restrict__ dist,
-                                                        uint32_t* __restrict__ stamp,
-                                                        SsspDevCounters* __restrict__ ctr) {
-    if (ctl->done || !ctl->split_pending) return;
-    const uint32_t n = (uint32_t)ctl->n, epoch = ctl->epoch, old_thr = ctl->old_thr, thr = ctl->thr;
-    const bool rescan = ctl->overflow != 0;
-    const uint32_t nin = rescan ? n : ctl->n_far;
-    const uint32_t* far_in = ctl->far_sel ? far1 : far0;
-    uint32_t* far_out = ctl->far_sel ? far0 : far1;
-    uint32_t* near_q = ctl->near_sel ? near1 : near0;
-    uint32_t fmin = 0xFFFFFFFFu;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < nin; i0 += gridDim.x * blockDim.x) {  // uniform per block
-        const uint32_t i = i0 + threadIdx.x;
-        bool to_near = false, keep = false;
-        uint32_t v = 0;
-        if (i < nin) {
-            v = rescan ? i : far_in[i];
-            const uint32_t d = dist[v];
-            // on a rescan every unsettled vertex (distance >= old threshold) is a far entry
-            if (d != 0xFFFFFFFFu && d >= old_thr) {
-                if (d < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
-                else {
-                    keep = atomicMax(&stamp[v], epoch) < epoch;
-                    if (keep) fmin = min(fmin, d);
-                }
-            }
-        }
-        wave_append(to_near, v, near_q, &ctr->n_near, n);
-        wave_append(keep, v, far_out, &ctr->far_out, far_cap);
-    }
-#pragma unroll
-    for (int off = kWave / 2; off >= 1; off >>= 1) fmin = min(fmin, (uint32_t)__shfl_xor(fmin, off, kWave));
-    if ((threadIdx.x & (kWave - 1)) == 0 && fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
-}
-
-// One workgroup, after the split kernel (which ran iff a split was pending).
-__global__ __launch_bounds__(256) void k_sssp_split_decide(SsspCtl* __restrict__ ctl,
-                                                           SsspDevCounters* __restrict__ ctr,
-                                                           const float* __restrict__ policy) {
-    __shared__ uint32_t s_pick;
-    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH], s_best;
-    if (ctl->done || !ctl->split_pending) return;
-    const uint32_t n_near = ctr->n_near;
-    const uint32_t kept = ctr->far_out;
-    const uint32_t far_min = ctr->far_min;
-    SsspCtl c = *ctl;
-    c.far_sel ^= 1u;
-    c.n_far = kept;
-    c.overflow = 0;
-    c.epoch += 1;
-    c.n_near = n_near;
-    bool again = false;
-    if (n_near > 0) {
-        c.split_pending = 0;
-    } else if (kept == 0) {
-        c.done = 1;
-    } else {
-        again = true;  // empty bucket: restart from the smallest far distance
-    }
-    if (again) {
-        const uint32_t d = sssp_pick_delta(c, kept, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
-        c.old_thr = c.thr;
-        c.delta = d;
-        c.thr = sat_add(far_min > c.thr ? far_min : c.thr, d);
-        c.split_pending = 1;
-    }
-    __syncthreads();  // every wave has read the control block and counters
-    if (threadIdx.x == 0) {
-        *ctl = c;
-        ctr->n_near = 0;
-        ctr->n_far = kept;
         ctr->far_out = 0;
         ctr->far_min = 0xFFFFFFFFu;
         ctr->relax = 0;
@@ -436,12 +482,12 @@ __global__ __launch_bounds__(256) void k_sssp_split_decide(SsspCtl* __restrict__
 }
 
 __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, unsigned long long m, uint32_t thr,
-                                uint32_t mode, uint32_t wmean, uint32_t dflt) {
+                                uint32_t mode, uint32_t wmean, uint32_t dflt, uint32_t src_slices) {
     SsspCtl z{};
     z.epoch = 1;
     z.thr = thr;
     z.delta = thr;
-    z.n_near = 1;
+    z.n_near = src_slices;
     z.mode = mode;
     z.wmean = wmean;
     z.dflt_delta = dflt;
@@ -489,21 +535,22 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     SsspDevCounters* ctr = dev_counters<SsspDevCounters>(g);
     uint32_t* far0 = g->q_aux.p;
     uint32_t* far1 = g->q_aux.p + far_cap;
+    const size_t near_cap = near_capacity(n, m);
+    unsigned long long* near0 = g->near_q.p;
+    unsigned long long* near1 = g->near_q.p + near_cap;
+    const uint32_t src_slices = slices_of(g->h_row_ptr[src + 1] - g->h_row_ptr[src]);
 
     GraphCache& gc = g->sssp_graph;
-    const uintptr_t key[4] = {(uintptr_t)g->dist.p, (uintptr_t)policy, (uintptr_t)g->q_aux.p, (uintptr_t)g->w.p};
+    const uintptr_t key[4] = {(uintptr_t)g->dist.p ^ (uintptr_t)g->near_q.p, (uintptr_t)policy, (uintptr_t)g->q_aux.p,
+                              (uintptr_t)g->w.p};
     if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (uint32_t k = 0; k < kSsspBatch; ++k) {
-            hipLaunchKernelGGL(k_sssp_relax_dev, dim3(kSsspRelaxGrid), dim3(256), 0, s, ctl, g->q_small.p,
-                               g->q_large.p, far0, far1, (uint32_t)far_cap, g->row_ptr.p, g->col.p, g->w.p, g->dist.p,
-                               g->stamp.p, ctr);
-            hipLaunchKernelGGL(k_sssp_relax_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap, policy);
-            hipLaunchKernelGGL(k_sssp_split_dev, dim3(kSsspSplitGrid), dim3(256), 0, s, ctl, g->q_small.p,
-                               g->q_large.p, far0, far1, (uint32_t)far_cap, (const uint32_t*)g->dist.p, g->stamp.p,
-                               ctr);
-            hipLaunchKernelGGL(k_sssp_split_decide, dim3(1), dim3(256), 0, s, ctl, ctr, policy);
+            hipLaunchKernelGGL(k_sssp_work, dim3(kSsspGrid), dim3(256), 0, s, (const SsspCtl*)ctl, near0, near1,
+                               (uint32_t)near_cap, far0, far1, (uint32_t)far_cap, g->row_ptr.p, g->col.p, g->w.p,
+                               g->dist.p, g->stamp.p, ctr);
+            hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap, policy);
         }
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
@@ -515,10 +562,10 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_sssp_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->dist.p, g->stamp.p, src,
-                       g->q_small.p);
+                       src_slices, near0);
     MDS_LAUNCHED();
     hipLaunchKernelGGL(k_sssp_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, thr0, mode, wmean,
-                       width);
+                       width, src_slices);
     MDS_LAUNCHED();
     SsspCtl h{};
     // every replay either relaxes a non-empty queue or raises the threshold
@@ -549,11 +596,6 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const size_t far_cap = (size_t)std::max<uint64_t>(m, n) + n;
-    g->dist.ensure(n);
-    g->stamp.ensure(n);
-    g->q_small.ensure(n);  // near (current)
-    g->q_large.ensure(n);  // near (next)
-    g->q_aux.ensure(2 * far_cap);  // far in / far out
     SsspCounters* C = dev_counters<SsspCounters>(g);
     hipStream_t s = g->stream;
     g->trace.clear();
@@ -565,18 +607,20 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     const uint32_t delta_default = (uint32_t)std::min<uint64_t>(0xFFFFFFFEull, 2ull * wmean);
     Sched sched(o, Sched::kSssp);
 
-    uint32_t* near_cur = g->q_small.p;
-    uint32_t* near_nxt = g->q_large.p;
+    const size_t near_cap = near_capacity(n, m);
+    unsigned long long* near_cur = g->near_q.p;
+    unsigned long long* near_nxt = g->near_q.p + near_cap;
     uint32_t* far_a = g->q_aux.p;
     uint32_t* far_b = g->q_aux.p + far_cap;
+    const uint32_t src_slices = slices_of(g->h_row_ptr[src + 1] - g->h_row_ptr[src]);
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_sssp_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->dist.p, g->stamp.p, src,
-                       near_cur);
+                       src_slices, near_cur);
     MDS_LAUNCHED();
     hipLaunchKernelGGL(k_sssp_zero, dim3(1), dim3(1), 0, s, C, 0u);
     MDS_LAUNCHED();
-    uint32_t n_near = 1, n_far = 0, epoch = 1;
+    uint32_t n_near = src_slices, n_far = 0, epoch = 1;
     uint64_t near_deg = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
     uint64_t relaxed = 0;
     const bool timing = o.timing != 0;
@@ -621,8 +665,8 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
                 hipLaunchKernelGGL(k_sssp_zero, dim3(1), dim3(1), 0, s, C, 0u);
                 MDS_LAUNCHED();
                 hipLaunchKernelGGL(k_sssp_split, dim3(grid_for(n_far, 256, 4096)), dim3(256), 0, s, far_a, n_far,
-                                   g->dist.p, g->stamp.p, epoch, old_thr, thr, g->row_ptr.p, near_cur, n, far_b,
-                                   (uint32_t)far_cap, C);
+                                   g->dist.p, g->stamp.p, epoch, old_thr, thr, g->row_ptr.p, near_cur,
+                                   (uint32_t)near_cap, far_b, (uint32_t)far_cap, C);
                 MDS_LAUNCHED();
                 const SsspCounters sc = read_counters<SsspCounters>(g);
                 std::swap(far_a, far_b);
@@ -642,11 +686,10 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
         hipLaunchKernelGGL(k_sssp_zero, dim3(1), dim3(1), 0, s, C, 1u);
         MDS_LAUNCHED();
-        const uint32_t waves = (n_near + kWave - 1) / kWave;
-        // the far queue keeps growing at far_a[n_far..)
-        hipLaunchKernelGGL(k_sssp_relax, dim3(grid_for(waves, 4, 16384)), dim3(256), 0, s, near_cur, n_near,
-                           g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, epoch, thr, near_nxt, n, far_a,
-                           (uint32_t)far_cap, C);
+        // 256 entries per workgroup chunk; the far queue keeps growing at far_a[n_far..)
+        hipLaunchKernelGGL(k_sssp_relax, dim3(grid_for(n_near, 256, kSsspGrid)), dim3(256), 0, s, near_cur, n_near,
+                           g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, epoch, thr, near_nxt,
+                           (uint32_t)near_cap, far_a, (uint32_t)far_cap, C);
         MDS_LAUNCHED();
         if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
         const SsspCounters c = read_counters<SsspCounters>(g);
@@ -736,9 +779,8 @@ extern "C" int medusa_sssp(medusa_graph* g, uint32_t source, uint32_t* dist, con
         const size_t far_cap = (size_t)std::max<uint64_t>(g->m, n) + n;
         g->dist.ensure(n);
         g->stamp.ensure(n);
-        g->q_small.ensure(n);          // near queue A
-        g->q_large.ensure(n);          // near queue B
-        g->q_aux.ensure(2 * far_cap);  // far queues A and B
+        g->near_q.ensure(2 * near_capacity(n, g->m));  // near queues A and B ((vertex, slice) entries)
+        g->q_aux.ensure(2 * far_cap);                   // far queues A and B
         medusa_stats st{};
         if (o.timing || o.trace) sssp_run_host(g, source, dist, o, st);
         else sssp_run_device(g, source, dist, o, st);
