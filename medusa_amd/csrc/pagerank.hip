@@ -245,6 +245,57 @@ __global__ __launch_bounds__(256) void k_pr_base(const float* __restrict__ parti
     if (lane == 0) base_out[w] = fmaf(d, D / (float)P.n, tele);
 }
 
+constexpr uint32_t kSmallBlocks = 64;  // dangling blocks up to which one workgroup does both steps
+
+// Both dangling-mass steps in one workgroup when there are at most kSmallBlocks blocks:
+// wave w sums blocks w, w+4, ... into LDS, then wave w merges graphs w, w+4, ... in block
+// order.  Same operations and order as k_pr_dangle followed by k_pr_base.
+__global__ __launch_bounds__(256) void k_pr_dangle_base(const float* __restrict__ rank,
+                                                        const PrBlock* __restrict__ blocks, uint32_t nblk,
+                                                        const PrPart* __restrict__ parts, uint32_t nparts, float d,
+                                                        float* __restrict__ base_out) {
+    __shared__ float s_s[kSmallBlocks], s_c[kSmallBlocks];
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (uint32_t b = wave; b < nblk; b += 4) {
+        const PrBlock B = blocks[b];
+        const float* x = rank + B.begin;
+        float s = 0.f, c = 0.f;
+        for (uint32_t r0 = 0; r0 * kWave < B.len; r0 += 16) {
+            float v[16];
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j) {
+                const uint32_t k = lane + (r0 + j) * kWave;
+                v[j] = k < B.len ? x[k] : 0.f;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 16; ++j)
+                if (lane + (r0 + j) * kWave < B.len) kahan_add(s, c, v[j]);
+        }
+        group_tree(s, c, kWave);
+        if (lane == 0) {
+            s_s[b] = s;
+            s_c[b] = c;
+        }
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    for (uint32_t g = wave; g < nparts; g += 4) {
+        const PrPart P = parts[g];
+        float as = 0.f, ac = 0.f;
+        for (uint32_t i = 0; i < P.nblocks; ++i) {
+            if (i == 0) {
+                as = s_s[P.first_block];
+                ac = s_c[P.first_block];
+            } else {
+                kahan_combine(as, ac, s_s[P.first_block + i], s_c[P.first_block + i]);
+            }
+        }
+        const float D = as - ac;
+        const float tele = (float)((1.0 - (double)d) / (double)P.n);
+        base_out[g] = fmaf(d, D / (float)P.n, tele);
+    }
+}
+
 // r_0 = (float)(1 / n_g) for every vertex of graph g (graph found by binary search
 // over the range starts, which internal ids keep).
 __global__ void k_pr_init(uint32_t n, const uint32_t* __restrict__ part_begin, uint32_t nparts,
@@ -328,6 +379,12 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     const uint32_t dangle_wgs = (nblk + 3) / 4, base_wgs = (nparts + 3) / 4;
     // base of the next superstep from the ranks in `r`
     auto enqueue_base = [&](const float* r) {
+        if (nblk <= kSmallBlocks) {
+            hipLaunchKernelGGL(k_pr_dangle_base, dim3(1), dim3(256), 0, s, r, (const PrBlock*)L.d_blocks.p, nblk,
+                               (const PrPart*)L.d_parts.p, nparts, d_f, L.base.p);
+            MDS_LAUNCHED();
+            return;
+        }
         if (nblk) {
             hipLaunchKernelGGL(k_pr_dangle, dim3(dangle_wgs), dim3(256), 0, s, r, (const PrBlock*)L.d_blocks.p, nblk,
                                L.partials.p);
