@@ -63,7 +63,7 @@ enum MessageMode { EDGE_MESSAGE, VERTEX_MESSAGE, NO_MESSAGE };
 enum MedusaDS { DS_MEG, DS_AA, DS_HY, DS_ELL };
 // combiner element type / operator (the reference passes CUDPP enums, MPG/main.cu:42-43)
 enum MedusaCombinerType { MEDUSA_COMBINE_FLOAT, MEDUSA_COMBINE_INT, MEDUSA_COMBINE_UINT };
-enum MedusaCombinerOp { MEDUSA_COMBINE_ADD, MEDUSA_COMBINE_MIN, MEDUSA_COMBINE_MAX };
+enum MedusaCombinerOp { MEDUSA_COMBINE_ADD, MEDUSA_COMBINE_MIN, MEDUSA_COMBINE_MAX, MEDUSA_COMBINE_MUL };
 // functor kinds (MultipleGPU/MultiAPIHoster.h:18)
 enum EMVAPIType { EDGE, EDGELIST, MESSAGE, MESSAGELIST, VERTEX, EDGE_HT };
 
@@ -389,7 +389,12 @@ __device__ inline void kahan_combine_(float& s, float& c, float s2, float c2) {
 }
 template <class T>
 __device__ inline T combine_op(T a, T b, int op) {
-    return op == MEDUSA_COMBINE_ADD ? a + b : (op == MEDUSA_COMBINE_MIN ? (b < a ? b : a) : (b > a ? b : a));
+    switch (op) {
+        case MEDUSA_COMBINE_ADD: return a + b;
+        case MEDUSA_COMBINE_MUL: return a * b;
+        case MEDUSA_COMBINE_MIN: return b < a ? b : a;
+        default: return b > a ? b : a;
+    }
 }
 template <class T>
 __global__ void combine_kernel(int n, const int* __restrict__ msg_index, const T* __restrict__ msg, T* __restrict__ out,

@@ -64,7 +64,8 @@ typedef struct medusa_opts {
     const medusa_policy* policy; /* learned scheduler; NULL = plain engine */
     uint32_t use_graph;    /* nonzero: run the fixed superstep sequence as one captured hipGraph (PageRank) */
     uint32_t grid;         /* workgroups of the BFS/MBFS frontier-expansion launch; 0 = default (or tuned) */
-    uint32_t reserved[6];
+    uint32_t far_limit;    /* SSSP: far-queue entries kept before it is rebuilt from the distances; 0 = m+n */
+    uint32_t reserved[5];
 } medusa_opts;
 
 /* Per-run statistics.  Reference: printf of wall time / steps (Algorithm/main.cu:316,355,

@@ -58,7 +58,8 @@ class Opts(ctypes.Structure):
         ("policy", ctypes.c_void_p),
         ("use_graph", ctypes.c_uint32),
         ("grid", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 6),
+        ("far_limit", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 5),
     ]
 
 

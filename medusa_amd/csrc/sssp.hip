@@ -63,21 +63,6 @@ __global__ void k_sssp_zero(SsspCounters* c, uint32_t keep_far) {
     c->relax = 0;
 }
 
-// Appends v (where `take`) to queue q through one atomic per wave.
-__device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t* __restrict__ q, uint32_t* counter,
-                                            uint32_t cap) {
-    const unsigned long long mask = __ballot(take);
-    if (!mask) return;
-    uint32_t base = 0;
-    const uint32_t leader = __builtin_ctzll(mask);
-    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, kWave);
-    if (take) {
-        const uint32_t pos = base + lanes_below(mask);
-        if (pos < cap) q[pos] = v;
-    }
-}
-
 // ---- workgroup-staged queue appends ----------------------------------------------------
 // Appends from a workgroup collect in LDS and go to HBM in bulk, one atomic per queue per
 // flush, instead of one atomic per wave per 64 edges on a single counter (which serialises
@@ -285,7 +270,8 @@ __global__ __launch_bounds__(256) void k_sssp_relax(const unsigned long long* __
 
 // Split of the far queue at a new threshold (host-driven loop).
 __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__ far_in, uint32_t nfar,
-                                                    const uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
+                                                    uint32_t rescan, const uint32_t* __restrict__ dist,
+                                                    uint32_t* __restrict__ stamp,
                                                     uint32_t epoch, uint32_t old_thr, uint32_t thr,
                                                     const uint32_t* __restrict__ row_ptr,
                                                     unsigned long long* __restrict__ near_q, uint32_t near_cap,
@@ -295,26 +281,13 @@ __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__
     stage_init(st);
     const Sinks k{near_q, &ctr->n_near, near_cap, far_out, &ctr->n_far, far_cap, row_ptr};
     unsigned long long ndeg = 0;
-    uint32_t fmin = split_block(far_in, nfar, false, dist, stamp, epoch, old_thr, thr, st, k, ndeg);
+    uint32_t fmin = split_block(far_in, nfar, rescan != 0, dist, stamp, epoch, old_thr, thr, st, k, ndeg);
     stage_flush(st, k, true);
     ndeg = wave_sum_u64(ndeg);
     fmin = wave_min_u32(fmin);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (ndeg) atomicAdd(&ctr->near_deg, ndeg);
         if (fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
-    }
-}
-
-// Rebuilds the far queue from scratch after an overflow: every reached vertex
-// with distance >= thr.
-__global__ __launch_bounds__(256) void k_sssp_refill(uint32_t n, const uint32_t* __restrict__ dist, uint32_t thr,
-                                                     uint32_t* __restrict__ far_q, uint32_t far_cap,
-                                                     SsspCounters* __restrict__ ctr) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t v0 = blockIdx.x * blockDim.x; v0 < n; v0 += stride) {
-        const uint32_t v = v0 + threadIdx.x;
-        const bool take = v < n && dist[v] != 0xFFFFFFFFu && dist[v] >= thr;
-        wave_append(take, v, far_q, &ctr->n_far, far_cap);
     }
 }
 
@@ -427,12 +400,12 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
         c.far_sel ^= 1u;
         n_far = min(k.far_out, far_cap);
         c.n_far = n_far;
-        c.overflow = 0;
+        c.overflow = k.far_out > far_cap ? 1u : 0u;  // kept entries beyond the cap: rescan next time
         c.epoch += 1;
         c.n_near = k.n_near;
         if (k.n_near > 0) {
             c.split_pending = 0;
-        } else if (n_far == 0) {
+        } else if (n_far == 0 && !c.overflow) {
             c.done = 1;
         } else {  // empty bucket: restart from the smallest far distance
             const uint32_t d = sssp_pick_delta(c, n_far, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
@@ -501,6 +474,14 @@ __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, un
     c->relax = 0;
 }
 
+// Far-queue capacity: m + n entries, or the caller's smaller limit (a full queue is rebuilt
+// from the distances, so any limit >= 1 gives the same result).
+static size_t sssp_far_alloc(const medusa_graph* g) { return (size_t)std::max<uint64_t>(g->m, g->n) + g->n; }
+static size_t sssp_far_cap(const medusa_graph* g, const medusa_opts& o) {
+    const size_t full = sssp_far_alloc(g);
+    return o.far_limit ? std::min<size_t>(o.far_limit, full) : full;
+}
+
 // Mean weight (sampled) and the default bucket width, a small multiple of it.
 static void sssp_widths(const medusa_graph* g, uint32_t& wmean, uint32_t& dflt) {
     uint64_t wsum = 0, wcnt = 0;
@@ -514,7 +495,7 @@ static void sssp_widths(const medusa_graph* g, uint32_t& wmean, uint32_t& dflt) 
 static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const medusa_opts& o, medusa_stats& st) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    const size_t far_cap = (size_t)std::max<uint64_t>(m, n) + n;
+    const size_t far_cap = sssp_far_cap(g, o);
     hipStream_t s = g->stream;
     uint32_t wmean, dflt;
     sssp_widths(g, wmean, dflt);
@@ -534,7 +515,7 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     static_assert(sizeof(SsspCtl) <= sizeof(CtlStorage), "control block storage");
     SsspDevCounters* ctr = dev_counters<SsspDevCounters>(g);
     uint32_t* far0 = g->q_aux.p;
-    uint32_t* far1 = g->q_aux.p + far_cap;
+    uint32_t* far1 = g->q_aux.p + sssp_far_alloc(g);
     const size_t near_cap = near_capacity(n, m);
     unsigned long long* near0 = g->near_q.p;
     unsigned long long* near1 = g->near_q.p + near_cap;
@@ -595,7 +576,7 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
 static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const medusa_opts& o, medusa_stats& st) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
-    const size_t far_cap = (size_t)std::max<uint64_t>(m, n) + n;
+    const size_t far_cap = sssp_far_cap(g, o);
     SsspCounters* C = dev_counters<SsspCounters>(g);
     hipStream_t s = g->stream;
     g->trace.clear();
@@ -611,7 +592,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     unsigned long long* near_cur = g->near_q.p;
     unsigned long long* near_nxt = g->near_q.p + near_cap;
     uint32_t* far_a = g->q_aux.p;
-    uint32_t* far_b = g->q_aux.p + far_cap;
+    uint32_t* far_b = g->q_aux.p + sssp_far_alloc(g);
     const uint32_t src_slices = slices_of(g->h_row_ptr[src + 1] - g->h_row_ptr[src]);
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
@@ -621,6 +602,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     hipLaunchKernelGGL(k_sssp_zero, dim3(1), dim3(1), 0, s, C, 0u);
     MDS_LAUNCHED();
     uint32_t n_near = src_slices, n_far = 0, epoch = 1;
+    bool far_incomplete = false;  // the far queue hit its cap: the next split rescans all distances
     uint64_t near_deg = g->h_row_ptr[src + 1] - g->h_row_ptr[src];
     uint64_t relaxed = 0;
     const bool timing = o.timing != 0;
@@ -646,7 +628,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     };
     for (uint32_t step = 0;; ++step) {
         if (n_near == 0) {
-            if (n_far == 0) break;
+            if (n_far == 0 && !far_incomplete) break;
             close_phase();
             // advance the threshold and split the far queue; skip buckets that come out empty
             uint32_t old_thr = thr;
@@ -664,16 +646,19 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
                 ++epoch;
                 hipLaunchKernelGGL(k_sssp_zero, dim3(1), dim3(1), 0, s, C, 0u);
                 MDS_LAUNCHED();
-                hipLaunchKernelGGL(k_sssp_split, dim3(grid_for(n_far, 256, 4096)), dim3(256), 0, s, far_a, n_far,
+                const uint32_t nin = far_incomplete ? n : n_far;
+                hipLaunchKernelGGL(k_sssp_split, dim3(grid_for(nin, 256, 4096)), dim3(256), 0, s, far_a, nin,
+                                   far_incomplete ? 1u : 0u,
                                    g->dist.p, g->stamp.p, epoch, old_thr, thr, g->row_ptr.p, near_cur,
                                    (uint32_t)near_cap, far_b, (uint32_t)far_cap, C);
                 MDS_LAUNCHED();
                 const SsspCounters sc = read_counters<SsspCounters>(g);
                 std::swap(far_a, far_b);
                 n_near = sc.n_near;
+                far_incomplete = sc.n_far > far_cap;
                 n_far = std::min<uint32_t>(sc.n_far, (uint32_t)far_cap);
                 near_deg = sc.near_deg;
-                if (n_near || n_far == 0) break;
+                if (n_near || (n_far == 0 && !far_incomplete)) break;
                 old_thr = thr;
                 if (sc.far_min != 0xFFFFFFFFu && sc.far_min > thr) thr = sc.far_min;
             }
@@ -708,14 +693,9 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         std::swap(near_cur, near_nxt);
         n_near = c.n_near;
         near_deg = c.near_deg;
-        if (c.n_far > far_cap) {
-            // far queue overflowed: rebuild it from the distances
-            hipLaunchKernelGGL(k_sssp_zero, dim3(1), dim3(1), 0, s, C, 0u);
-            MDS_LAUNCHED();
-            hipLaunchKernelGGL(k_sssp_refill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->dist.p, thr,
-                               far_a, (uint32_t)far_cap, C);
-            MDS_LAUNCHED();
-            n_far = read_counters<SsspCounters>(g).n_far;
+        if (c.n_far > far_cap) {  // the far queue is full: keep what fits, rescan at the next split
+            far_incomplete = true;
+            n_far = (uint32_t)far_cap;
         } else {
             n_far = c.n_far;
         }
@@ -776,7 +756,7 @@ extern "C" int medusa_sssp(medusa_graph* g, uint32_t source, uint32_t* dist, con
         medusa_opts o{};
         if (opts) o = *opts;
         const uint32_t n = g->n;
-        const size_t far_cap = (size_t)std::max<uint64_t>(g->m, n) + n;
+        const size_t far_cap = sssp_far_alloc(g);
         g->dist.ensure(n);
         g->stamp.ensure(n);
         g->near_q.ensure(2 * near_capacity(n, g->m));  // near queues A and B ((vertex, slice) entries)

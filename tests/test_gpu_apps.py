@@ -97,3 +97,33 @@ def test_mbfs_app(tmp_path, oracle):
     run("mbfs_app", g, out, *srcs)
     got = np.fromfile(out, np.uint8).reshape(len(srcs), -1)
     assert np.array_equal(got, oracle.mbfs(rp, col, np.array(srcs, np.uint32)))
+
+
+@pytest.mark.parametrize("kind", ["int", "float"])
+def test_combiner_ops_app(tmp_path, kind):
+    """Every user-API combiner operator (the reference's CUDPP ADD/MULTIPLY/MIN/MAX) over
+    each vertex's in-edge weights; vertices without in-edges read the init value -1."""
+    rng = np.random.default_rng(5)
+    n, m = 3000, 6000
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    from conftest import csr_from_edges
+    w = rng.integers(1, 4, m).astype(np.uint32)   # small weights keep products in range
+    rp, col, wc = csr_from_edges(n, np.stack([src, dst], 1), w)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col, wc)
+    dt = np.int32 if kind == "int" else np.float32
+    indeg = np.bincount(col, minlength=n)
+    vals = wc.astype(np.float64)
+    for op in ("add", "min", "max", "mul"):
+        out = tmp_path / f"{op}.bin"
+        run(f"combine_{kind}_app", g, out, op)
+        got = np.fromfile(out, dt).astype(np.float64)
+        want = np.full(n, -1.0)
+        for v in np.flatnonzero(indeg):
+            xs = vals[col == v]
+            want[v] = {"add": xs.sum(), "min": xs.min(), "max": xs.max(), "mul": np.prod(xs)}[op]
+        if kind == "float" and op in ("add", "mul"):
+            np.testing.assert_allclose(got, want, rtol=1e-6)
+        else:
+            assert np.array_equal(got, want), op

@@ -209,3 +209,55 @@ def test_mbfs_rmat20_config(oracle):
         assert np.array_equal(out, want)
         deg = np.diff(rp.astype(np.int64))
         assert st.edges_traversed == sum(deg[want[i] != 255].sum() for i in range(len(srcs)))
+
+
+@pytest.mark.parametrize("far_limit", [1, 7, 64])
+def test_sssp_far_queue_limit_rebuilds(oracle, far_limit):
+    """A far queue capped far below its need is rebuilt from the distances at every split;
+    distances stay exact on both drivers and for every bucket width."""
+    rp, col, _ = random_graph(6000, 36000, 31)
+    w = np.random.default_rng(31).integers(1, 1001, len(col)).astype(np.uint32)
+    want = oracle.sssp_dijkstra(rp, col, w, 7)
+    with md.Graph(rp, col, w) as G:
+        for delta in (0, 50, 400):
+            for timing in (0, 1):
+                d, _ = G.sssp(7, delta=delta, far_limit=far_limit, timing=timing)
+                assert np.array_equal(d, want), (delta, timing)
+
+
+def test_sssp_zero_weights_duplicates_and_self_loops(oracle):
+    """The reference draws weights from rand32() % 1000, so 0 occurs; duplicates and self
+    loops are kept."""
+    rng = np.random.default_rng(8)
+    n, m = 4000, 30000
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    src[:500] = dst[:500]                         # self loops
+    src[500:1500] = src[1500:2500]                # duplicate arcs
+    dst[500:1500] = dst[1500:2500]
+    w = rng.integers(0, 1000, m).astype(np.uint32)
+    w[::7] = 0
+    rp, col, wc = csr_from_edges(n, np.stack([src, dst], 1), w)
+    want = oracle.sssp_dijkstra(rp, col, wc, 11)
+    with md.Graph(rp, col, wc) as G:
+        for delta in (0, 1, 37, 0xFFFFFFFF):
+            for timing in (0, 1):
+                d, _ = G.sssp(11, delta=delta, timing=timing)
+                assert np.array_equal(d, want), (delta, timing)
+
+
+def test_sssp_hub_source_slices(oracle):
+    """A source whose out-degree spans many near-queue slices, plus hubs inside the graph."""
+    rng = np.random.default_rng(12)
+    n = 50000
+    hub = np.stack([np.zeros(20000, np.int64), rng.integers(1, n, 20000)], 1)
+    hubs2 = np.stack([np.repeat(rng.integers(0, n, 8), 3000), rng.integers(0, n, 24000)], 1)
+    rest = rng.integers(0, n, (150000, 2))
+    rp, col, _ = csr_from_edges(n, np.concatenate([hub, hubs2, rest]))
+    w = rng.integers(1, 1001, len(col)).astype(np.uint32)
+    want = oracle.sssp_dijkstra(rp, col, w, 0)
+    with md.Graph(rp, col, w) as G:
+        for delta in (0, 100, 0xFFFFFFFF):
+            for timing in (0, 1):
+                d, _ = G.sssp(0, delta=delta, timing=timing)
+                assert np.array_equal(d, want), (delta, timing)
