@@ -65,7 +65,6 @@ __global__ void k_bfs_reset(uint32_t n, int32_t* __restrict__ level, uint32_t si
         ctr->n_small = 0;
         ctr->n_large = 0;
         ctr->n_new = 0;
-        ctr->pad = 0;
         ctr->deg_sum = 0;
         ctr->probes = 0;
         ctr->deg_large = 0;
@@ -81,7 +80,6 @@ __global__ void k_zero_counters(FrontierCounters* c) {
     c->n_small = 0;
     c->n_large = 0;
     c->n_new = 0;
-    c->pad = 0;
     c->deg_sum = 0;
     c->probes = 0;
     c->deg_large = 0;
@@ -243,15 +241,26 @@ __device__ __forceinline__ void compact(uint32_t n, const int32_t* __restrict__ 
         unsigned long long msm[kCompactK], mlg[kCompactK];
         uint32_t csm = 0, clg = 0, cnew = 0;
         unsigned long long dsum = 0, dlg = 0;
+        // all loads of the tile first (independent, so they overlap), then the ballots
+        int32_t lv[kCompactK];
+        uint32_t deg_of[kCompactK];
+#pragma unroll
+        for (uint32_t k = 0; k < kCompactK; ++k) {
+            const uint32_t v = (tile * (4 * kCompactK) + k * 4 + wv) * kWave + lane;
+            lv[k] = v < n ? level[v] : -1;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kCompactK; ++k) {
+            const uint32_t v = (tile * (4 * kCompactK) + k * 4 + wv) * kWave + lane;
+            deg_of[k] = lv[k] == want ? row_ptr[v + 1] - row_ptr[v] : 0u;
+        }
 #pragma unroll
         for (uint32_t k = 0; k < kCompactK; ++k) {
             const uint32_t chunk = tile * (4 * kCompactK) + k * 4 + wv;
-            const uint32_t v = chunk * kWave + lane;
-            const bool pred = v < n && level[v] == want;
+            const bool pred = lv[k] == want;
             const unsigned long long mask = __ballot(pred);
             if (lane == 0 && chunk < nwords) bitmap[chunk] = mask;
-            uint32_t deg = 0;
-            if (pred) deg = row_ptr[v + 1] - row_ptr[v];
+            const uint32_t deg = deg_of[k];
             msm[k] = __ballot(pred && deg > 0 && deg <= large);
             mlg[k] = __ballot(pred && deg > large);
             cnew += __popcll(mask);

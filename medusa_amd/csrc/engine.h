@@ -85,15 +85,18 @@ struct HBuf {
     }
 };
 
+// A counter that workgroups update with atomics sits on its own 128-byte line: atomics on
+// counters that share a line serialise on that line.
+#define MDS_COUNTER(type, name) alignas(128) type name
+
 // Frontier counters written by the compaction kernels, read back each superstep.
 struct FrontierCounters {
-    uint32_t n_small;    // frontier vertices with 1 <= outdeg <= large threshold (wave-cooperative queue)
-    uint32_t n_large;    // frontier vertices with outdeg > threshold (workgroup queue)
-    uint32_t n_new;      // vertices that joined the frontier (any outdeg)
-    uint32_t pad;
-    unsigned long long deg_sum;    // sum of their out-degrees
-    unsigned long long probes;     // in-edges examined by a pull superstep
-    unsigned long long deg_large;  // out-degree sum of the workgroup (hub) queue
+    MDS_COUNTER(uint32_t, n_small);              // frontier vertices with 1 <= outdeg <= large (wave queue)
+    MDS_COUNTER(uint32_t, n_large);              // frontier vertices with outdeg > large (workgroup queue)
+    MDS_COUNTER(uint32_t, n_new);                // vertices that joined the frontier (any outdeg)
+    MDS_COUNTER(unsigned long long, deg_sum);    // sum of their out-degrees
+    MDS_COUNTER(unsigned long long, probes);     // in-edges examined by a pull superstep
+    MDS_COUNTER(unsigned long long, deg_large);  // out-degree sum of the workgroup (hub) queue
 };
 
 // One contiguous run of PageRank vertices handled with the same lane mapping.
@@ -241,7 +244,7 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_wg, uint32_t cap = 8192) {
 }
 void build_pr_layout(medusa_graph* g);
 
-constexpr size_t kCounterWords = 32;  // 256 bytes
+constexpr size_t kCounterWords = 256;  // 2 KiB
 template <class T>
 T* dev_counters(medusa_graph* g) {
     static_assert(sizeof(T) <= kCounterWords * 8, "counter struct too large");

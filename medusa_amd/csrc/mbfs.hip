@@ -36,15 +36,15 @@ constexpr uint32_t kMbfsBatch = 8;
 constexpr uint32_t kMbfsSplit = 8;  // minimum workgroup items per hub
 
 struct MbfsCounters {
-    uint32_t n_small;    // active vertices with 1 <= outdeg <= large
-    uint32_t n_large;    // active vertices with outdeg > large
-    uint32_t n_active;   // all active vertices
-    uint32_t saturated;  // a level >= 255 was clamped to 0xFF
-    unsigned long long active_deg;
-    unsigned long long new_bits;
-    unsigned long long reached_edges;  // sum over new (source, vertex) pairs of outdeg(vertex)
-    unsigned long long probes;
-    unsigned long long deg_large;      // out-degree sum of the hub queue
+    MDS_COUNTER(uint32_t, n_small);    // active vertices with 1 <= outdeg <= large
+    MDS_COUNTER(uint32_t, n_large);    // active vertices with outdeg > large
+    MDS_COUNTER(uint32_t, n_active);   // all active vertices
+    MDS_COUNTER(uint32_t, saturated);  // a level >= 255 was clamped to 0xFF
+    MDS_COUNTER(unsigned long long, active_deg);
+    MDS_COUNTER(unsigned long long, new_bits);
+    MDS_COUNTER(unsigned long long, reached_edges);  // sum over new (source, vertex) pairs of outdeg(vertex)
+    MDS_COUNTER(unsigned long long, probes);
+    MDS_COUNTER(unsigned long long, deg_large);      // out-degree sum of the hub queue
 };
 
 struct MbfsCtl {
@@ -251,6 +251,7 @@ __global__ __launch_bounds__(256) void k_mbfs_update(const MbfsCtl* __restrict__
                                                      uint8_t* __restrict__ level, uint32_t* __restrict__ q_small,
                                                      uint32_t* __restrict__ q_large, MbfsCounters* __restrict__ ctr) {
     __shared__ uint32_t s_cnt[4][3];
+    __shared__ unsigned long long s_sum[4][4];
     __shared__ uint32_t s_base[2];
     if (ctl && ctl->done) return;
     const int32_t level_val = ctl ? ctl->step + 1 : level_fixed;
@@ -260,25 +261,42 @@ __global__ __launch_bounds__(256) void k_mbfs_update(const MbfsCtl* __restrict__
         unsigned long long msm[4], mlg[4];
         uint32_t csm = 0, clg = 0, cact = 0;
         unsigned long long adeg = 0, bits = 0, redges = 0, dlg = 0;
+        // loads of the tile first (independent across chunks, so they overlap)
+        unsigned long long raw[4], fr[4], vis[4];
+        uint32_t dg[4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint32_t v = (tile * 16 + c * 4 + wv) * kWave + lane;
+            raw[c] = v < n ? next[v] : 0ull;
+            fr[c] = v < n ? frontier[v] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint32_t v = (tile * 16 + c * 4 + wv) * kWave + lane;
+            vis[c] = raw[c] ? visited[v] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+            const uint32_t v = (tile * 16 + c * 4 + wv) * kWave + lane;
+            dg[c] = (raw[c] & ~vis[c]) ? row_ptr[v + 1] - row_ptr[v] : 0u;
+        }
 #pragma unroll
         for (uint32_t c = 0; c < 4; ++c) {
             const uint32_t v = (tile * 16 + c * 4 + wv) * kWave + lane;
             unsigned long long nx = 0;
             if (v < n) {
-                const unsigned long long raw = next[v];
-                const unsigned long long fr = frontier[v];
-                if (raw) {
+                if (raw[c]) {
                     next[v] = 0;
-                    nx = raw & ~visited[v];
+                    nx = raw[c] & ~vis[c];
                 }
-                if (nx) visited[v] |= nx;
-                if (fr != nx) frontier[v] = nx;
+                if (nx) visited[v] = vis[c] | nx;
+                if (fr[c] != nx) frontier[v] = nx;
             }
             const unsigned long long amask = __ballot(nx != 0);
             msm[c] = 0;
             mlg[c] = 0;
             if (amask) {
-                const uint32_t deg = nx ? row_ptr[v + 1] - row_ptr[v] : 0u;
+                const uint32_t deg = nx ? dg[c] : 0u;
                 msm[c] = __ballot(nx != 0 && deg > 0 && deg <= large);
                 mlg[c] = __ballot(nx != 0 && deg > large);
                 adeg += deg;
@@ -299,32 +317,38 @@ __global__ __launch_bounds__(256) void k_mbfs_update(const MbfsCtl* __restrict__
                 clg += __popcll(mlg[c]);
             }
         }
-        if (lane == 0) {
-            s_cnt[wv][0] = csm;
-            s_cnt[wv][1] = clg;
-            s_cnt[wv][2] = cact;
-        }
         adeg = wave_sum_u64(adeg);
         bits = wave_sum_u64(bits);
         redges = wave_sum_u64(redges);
         dlg = wave_sum_u64(dlg);
-        if (lane == 0 && (adeg | bits)) {
-            atomicAdd(&ctr->active_deg, adeg);
-            atomicAdd(&ctr->new_bits, bits);
-            atomicAdd(&ctr->reached_edges, redges);
-            if (dlg) atomicAdd(&ctr->deg_large, dlg);
+        if (lane == 0) {
+            s_cnt[wv][0] = csm;
+            s_cnt[wv][1] = clg;
+            s_cnt[wv][2] = cact;
+            s_sum[wv][0] = adeg;
+            s_sum[wv][1] = bits;
+            s_sum[wv][2] = redges;
+            s_sum[wv][3] = dlg;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0) {  // one set of atomics per tile
             uint32_t tsm = 0, tlg = 0, tact = 0;
+            unsigned long long t[4] = {0, 0, 0, 0};
             for (int w = 0; w < 4; ++w) {
                 tsm += s_cnt[w][0];
                 tlg += s_cnt[w][1];
                 tact += s_cnt[w][2];
+                for (int j = 0; j < 4; ++j) t[j] += s_sum[w][j];
             }
             s_base[0] = tsm ? atomicAdd(&ctr->n_small, tsm) : 0u;
             s_base[1] = tlg ? atomicAdd(&ctr->n_large, tlg) : 0u;
-            if (tact) atomicAdd(&ctr->n_active, tact);
+            if (tact) {
+                atomicAdd(&ctr->n_active, tact);
+                atomicAdd(&ctr->active_deg, t[0]);
+                atomicAdd(&ctr->new_bits, t[1]);
+                atomicAdd(&ctr->reached_edges, t[2]);
+                if (t[3]) atomicAdd(&ctr->deg_large, t[3]);
+            }
             if (level_val >= 255 && tact) ctr->saturated = 1;
         }
         __syncthreads();

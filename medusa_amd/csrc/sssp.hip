@@ -29,12 +29,11 @@
 namespace medusa {
 
 struct SsspCounters {
-    uint32_t n_near;   // entries in the next near queue
-    uint32_t n_far;    // entries in the far queue
-    uint32_t far_min;  // smallest distance among far entries kept by a split
-    uint32_t pad;
-    unsigned long long near_deg;  // out-edges of the next near queue
-    unsigned long long relax;     // edges relaxed this superstep
+    MDS_COUNTER(uint32_t, n_near);              // entries in the next near queue
+    MDS_COUNTER(uint32_t, n_far);               // entries in the far queue
+    MDS_COUNTER(uint32_t, far_min);             // smallest distance among far entries kept by a split
+    MDS_COUNTER(unsigned long long, near_deg);  // out-edges of the next near queue
+    MDS_COUNTER(unsigned long long, relax);     // edges relaxed this superstep
 };
 
 constexpr uint32_t kSlice = 128;       // out-edges per near-queue entry
@@ -58,7 +57,6 @@ __global__ void k_sssp_zero(SsspCounters* c, uint32_t keep_far) {
     c->n_near = 0;
     if (!keep_far) c->n_far = 0;
     c->far_min = 0xFFFFFFFFu;
-    c->pad = 0;
     c->near_deg = 0;
     c->relax = 0;
 }
@@ -314,11 +312,11 @@ struct SsspCtl {
 };
 
 struct SsspDevCounters {
-    uint32_t n_near;   // near entries written by this slot's work kernel
-    uint32_t n_far;    // far queue length (relax appends after the current entries)
-    uint32_t far_out;  // far entries kept by a split
-    uint32_t far_min;  // smallest distance kept by a split
-    unsigned long long relax;
+    MDS_COUNTER(uint32_t, n_near);   // near entries written by this slot's work kernel
+    MDS_COUNTER(uint32_t, n_far);    // far queue length (relax appends after the current entries)
+    MDS_COUNTER(uint32_t, far_out);  // far entries kept by a split
+    MDS_COUNTER(uint32_t, far_min);  // smallest distance kept by a split
+    MDS_COUNTER(unsigned long long, relax);
 };
 
 // Bucket width at a threshold advance; all threads of the block call it, thread 0's value counts.
