@@ -219,8 +219,15 @@ __device__ __forceinline__ void pull_all(uint32_t n, const uint32_t* __restrict_
             }
         }
     }
+    // one atomic per workgroup (every wave of the block reaches this point)
+    __shared__ unsigned long long s_probe[4];
     probes = wave_sum_u64(probes);
-    if (lane == 0 && probes) atomicAdd(&ctr->probes, probes);
+    if (lane == 0) s_probe[threadIdx.x / kWave] = probes;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = s_probe[0] + s_probe[1] + s_probe[2] + s_probe[3];
+        if (t) atomicAdd(&ctr->probes, t);
+    }
 }
 
 constexpr uint32_t kCompactK = 8;                         // 64-vertex chunks per wave per tile
@@ -368,8 +375,7 @@ __global__ __launch_bounds__(256) void k_bfs_compact_dev(const BfsCtl* __restric
 __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, uint32_t n, uint32_t nwords,
                                                     FrontierCounters* __restrict__ ctr,
                                                     const float* __restrict__ policy, uint32_t grid) {
-    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
-    __shared__ float s_cost[2];
+    __shared__ MlpShared s_mlp;
     if (ctl->done) return;
     const uint32_t n_new = ctr->n_new;
     const uint32_t n_small = ctr->n_small;
@@ -406,18 +412,11 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
         } else if (c.mode == 3 && policy) {
             const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                      (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
-            for (int a = 0; a < 2; ++a) {
-                if (threadIdx.x == 0) {
-                    float x[MEDUSA_POLICY_INPUTS];
-                    direction_row(rec, (double)a, x);
-                    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
-                }
-                __syncthreads();
-                const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
-                if (threadIdx.x == 0) s_cost[a] = cost;
-                __syncthreads();
-            }
-            pull = s_cost[1] < s_cost[0];
+            policy_direction_costs(rec, policy, s_mlp);
+            // the heuristic's choice stands unless the policy predicts a clear gain
+            const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlpha
+                                     : !((double)c.n_f < (double)c.n / Sched::kBeta);
+            pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain);
         } else {
             // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
             if (!pull)
@@ -448,8 +447,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
 __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ ctl, FrontierCounters* __restrict__ ctr,
                                                           const float* __restrict__ policy, uint32_t n,
                                                           unsigned long long m, uint32_t mode, uint32_t grid) {
-    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
-    __shared__ float s_cost[2];
+    __shared__ MlpShared s_mlp;
     BfsCtl c{};
     c.mode = mode;
     c.n = n;
@@ -472,18 +470,8 @@ __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ c
     } else if (c.mode == 3 && policy) {
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
-        for (int a = 0; a < 2; ++a) {
-            if (threadIdx.x == 0) {
-                float x[MEDUSA_POLICY_INPUTS];
-                direction_row(rec, (double)a, x);
-                for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
-            }
-            __syncthreads();
-            const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
-            if (threadIdx.x == 0) s_cost[a] = cost;
-            __syncthreads();
-        }
-        pull = s_cost[1] < s_cost[0];
+        policy_direction_costs(rec, policy, s_mlp);
+        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlpha);
     } else if (c.mode == 0) {
         pull = (double)c.m_f > (double)c.m_u / Sched::kAlpha;
     }

@@ -211,8 +211,15 @@ __device__ __forceinline__ void mbfs_pull_all(uint32_t n, unsigned long long ful
             if (lane == 0 && acc) next[base + owner] = acc;
         }
     }
+    // one atomic per workgroup (every wave of the block reaches this point)
+    __shared__ unsigned long long s_probe[4];
     probes = wave_sum_u64(probes);
-    if (lane == 0 && probes) atomicAdd(&ctr->probes, probes);
+    if (lane == 0) s_probe[threadIdx.x / kWave] = probes;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = s_probe[0] + s_probe[1] + s_probe[2] + s_probe[3];
+        if (t) atomicAdd(&ctr->probes, t);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mbfs_expand(const MbfsCtl* __restrict__ ctl,
@@ -371,25 +378,16 @@ __global__ __launch_bounds__(256) void k_mbfs_update(const MbfsCtl* __restrict__
 
 // Direction of the next superstep: fixed, learned (two MLP evaluations by the block), or the
 // direction-optimising heuristic.  All threads of the block call it.
-__device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restrict__ policy, float* s_x, float* s_a1,
-                               float* s_a2, float* s_cost) {
+__device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restrict__ policy, MlpShared& m) {
     if (c.mode == 1) return false;
     if (c.mode == 2) return true;
     if (c.mode == 3 && policy) {
         const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                  (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
-        for (int a = 0; a < 2; ++a) {
-            if (threadIdx.x == 0) {
-                float x[MEDUSA_POLICY_INPUTS];
-                direction_row(rec, (double)a, x);
-                for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
-            }
-            __syncthreads();
-            const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
-            if (threadIdx.x == 0) s_cost[a] = cost;
-            __syncthreads();
-        }
-        return s_cost[1] < s_cost[0];
+        policy_direction_costs(rec, policy, m);
+        const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlpha
+                                 : !((double)c.n_f < (double)c.n / Sched::kBeta);
+        return policy_pull(m.out[0], m.out[1], plain);
     }
     if (!pull) return (double)c.m_f > (double)c.m_u / Sched::kAlpha;
     return !((double)c.n_f < (double)c.n / Sched::kBeta);
@@ -399,8 +397,7 @@ __device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restr
 // into the control block and decides the next superstep.
 __global__ __launch_bounds__(256) void k_mbfs_decide(MbfsCtl* __restrict__ ctl, MbfsCounters* __restrict__ ctr,
                                                      const float* __restrict__ policy, uint32_t grid, uint32_t first) {
-    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH];
-    __shared__ float s_cost[2];
+    __shared__ MlpShared s_mlp;
     if (ctl->done) return;
     const MbfsCounters k = *ctr;
     MbfsCtl c = *ctl;
@@ -429,7 +426,7 @@ __global__ __launch_bounds__(256) void k_mbfs_decide(MbfsCtl* __restrict__ ctl, 
     c.n_active = k.n_active;
     c.hub_wgs = mbfs_hub_wgs(grid, k.n_small, k.n_large, k.deg_large, k.active_deg);
     if (k.n_active == 0) c.done = 1;
-    const bool pull = c.done ? false : mbfs_direction(c, first ? false : c.pull != 0, policy, s_x, s_a1, s_a2, s_cost);
+    const bool pull = c.done ? false : mbfs_direction(c, first ? false : c.pull != 0, policy, s_mlp);
     c.pull = pull ? 1u : 0u;
     __syncthreads();  // every wave has read the control block and counters
     if (threadIdx.x == 0) {

@@ -13,6 +13,19 @@ constexpr int kMlpIn = MEDUSA_POLICY_INPUTS;  // 16
 constexpr int kMlpH = 32;
 constexpr uint32_t kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1;
 constexpr float kMlpSlope = 1.0f / 64.0f;
+// The policy overrides the plain engine's choice only when it predicts the other action to
+// be cheaper by more than a margin (log2 units of cost); otherwise the plain engine's choice
+// stands, so an uncertain model falls back to it.  Direction: 0.5 (about 41 %); bucket
+// width: 0.25 (about 19 %).
+constexpr float kPolicyMargin = 0.25f;
+constexpr float kDirectionMargin = 0.5f;
+
+// Direction from the two predicted costs and the plain engine's choice.
+inline __host__ __device__ bool policy_pull(float c_push, float c_pull, bool plain_pull) {
+    if (c_pull + kDirectionMargin < c_push) return true;
+    if (c_push + kDirectionMargin < c_pull) return false;
+    return plain_pull;
+}
 
 float mlp_eval_host(const float* params, const float* x);
 const float* policy_params(const medusa_policy* p);
@@ -24,34 +37,41 @@ uint32_t policy_kind(const medusa_policy* p);
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 namespace medusa {
-// A workgroup evaluates the MLP on one input row held in LDS (x[16]): threads 0..31
-// each compute one hidden unit, thread 0 accumulates the output in unit order.  The
-// FP32 operations and their order are those of mlp_eval_host, so host and device pick
-// the same action.  Every thread of the block must call it; thread 0 gets the value.
-__device__ inline float mlp_eval_block(const float* __restrict__ P, const float* x, float* s_a1, float* s_a2) {
-    const int t = threadIdx.x;
-    const float* W1 = P;
+// A workgroup evaluates the MLP on `rows` input rows held in LDS (x[rows][16]) with the
+// parameters staged once into LDS (sP[kMlpParams]).  Hidden units of all rows are computed
+// in parallel, one (row, unit) per thread; out[r] is written for every row.  The FP32
+// operations and their order per row are those of mlp_eval_host, so host and device pick
+// the same action.  Every thread of the block must call it; out[] is valid afterwards.
+__device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, const float* x, int rows, float* s_a1,
+                                     float* s_a2, float* out) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int i = t; i < (int)kMlpParams; i += nt) sP[i] = P[i];
+    __syncthreads();
+    const float* W1 = sP;
     const float* b1 = W1 + kMlpH * kMlpIn;
     const float* W2 = b1 + kMlpH;
     const float* b2 = W2 + kMlpH * kMlpH;
     const float* W3 = b2 + kMlpH;
-    if (t < kMlpH) {
-        float z = b1[t];
-        for (int k = 0; k < kMlpIn; ++k) z = fmaf(W1[t * kMlpIn + k], x[k], z);
-        s_a1[t] = z > 0.f ? z : z * kMlpSlope;
+    for (int j = t; j < rows * kMlpH; j += nt) {
+        const int r = j / kMlpH, i = j % kMlpH;
+        float z = b1[i];
+        for (int k = 0; k < kMlpIn; ++k) z = fmaf(W1[i * kMlpIn + k], x[r * kMlpIn + k], z);
+        s_a1[j] = z > 0.f ? z : z * kMlpSlope;
     }
     __syncthreads();
-    if (t < kMlpH) {
-        float z = b2[t];
-        for (int k = 0; k < kMlpH; ++k) z = fmaf(W2[t * kMlpH + k], s_a1[k], z);
-        s_a2[t] = z > 0.f ? z : z * kMlpSlope;
+    for (int j = t; j < rows * kMlpH; j += nt) {
+        const int r = j / kMlpH, i = j % kMlpH;
+        float z = b2[i];
+        for (int k = 0; k < kMlpH; ++k) z = fmaf(W2[i * kMlpH + k], s_a1[r * kMlpH + k], z);
+        s_a2[j] = z > 0.f ? z : z * kMlpSlope;
     }
     __syncthreads();
-    float o = W3[kMlpH];
-    if (t == 0)
-        for (int i = 0; i < kMlpH; ++i) o = fmaf(W3[i], s_a2[i], o);
+    for (int r = t; r < rows; r += nt) {
+        float o = W3[kMlpH];
+        for (int i = 0; i < kMlpH; ++i) o = fmaf(W3[i], s_a2[r * kMlpH + i], o);
+        out[r] = o;
+    }
     __syncthreads();
-    return o;
 }
 }  // namespace medusa
 #endif

@@ -55,6 +55,24 @@ __host__ __device__ inline void delta_row(const double* r, double action, float*
     x[8] = x[1] * la / 8.f;
 }
 
+#if defined(__HIPCC__)
+// LDS working set of a policy evaluation inside a decide kernel (up to 12 candidate rows).
+struct MlpShared {
+    float P[kMlpParams];
+    float x[12 * MEDUSA_POLICY_INPUTS];
+    float a1[12 * kMlpH];
+    float a2[12 * kMlpH];
+    float out[12];
+};
+
+// Predicted costs of push (out[0]) and pull (out[1]) for the superstep described by `rec`.
+// Every thread of the block calls it.
+__device__ inline void policy_direction_costs(const double* rec, const float* __restrict__ policy, MlpShared& m) {
+    if (threadIdx.x < 2) direction_row(rec, (double)threadIdx.x, m.x + threadIdx.x * MEDUSA_POLICY_INPUTS);
+    mlp_eval_rows(policy, m.P, m.x, 2, m.a1, m.a2, m.out);  // its first barrier publishes the rows
+}
+#endif
+
 // Frontier features at the start of a superstep.
 struct Frontier {
     uint32_t step = 0;
@@ -91,19 +109,22 @@ struct Sched {
         const float c_push = mlp_eval_host(policy_params(o.policy), x);
         policy_row(MEDUSA_POLICY_DIRECTION, rec, 1.0, x);
         const float c_pull = mlp_eval_host(policy_params(o.policy), x);
-        return c_pull < c_push;
+        return policy_pull(c_push, c_pull, heuristic_pull(f, cur_pull));
     }
     // SSSP bucket width at a threshold advance: the learned policy picks among
-    // wmean * 2^k (k = -2..8) the width with the lowest predicted time per unit of distance.
+    // wmean * 2^k (k = -2..8) the width with the lowest predicted time per unit of distance,
+    // if it beats the predicted cost of the default width by the policy margin.
     uint32_t sssp_delta(const Frontier& f, uint32_t dflt, uint32_t wmean, uint32_t thr) const {
         if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_DELTA || o.delta) return dflt;
         double rec[MEDUSA_TRACE_FIELDS];
         to_sssp_record(f, wmean, thr, 0.0, 0.0, rec);
+        float x[MEDUSA_POLICY_INPUTS];
+        policy_row(MEDUSA_POLICY_DELTA, rec, (double)dflt, x);
+        const float c_dflt = mlp_eval_host(policy_params(o.policy), x);
         float best = 0.f;
         uint32_t pick = dflt;
         for (int k = -2; k <= 8; ++k) {
             const double d = std::max(1.0, std::ldexp((double)wmean, k));
-            float x[MEDUSA_POLICY_INPUTS];
             policy_row(MEDUSA_POLICY_DELTA, rec, d, x);
             const float c = mlp_eval_host(policy_params(o.policy), x);
             if (k == -2 || c < best) {
@@ -111,7 +132,7 @@ struct Sched {
                 pick = (uint32_t)d;
             }
         }
-        return pick;
+        return best + kPolicyMargin < c_dflt ? pick : dflt;
     }
     static void to_record(const Frontier& f, double action, double kms, double* r) {
         r[0] = f.step;

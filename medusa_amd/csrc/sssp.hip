@@ -319,28 +319,22 @@ struct SsspDevCounters {
     MDS_COUNTER(unsigned long long, relax);
 };
 
-// Bucket width at a threshold advance; all threads of the block call it, thread 0's value counts.
-__device__ uint32_t sssp_pick_delta(const SsspCtl& c, uint32_t n_far, const float* __restrict__ policy, float* s_x,
-                                    float* s_a1, float* s_a2, float* s_best, uint32_t* s_pick) {
+// Bucket width at a threshold advance: the learned policy's cheapest width among
+// wmean * 2^k (k = -2..8) if it beats the default width by the policy margin, else the
+// default.  All threads of the block call it and get the same value.
+__device__ uint32_t sssp_pick_delta(const SsspCtl& c, uint32_t n_far, const float* __restrict__ policy,
+                                    MlpShared& m) {
     if (c.mode != 2 || !policy) return c.dflt_delta;
     const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.phase, (double)n_far, (double)c.wmean, (double)c.thr, 0.0,
                                              (double)c.n, (double)c.m, 0.0, 0.0};
-    for (int k = -2; k <= 8; ++k) {
-        const double d = fmax(1.0, ldexp((double)c.wmean, k));
-        if (threadIdx.x == 0) {
-            float x[MEDUSA_POLICY_INPUTS];
-            delta_row(rec, d, x);
-            for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) s_x[i] = x[i];
-        }
-        __syncthreads();
-        const float cost = mlp_eval_block(policy, s_x, s_a1, s_a2);
-        if (threadIdx.x == 0 && (k == -2 || cost < *s_best)) {
-            *s_best = cost;
-            *s_pick = (uint32_t)d;
-        }
-        __syncthreads();
-    }
-    return *s_pick;
+    // row 0: the default width; rows 1..11: k = -2..8
+    auto width = [&](int r) { return r == 0 ? (double)c.dflt_delta : fmax(1.0, ldexp((double)c.wmean, r - 3)); };
+    if (threadIdx.x < 12) delta_row(rec, width(threadIdx.x), m.x + threadIdx.x * MEDUSA_POLICY_INPUTS);
+    mlp_eval_rows(policy, m.P, m.x, 12, m.a1, m.a2, m.out);
+    int best = 1;
+    for (int r = 2; r < 12; ++r)
+        if (m.out[r] < m.out[best]) best = r;
+    return m.out[best] + kPolicyMargin < m.out[0] ? (uint32_t)width(best) : c.dflt_delta;
 }
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
@@ -387,8 +381,7 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
 // One workgroup after the work kernel (its counters are visible at the kernel boundary).
 __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
                                                      uint32_t far_cap, const float* __restrict__ policy) {
-    __shared__ uint32_t s_pick;
-    __shared__ float s_x[MEDUSA_POLICY_INPUTS], s_a1[kMlpH], s_a2[kMlpH], s_best;
+    __shared__ MlpShared s_mlp;
     if (ctl->done) return;
     const SsspDevCounters k = *ctr;
     SsspCtl c = *ctl;
@@ -406,7 +399,7 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
         } else if (n_far == 0 && !c.overflow) {
             c.done = 1;
         } else {  // empty bucket: restart from the smallest far distance
-            const uint32_t d = sssp_pick_delta(c, n_far, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
+            const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
             c.old_thr = c.thr;
             c.delta = d;
             c.thr = sat_add(k.far_min > c.thr ? k.far_min : c.thr, d);
@@ -432,7 +425,7 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
             if (n_far == 0 && !c.overflow) {
                 c.done = 1;
             } else {
-                const uint32_t d = sssp_pick_delta(c, n_far, policy, s_x, s_a1, s_a2, &s_best, &s_pick);
+                const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
                 c.old_thr = c.thr;
                 c.delta = d;
                 c.thr = sat_add(c.thr, d);

@@ -65,7 +65,8 @@ def _rows(records, action):
 def _train_direction_policy():
     Xs, ys = [], []
     graphs = [wl.rmat_graph(scale=s, edge_factor=16, seed=10 + s) for s in (14, 15, 16, 17)]
-    graphs += [wl.er_graph(2 ** 15, 2 ** 19, seed=21), wl.chung_lu_graph(200_000, 420_000, seed=22)]
+    graphs += [wl.er_graph(2 ** 15, 2 ** 19, seed=21), wl.chung_lu_graph(200_000, 420_000, seed=22),
+               wl.chung_lu_graph(1_200_000, 2_500_000, seed=23)]
     for rp, col in graphs:
         with md.Graph(rp, col) as G:
             for src in (wl.max_outdeg_vertex(rp), 3):
