@@ -194,12 +194,13 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
                 v = col[x];
                 const unsigned long long nd64 = (unsigned long long)s_du[lo] + w[x];
                 if (nd64 < dist[v]) {
+                    // dist[v] falls this superstep (to nd or below), so v needs relaxing at its
+                    // new distance whoever wins: the min needs no return value, and the stamp
+                    // keeps one near entry per vertex
                     const uint32_t nd = (uint32_t)nd64;
-                    const uint32_t old = atomicMin(&dist[v], nd);
-                    if (nd < old) {
-                        if (nd < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
-                        else to_far = true;
-                    }
+                    atomicMin(&dist[v], nd);
+                    if (nd < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
+                    else to_far = true;
                 }
             }
             stage_put(st, k, to_near, to_far, v, ndeg);
