@@ -186,7 +186,10 @@ int medusa_autotune(medusa_graph* g, uint32_t algo, const uint32_t* sources, uin
  * Reference analogue: frontier counters of Algorithm/CPUFunctorHoster.cpp:13-15,21-30
  * and per-operator timers SSSP/UserOperators.h:198-262.  Each record has
  * MEDUSA_TRACE_FIELDS doubles: step, frontier vertices, frontier out-edge sum,
- * unvisited vertices, unvisited out-edge sum, n, m, action, kernel ms. */
+ * unvisited vertices, unvisited out-edge sum, n, m, action, kernel ms (BFS/MBFS);
+ * phase, far entries, mean weight, threshold, 0, n, m, width, phase ms (SSSP);
+ * step, dangling vertices, vertices of in-degree > 64, vertices of in-degree 1..64, 0, n, m,
+ * variant, superstep ms (PageRank). */
 #define MEDUSA_TRACE_FIELDS 9
 int medusa_trace_count(const medusa_graph* g, uint32_t* count);
 int medusa_trace_read(const medusa_graph* g, double* out, uint32_t capacity);
@@ -201,6 +204,7 @@ int medusa_trace_read(const medusa_graph* g, double* out, uint32_t capacity);
 #define MEDUSA_POLICY_PARAMS 1633
 #define MEDUSA_POLICY_DIRECTION 1 /* BFS / MBFS: push or pull per superstep */
 #define MEDUSA_POLICY_DELTA 2     /* SSSP: bucket width at each threshold advance */
+#define MEDUSA_POLICY_VARIANT 3   /* PageRank: pull or push kernel variant per superstep */
 int medusa_policy_create(uint32_t kind, uint64_t seed, medusa_policy** out);
 int medusa_policy_destroy(medusa_policy* p);
 /* Reads (get != NULL) and/or replaces (set != NULL) the MEDUSA_POLICY_PARAMS parameters. */

@@ -38,7 +38,7 @@ EXPORTED = [
 TUNE_BFS, TUNE_MBFS = 1, 2
 POLICY_INPUTS = 16
 POLICY_PARAMS = 1633
-POLICY_DIRECTION, POLICY_DELTA = 1, 2
+POLICY_DIRECTION, POLICY_DELTA, POLICY_VARIANT = 1, 2, 3
 
 
 class MedusaError(RuntimeError):

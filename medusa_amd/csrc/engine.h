@@ -157,7 +157,7 @@ struct PrLayout {
     // captured superstep sequence, reused while (steps, variant, damping) stay the same
     hipGraphExec_t exec = nullptr;
     uint32_t exec_steps = 0;
-    int exec_variant = -1;
+    uint64_t exec_sig = 0;  // signature of the per-superstep kernel variants
     float exec_damping = -1.f;
     ~PrLayout() {
         if (exec) (void)hipGraphExecDestroy(exec);

@@ -268,7 +268,8 @@ using namespace medusa;
 extern "C" int medusa_policy_create(uint32_t kind, uint64_t seed, medusa_policy** out) {
     return guard([&] {
         MDS_CHECK(out, MEDUSA_ERR_ARG, "policy_create: null out");
-        MDS_CHECK(kind == MEDUSA_POLICY_DIRECTION || kind == MEDUSA_POLICY_DELTA, MEDUSA_ERR_ARG,
+        MDS_CHECK(kind == MEDUSA_POLICY_DIRECTION || kind == MEDUSA_POLICY_DELTA || kind == MEDUSA_POLICY_VARIANT,
+                  MEDUSA_ERR_ARG,
                   "policy_create: unknown kind %u", kind);
         auto p = std::make_unique<medusa_policy>();
         p->kind = kind;

@@ -29,6 +29,7 @@
 #include <cmath>
 
 #include "engine.h"
+#include "sched.h"
 
 namespace medusa {
 
@@ -360,8 +361,25 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     const uint32_t nd = L.n_dangling;
     const uint32_t nblk = (uint32_t)L.blocks.size();
     const uint32_t nparts = (uint32_t)L.parts.size();
-    const bool push = o.variant == MEDUSA_PR_PUSH;
-    MDS_CHECK(o.variant == MEDUSA_PR_PULL || push, MEDUSA_ERR_ARG, "pagerank: unknown variant %d", o.variant);
+    MDS_CHECK(o.variant == MEDUSA_PR_PULL || o.variant == MEDUSA_PR_PUSH, MEDUSA_ERR_ARG,
+              "pagerank: unknown variant %d", o.variant);
+    // kernel variant of every superstep: the caller's, or the learned policy's per superstep
+    // (it keeps the caller's unless it predicts a clear gain); the ranks are the same bits
+    uint64_t wave_vertices = 0, lane_vertices = 0;
+    for (const PrRun& r : L.runs) {
+        if (r.kind == kPrWave) wave_vertices += r.count;
+        if (r.kind == kPrLane) lane_vertices += r.count;
+    }
+    Sched sched(o, Sched::kBfs);
+    std::vector<uint8_t> var(steps);
+    uint64_t sig = 1469598103934665603ull;  // FNV-1a over the sequence, keys the captured graph
+    for (uint32_t t = 0; t < steps; ++t) {
+        const double rec[MEDUSA_TRACE_FIELDS] = {(double)t, (double)nd, (double)wave_vertices, (double)lane_vertices,
+                                                 0.0, (double)n, (double)m, 0.0, 0.0};
+        var[t] = (uint8_t)sched.pr_variant(rec, o.variant);
+        sig = (sig ^ var[t]) * 1099511628211ull;
+    }
+    const bool push = std::find(var.begin(), var.end(), (uint8_t)MEDUSA_PR_PUSH) != var.end();
     L.rank.ensure(n);
     L.contrib0.ensure(n);
     L.contrib1.ensure(n);
@@ -400,8 +418,10 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     MDS_LAUNCHED();
     enqueue_base(L.rank.p);
 
-    const bool timing = o.timing != 0;
-    if (timing) ensure_events(g, 2 * (size_t)steps);
+    const bool timing = o.timing != 0 || o.trace != 0;
+    // per superstep: [3t] superstep start, [3t+1] main kernel start, [3t+2] main kernel end
+    if (timing) ensure_events(g, 3 * (size_t)steps);
+    g->trace.clear();
     float* rk[2] = {L.rank.p, L.rank_prev.p};
     float* ct[2] = {L.contrib0.p, L.contrib1.p};
 
@@ -410,13 +430,15 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
             float* r_out = rk[(t + 1) & 1];
             const float* c_in = ct[t & 1];
             float* c_out = ct[(t + 1) & 1];
-            if (push) {
+            const bool step_push = var[t] == MEDUSA_PR_PUSH;
+            if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t], s));
+            if (step_push) {
                 hipLaunchKernelGGL(k_pr_scatter, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, n, L.optr.p,
                                    L.oslot.p, c_in, L.msg.p);
                 MDS_LAUNCHED();
             }
-            if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t], s));
-            if (push)
+            if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 1], s));
+            if (step_push)
                 hipLaunchKernelGGL(k_pr_main<true>, dim3(L.total_wgs), dim3(256), 0, s, (const PrRun*)L.d_runs.p,
                                    (uint32_t)L.runs.size(), (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p,
                                    (const float*)L.msg.p, L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
@@ -425,18 +447,18 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
                                    (uint32_t)L.runs.size(), (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p, c_in,
                                    L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             MDS_LAUNCHED();
-            if (with_events) MDS_HIP(hipEventRecord(g->events[2 * t + 1], s));
+            if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 2], s));
             if (t + 1 < steps) enqueue_base(r_out);
         }
     };
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     if (o.use_graph && !timing && steps > 0) {
-        // the captured sequence only depends on (steps, variant, damping): buffers are fixed per layout
-        if (!L.exec || L.exec_steps != steps || L.exec_variant != o.variant || L.exec_damping != damping) {
+        // the captured sequence only depends on (steps, variants, damping): buffers are fixed per layout
+        if (!L.exec || L.exec_steps != steps || L.exec_sig != sig || L.exec_damping != damping) {
             if (L.exec) (void)hipGraphExecDestroy(L.exec);
             L.exec = nullptr;
-            L.exec_variant = -1;
+            L.exec_sig = 0;
             // capture must not include the event record issued above
             MDS_HIP(hipStreamSynchronize(s));
             hipGraph_t graph = nullptr;
@@ -454,7 +476,7 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
             (void)hipGraphDestroy(graph);
             MDS_HIP(e);
             L.exec_steps = steps;
-            L.exec_variant = o.variant;
+            L.exec_sig = sig;
             L.exec_damping = damping;
         }
         MDS_HIP(hipGraphLaunch(L.exec, s));
@@ -478,24 +500,37 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     float ms = 0;
     MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
     st.device_ms = ms;
-    if (timing) st.kernel_ms = sum_event_pairs(g, steps);
-    st.kernel_launches = steps;
     // byte model of the main kernel per launch: in-edge sources (pull only; ELL slab or CSC),
     // gathered contribution or message slot per in-edge, offsets of the vertices that read
     // them (wave runs; every vertex in the push variant), out-degree read, rank write,
-    // contrib write (non-dangling vertices)
-    uint64_t wave_vertices = 0;
-    for (const PrRun& r : L.runs)
-        if (r.kind == kPrWave) wave_vertices += r.count;
-    const uint64_t main_bytes = (push ? 4ull * m + 8ull * n : 8ull * m + 8ull * wave_vertices) + 8ull * n +
-                                4ull * (n - nd);
-    const uint64_t scatter_bytes = push ? (4ull * (n + 1) + 8ull * m + 4ull * n) : 0;
-    st.kernel_bytes = main_bytes * steps;
-    st.alg_bytes = (main_bytes + scatter_bytes + 4ull * nd) * steps;
+    // contrib write (non-dangling vertices); the push variant adds its scatter
+    const uint64_t pull_bytes = 8ull * m + 8ull * wave_vertices + 8ull * n + 4ull * (n - nd);
+    const uint64_t push_bytes = 4ull * m + 8ull * n + 8ull * n + 4ull * (n - nd);
+    const uint64_t scatter_bytes = 4ull * (n + 1) + 8ull * m + 4ull * n;
+    for (uint32_t t = 0; t < steps; ++t) {
+        const bool p = var[t] == MEDUSA_PR_PUSH;
+        st.kernel_bytes += p ? push_bytes : pull_bytes;
+        st.alg_bytes += (p ? push_bytes + scatter_bytes : pull_bytes) + 4ull * nd;
+        if (p) ++st.push_steps; else ++st.pull_steps;
+        if (timing) {
+            float k_ms = 0, s_ms = 0;
+            MDS_HIP(hipEventElapsedTime(&k_ms, g->events[3 * t + 1], g->events[3 * t + 2]));
+            MDS_HIP(hipEventElapsedTime(&s_ms, g->events[3 * t], g->events[3 * t + 2]));
+            st.kernel_ms += k_ms;
+            if (o.trace) {
+                TraceRec r;
+                const double rec[MEDUSA_TRACE_FIELDS] = {(double)t, (double)nd, (double)wave_vertices,
+                                                         (double)lane_vertices, 0.0, (double)n, (double)m,
+                                                         (double)var[t], (double)s_ms};
+                std::copy(rec, rec + MEDUSA_TRACE_FIELDS, r.f);
+                g->trace.push_back(r);
+            }
+        }
+    }
+    st.kernel_launches = steps;
     st.edges_traversed = (uint64_t)steps * m;
     st.supersteps = steps;
     st.reached = n;
-    if (push) st.push_steps = steps; else st.pull_steps = steps;
     if (steps > 0) MDS_HIP(hipMemcpy(&st.residual_l1, L.resid.p, sizeof(double), hipMemcpyDeviceToHost));
     copy_out(g, out, L.unperm.p, (size_t)n * 4);
 }

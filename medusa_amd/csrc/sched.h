@@ -55,6 +55,27 @@ __host__ __device__ inline void delta_row(const double* r, double action, float*
     x[8] = x[1] * la / 8.f;
 }
 
+// Kernel-variant rows (PageRank record = step, dangling, wave-run vertices, lane-run vertices,
+// 0, n, m, ...): size, mean degree, the shares of dangling and high in-degree vertices, step,
+// and the candidate variant (0 pull, 1 push) with its interactions.
+__host__ __device__ inline void variant_row(const double* r, double action, float* x) {
+    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
+    const double n = r[5] > 1.0 ? r[5] : 1.0, m = r[6] > 1.0 ? r[6] : 1.0;
+    const float a = action != 0.0 ? 1.f : 0.f;
+    x[0] = lg1(n) / 32.f;
+    x[1] = lg1(m / n) / 8.f;
+    x[2] = (float)(r[1] / n);
+    x[3] = (float)(r[2] / n);
+    x[4] = (float)(r[3] / n);
+    x[5] = (float)(r[0] < 64.0 ? r[0] : 64.0) / 64.f;
+    x[6] = a;
+    x[7] = a * x[0];
+    x[8] = a * x[1];
+    x[9] = a * x[2];
+    x[10] = a * x[3];
+    x[11] = 1.f - a;
+}
+
 #if defined(__HIPCC__)
 // LDS working set of a policy evaluation inside a decide kernel (up to 12 candidate rows).
 struct MlpShared {
@@ -133,6 +154,19 @@ struct Sched {
             }
         }
         return best + kPolicyMargin < c_dflt ? pick : dflt;
+    }
+    // PageRank kernel variant for one superstep (record as in variant_row): the policy's
+    // cheaper variant if it predicts a clear gain over `dflt`, else `dflt`.
+    int pr_variant(const double* rec, int dflt) const {
+        if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_VARIANT) return dflt;
+        float x[MEDUSA_POLICY_INPUTS];
+        policy_row(MEDUSA_POLICY_VARIANT, rec, 0.0, x);
+        const float c_pull = mlp_eval_host(policy_params(o.policy), x);
+        policy_row(MEDUSA_POLICY_VARIANT, rec, 1.0, x);
+        const float c_push = mlp_eval_host(policy_params(o.policy), x);
+        const float c_dflt = dflt == MEDUSA_PR_PUSH ? c_push : c_pull;
+        const float c_other = dflt == MEDUSA_PR_PUSH ? c_pull : c_push;
+        return c_other + kPolicyMargin < c_dflt ? (dflt == MEDUSA_PR_PUSH ? MEDUSA_PR_PULL : MEDUSA_PR_PUSH) : dflt;
     }
     static void to_record(const Frontier& f, double action, double kms, double* r) {
         r[0] = f.step;

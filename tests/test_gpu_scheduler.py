@@ -170,3 +170,54 @@ def test_device_and_host_drivers_agree(oracle):
             b, sb = G.sssp(5, delta=delta, timing=1)
             assert np.array_equal(a, want) and np.array_equal(b, want)
             assert sa.edges_traversed == sb.edges_traversed and sa.reached == sb.reached
+
+
+def _variant_samples(G, steps):
+    rows, ys, cost = [], [], {}
+    for v in (md.PR_PULL, md.PR_PUSH):
+        G.pagerank(0.85, steps, variant=v, trace=1)
+        tr = G.trace()
+        assert len(tr) == steps and np.all(tr[:, 7] == v)
+        cost[v] = tr[:, 8] * 1e3
+        for r in tr:
+            rows.append(md.policy_features(md.POLICY_VARIANT, r, r[7]))
+            ys.append(np.log2(1.0 + r[8] * 1e3))
+    return rows, ys, cost
+
+
+def test_learned_pagerank_variant(oracle):
+    """Kernel variant per superstep (pull or push): the ranks are the same bits whatever the
+    policy picks; a policy trained on the real traces keeps the cheaper variant, and one
+    trained on swapped costs really switches to push."""
+    Xs, ys = [], []
+    for sc in (12, 13, 14):
+        rp, col = wl.rmat_graph(scale=sc, edge_factor=16, seed=60 + sc)
+        with md.Graph(rp, col) as G:
+            x, y, _ = _variant_samples(G, 8)
+            Xs += x
+            ys += y
+    X = np.array(Xs, np.float32)
+    y = np.array(ys, np.float32)
+    pol = md.Policy(md.POLICY_VARIANT, seed=4)
+    pol.train(X, y, iters=2000, lr=3e-3)
+    swapped = md.Policy(md.POLICY_VARIANT, seed=4)
+    is_push = X[:, 6] > 0.5
+    swapped.train(X, np.where(is_push, y - 4.0, y + 4.0).astype(np.float32), iters=2000, lr=3e-3)
+    rp, col = wl.rmat_graph(scale=16, edge_factor=16, seed=77)
+    want32 = oracle.pagerank_f32(rp, col, 0.85, 12)
+    with md.Graph(rp, col) as G:
+        _, _, cost = _variant_samples(G, 12)
+        for p in (pol, swapped):
+            for use_graph in (0, 1):
+                r, st = G.pagerank(0.85, 12, policy=p, use_graph=use_graph)
+                assert np.array_equal(r.view(np.uint32), want32.view(np.uint32))
+        G.pagerank(0.85, 12, policy=pol, trace=1)
+        acts = G.trace()[:, 7].astype(int)
+        G.pagerank(0.85, 12, policy=swapped, trace=1)
+        acts_sw = G.trace()[:, 7].astype(int)
+        assert np.all(acts_sw == md.PR_PUSH)
+        c_plain = cost[md.PR_PULL].sum()
+        c_learn = sum(cost[a][i] for i, a in enumerate(acts))
+        print(f"[pagerank variant] modelled: pull-only {c_plain:.1f} us, learned {c_learn:.1f} us "
+              f"(push steps {int((acts == 1).sum())}/12)", flush=True)
+        assert c_learn <= 1.10 * c_plain
