@@ -127,3 +127,28 @@ def test_combiner_ops_app(tmp_path, kind):
             np.testing.assert_allclose(got, want, rtol=1e-6)
         else:
             assert np.array_equal(got, want), op
+
+
+def test_edgelist_and_message_functors_app(tmp_path):
+    """FunctorHolder<EDGELIST> walks each vertex's out-edges (CSR order) and sends messages;
+    FunctorHolder<MESSAGE> rewrites every message slot; the ADD combiner and a VERTEX functor
+    collect the result."""
+    rng = np.random.default_rng(9)
+    n, m = 4000, 20000
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    from conftest import csr_from_edges
+    w = rng.integers(0, 50, m).astype(np.uint32)
+    rp, col, wc = csr_from_edges(n, np.stack([src, dst], 1), w)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col, wc)
+    out = tmp_path / "f.bin"
+    run("functors_app", g, out)
+    got = np.fromfile(out, np.int32)
+    wsum, result = got[:n], got[n:]
+    rows = np.repeat(np.arange(n), np.diff(rp.astype(np.int64)))
+    want_wsum = np.bincount(rows, weights=wc, minlength=n).astype(np.int64)
+    want_res = np.bincount(col, weights=2 * wc.astype(np.int64) + 1, minlength=n).astype(np.int64)
+    want_res[np.bincount(col, minlength=n) == 0] = -1
+    assert np.array_equal(wsum, want_wsum)
+    assert np.array_equal(result, want_res)
