@@ -261,3 +261,28 @@ def test_sssp_hub_source_slices(oracle):
             for timing in (0, 1):
                 d, _ = G.sssp(0, delta=delta, timing=timing)
                 assert np.array_equal(d, want), (delta, timing)
+
+
+@pytest.mark.parametrize("case", ["n1_empty", "n1_selfloop", "n2_empty", "n3_chain"])
+def test_tiny_graphs_every_engine(oracle, case):
+    """The smallest inputs through every engine and driver: one vertex with and without a
+    self loop, two isolated vertices, a 3-chain."""
+    edges, n = {"n1_empty": ([], 1), "n1_selfloop": ([(0, 0)], 1), "n2_empty": ([], 2),
+                "n3_chain": ([(0, 1), (1, 2)], 3)}[case]
+    rp, col, w = csr_from_edges(n, np.array(edges, np.int64).reshape(-1, 2), [7] * len(edges))
+    w = np.asarray(w if w is not None else [], np.uint32)
+    with md.Graph(rp, col, w) as G:
+        for src in range(n):
+            want = oracle.bfs_queue(rp, col, src)
+            for d in _dirs():
+                for timing in (0, 1):
+                    assert np.array_equal(G.bfs(src, direction=d, timing=timing)[0], want)
+            dwant = oracle.sssp_dijkstra(rp, col, w, src)
+            for timing in (0, 1):
+                assert np.array_equal(G.sssp(src, timing=timing)[0], dwant)
+            srcs = np.array([src, src], np.uint32)
+            mwant = oracle.mbfs(rp, col, srcs)
+            for timing in (0, 1):
+                assert np.array_equal(G.mbfs(srcs, timing=timing)[0], mwant)
+        r, _ = G.pagerank(0.85, 4)
+        assert np.array_equal(r.view(np.uint32), oracle.pagerank_f32(rp, col, 0.85, 4).view(np.uint32))
