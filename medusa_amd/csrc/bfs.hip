@@ -384,8 +384,10 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
     const unsigned long long probes = ctr->probes;
     const unsigned long long deg_large = ctr->deg_large;
     BfsCtl c = *ctl;
-    // byte model of the superstep just finished (same as the host-driven driver)
-    const unsigned long long kb = c.pull ? 4ull * n + 4ull * (n + 1) + 8ull * probes + 4ull * n_new
+    // byte model of the superstep just finished (same as the host-driven driver): push =
+    // 12 per frontier vertex + 8 per out-edge + 8 per claim; pull = 2 bitmap passes (n/4) +
+    // 8 per unvisited vertex + 8 per probed in-edge + 4 per claim
+    const unsigned long long kb = c.pull ? n / 4 + 8ull * c.n_u + 8ull * probes + 4ull * n_new
                                          : 12ull * c.n_f + 8ull * c.m_f + 8ull * n_new;
     c.kernel_bytes += kb;
     c.alg_bytes += kb + 4ull * n + nwords * 8ull + 12ull * n_new;
@@ -656,13 +658,14 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
             ++st.kernel_launches;
         }
         // byte model: push = queue + offsets + col + level probe + claims;
-        // pull = level scan + offsets + probed in-edges (source + bitmap word) + claims
+        // pull = 2 bitmap passes + level and offsets of unvisited vertices + probed in-edges
+        // (source + bitmap word) + claims
         uint64_t kb;
         if (!pull) {
             kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * c.n_new;
             ++st.push_steps;
         } else {
-            kb = 4ull * n + 4ull * (n + 1) + 8ull * c.probes + 4ull * c.n_new;
+            kb = n / 4 + 8ull * f.n_u + 8ull * c.probes + 4ull * c.n_new;
             ++st.pull_steps;
         }
         st.kernel_bytes += kb;

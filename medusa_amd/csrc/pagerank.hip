@@ -500,11 +500,11 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     float ms = 0;
     MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
     st.device_ms = ms;
-    // byte model of the main kernel per launch: in-edge sources (pull only; ELL slab or CSC),
-    // gathered contribution or message slot per in-edge, offsets of the vertices that read
-    // them (wave runs; every vertex in the push variant), out-degree read, rank write,
-    // contrib write (non-dangling vertices); the push variant adds its scatter
-    const uint64_t pull_bytes = 8ull * m + 8ull * wave_vertices + 8ull * n + 4ull * (n - nd);
+    // byte model per superstep (pull over CSC): 4(n+1) offsets + 8m (source id and gathered
+    // contribution per in-edge) + 16n (rank write, contribution write, out-degree read,
+    // dangling read); the push variant reads message slots instead of gathering
+    // contributions and adds its scatter
+    const uint64_t pull_bytes = 4ull * (n + 1) + 8ull * m + 16ull * n;
     const uint64_t push_bytes = 4ull * m + 8ull * n + 8ull * n + 4ull * (n - nd);
     const uint64_t scatter_bytes = 4ull * (n + 1) + 8ull * m + 4ull * n;
     for (uint32_t t = 0; t < steps; ++t) {

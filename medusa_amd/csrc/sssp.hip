@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
             c.supersteps += 1;
             c.near_sel ^= 1u;
             c.relaxed += k.relax;
-            c.kernel_bytes += 16ull * N + 12ull * k.relax + 12ull * k.n_near;
+            c.kernel_bytes += 16ull * N + 16ull * k.relax + 8ull * k.n_near;
         }
         if (n_far > far_cap) {
             c.overflow = 1;
@@ -677,9 +677,9 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
             ++st.kernel_launches;
         }
         relaxed += c.relax;
-        // byte model: queue + offsets + dist of frontier, col + w + dist probe per edge,
-        // atomic (8 B) + enqueue per improvement (upper bound: every enqueued entry)
-        st.kernel_bytes += 16ull * n_near + 12ull * c.relax + 12ull * (c.n_near);
+        // byte model: 16 per frontier entry (entry, offsets, dist), 16 per relaxed edge (col,
+        // w, dist probe, atomic), 8 per enqueued entry
+        st.kernel_bytes += 16ull * n_near + 16ull * c.relax + 8ull * (c.n_near);
         ++st.supersteps;
         ++st.push_steps;
         std::swap(near_cur, near_nxt);
