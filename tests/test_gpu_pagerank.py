@@ -117,3 +117,19 @@ def test_pagerank_rmat20_config(oracle):
         # FP64 residual between the last two iterates is reported and small
         _, st = G.pagerank(0.85, 30)
         assert 0 < st.residual_l1 < 1e-2
+
+
+def test_chung_lu_config(oracle):
+    """The Chung-Lu configuration at full size: 2.4 M vertices, 5 M arcs, Zipf(2.0) out-weights
+    capped at 100,000, 40 % weight 0 (seed 5): direction-optimising BFS bit-exact, 20-step
+    PageRank bit-exact against the FP32 restatement and within L1 1e-6 of FP64."""
+    rp, col = wl.chung_lu_graph()
+    src = wl.max_outdeg_vertex(rp)
+    with md.Graph(rp, col) as G:
+        lv, st = G.bfs(src)
+        assert np.array_equal(lv, oracle.bfs_queue(rp, col, src))
+        assert st.push_steps > 0 and st.pull_steps > 0
+        r, _ = G.pagerank(0.85, 20, use_graph=1)
+        assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 20)))
+        l1 = np.abs(r.astype(np.float64) - oracle.pagerank_f64(rp, col, 0.85, 20)).sum()
+        assert l1 <= L1_TOL, l1
