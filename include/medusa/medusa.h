@@ -22,6 +22,9 @@
 //   MultiUtilities.h:48-69), MGLOBAL::com.combineAllDevice() (MedusaRT/Combiner.cu:82-142)
 //   GraphIR, GraphPartition, GTGraph::get_parition (MedusaRT/GraphConverter.h:37-89,
 //       MultipleGPU/PartitionManager.cu:116-534, one partition).
+//   QFunctorHolder<EQ2EQ|EQ2VQ|EQConsume|VQ2EQ|VQ2VQ|VQConsume|VA2EQ|VA2AQ|EA2EQ|EA2AQ>::Run
+//       over MedusaQueue (the reference's planned queue API, MultipleGPU/MultiAPIHoster.h:18-23,
+//       MedusaRT/APIKernelWrappers.h:53-140).
 //
 // Storage comes from libmedusa_amd.so (medusa_graph_create: CSR + CSC + slot map in HBM);
 // user fields are device arrays allocated here.  Every call runs on the current GPU; the
@@ -30,6 +33,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -41,6 +45,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../medusa_c.h"
@@ -487,6 +492,263 @@ struct FunctorHolder<MESSAGE> {
         hipLaunchKernelGGL(medusa_api::messageProcWrapper<OP>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
                            MGLOBAL::super_step, op, MGLOBAL::gpu_def[0].d_messageArray.size);
         medusa_api::check_launch("messageProcWrapper");
+    }
+};
+
+// ---- queue API (MultipleGPU/MultiAPIHoster.h:18-23 names the operations; the reference
+// gives a body only to EdgeQueue2EdgeQueue, MedusaRT/APIKernelWrappers.h:53-140, compiled
+// for MBFS only) ----------------------------------------------------------------------------
+// X2Y runs the functor once per item of X and writes Y:
+//   X: VQ = a vertex queue, EQ = an edge queue, VA = every vertex, EA = every edge;
+//   Y: VQ / AQ = a vertex queue, EQ = an edge queue, Consume = nothing.
+// The functor returns whether the item activates its target vertex (a vertex item: itself;
+// an edge item: its destination, as SetActive(dstVertexID) in EdgeQueue2EdgeQueue).  An
+// activated vertex goes to a vertex queue as itself and to an edge queue as its out-edges
+// (CSR edge indices, as EdgeQueue2EdgeQueue enqueues the activated vertex's adjacency).
+// Functors: bool operator()(D_Vertex v, int super_step) or (D_Edge e, int super_step); a
+// Consume functor may return void.  A vertex activated twice is queued twice: functors that
+// must not do so claim the vertex with an atomic (as the reference's ATOMIC variant does).
+// The order of a queue's items is unspecified.
+enum QAPIType { EQ2EQ, EQ2VQ, EQConsume, VQ2EQ, VQ2VQ, VQConsume, VA2EQ, VA2AQ, EA2EQ, EA2AQ };
+
+struct MedusaQueue {
+    int* d_items = nullptr;
+    unsigned long long* d_count = nullptr;  // may exceed capacity: then the queue overflowed
+    long long capacity = 0;
+    bool of_edges = false;
+};
+
+namespace medusa_api {
+template <class O, class It>
+__device__ inline bool activates(O& op, It item, int step) {
+    if constexpr (std::is_same<decltype(op(item, step)), void>::value) {
+        op(item, step);
+        return false;
+    } else {
+        return static_cast<bool>(op(item, step));
+    }
+}
+
+// Item i of the input (a queue entry or the index itself) through the functor; returns the
+// activated target vertex or -1.
+template <bool kEdges, bool kFromQueue, class O>
+__device__ inline int queue_visit(O& op, const int* __restrict__ in, unsigned long long i, int step) {
+    const int item = kFromQueue ? in[i] : (int)i;
+    if constexpr (kEdges) {
+        return activates(op, D_Edge(item), step) ? GRAPH_STORAGE_GPU::d_edgeArray.d_dstVertexID[item] : -1;
+    } else {
+        return activates(op, D_Vertex(item), step) ? item : -1;
+    }
+}
+
+__device__ inline unsigned long long queue_len(const unsigned long long* cnt, long long cap) {
+    const unsigned long long c = *cnt;
+    return c < (unsigned long long)cap ? c : (unsigned long long)cap;
+}
+
+// Exclusive prefix sum over a 256-thread block (4 waves); every thread gets the total.
+__device__ inline unsigned long long block_scan_u64(unsigned long long x, unsigned long long* s_wave,
+                                                    unsigned long long& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long incl = x;
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wv] = incl;
+    __syncthreads();
+    unsigned long long before = 0;
+    for (int w = 0; w < wv; ++w) before += s_wave[w];
+    total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();  // s_wave is reused by the next call
+    return before + incl - x;
+}
+
+// X2VQ / X2AQ / XConsume: one slot per activated item, reserved with one atomic per block.
+template <bool kEdges, bool kFromQueue, class O>
+__global__ __launch_bounds__(256) void queueFilterWrapper(int step, O op, const int* __restrict__ in,
+                                                          const unsigned long long* __restrict__ in_count,
+                                                          long long in_n, int* __restrict__ out,
+                                                          unsigned long long* __restrict__ out_count,
+                                                          long long out_cap) {
+    __shared__ unsigned long long s_wave[4];
+    __shared__ unsigned long long s_base;
+    const unsigned long long cnt = kFromQueue ? queue_len(in_count, in_n) : (unsigned long long)in_n;
+    for (unsigned long long b = (unsigned long long)blockIdx.x * 256; b < cnt; b += (unsigned long long)gridDim.x * 256) {
+        const unsigned long long i = b + threadIdx.x;
+        const int t = i < cnt ? queue_visit<kEdges, kFromQueue>(op, in, i, step) : -1;
+        if (!out) continue;  // Consume
+        unsigned long long total;
+        const unsigned long long off = block_scan_u64(t >= 0 ? 1ull : 0ull, s_wave, total);
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(out_count, total) : 0ull;
+        __syncthreads();
+        if (t >= 0 && s_base + off < (unsigned long long)out_cap) out[s_base + off] = t;
+        __syncthreads();  // s_base is rewritten in the next round
+    }
+}
+
+// X2EQ: the out-edges of every activated vertex.  The block reserves its whole output with
+// one atomic and writes it cooperatively: output j belongs to the thread whose exclusive
+// offset is the last one <= j.
+template <bool kEdges, bool kFromQueue, class O>
+__global__ __launch_bounds__(256) void queueExpandWrapper(int step, O op, const int* __restrict__ in,
+                                                          const unsigned long long* __restrict__ in_count,
+                                                          long long in_n, int* __restrict__ out,
+                                                          unsigned long long* __restrict__ out_count,
+                                                          long long out_cap) {
+    __shared__ unsigned long long s_wave[4];
+    __shared__ unsigned long long s_off[256];
+    __shared__ int s_first[256];
+    __shared__ unsigned long long s_base;
+    const int* __restrict__ row = GRAPH_STORAGE_GPU::d_vertexArray.d_edge_index;
+    const unsigned long long cnt = kFromQueue ? queue_len(in_count, in_n) : (unsigned long long)in_n;
+    for (unsigned long long b = (unsigned long long)blockIdx.x * 256; b < cnt; b += (unsigned long long)gridDim.x * 256) {
+        const unsigned long long i = b + threadIdx.x;
+        const int t = i < cnt ? queue_visit<kEdges, kFromQueue>(op, in, i, step) : -1;
+        const int first = t >= 0 ? row[t] : 0;
+        const unsigned long long deg = t >= 0 ? (unsigned long long)(row[t + 1] - first) : 0ull;
+        unsigned long long total;
+        const unsigned long long off = block_scan_u64(deg, s_wave, total);
+        s_off[threadIdx.x] = off;
+        s_first[threadIdx.x] = first;
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(out_count, total) : 0ull;
+        __syncthreads();
+        for (unsigned long long j = threadIdx.x; j < total; j += 256) {
+            int lo = 0, hi = 256;  // last k with s_off[k] <= j; that thread has deg > 0
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            const unsigned long long pos = s_base + j;
+            if (pos < (unsigned long long)out_cap) out[pos] = s_first[lo] + (int)(j - s_off[lo]);
+        }
+        __syncthreads();  // LDS is rewritten in the next round
+    }
+}
+
+inline void queue_set_count(unsigned long long* c, unsigned long long v) {
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, 0, c, v, 1);
+    check_launch("queue count");
+}
+
+inline MedusaQueue make_queue(bool edges, long long capacity) {
+    if (!MGLOBAL::gpu_def) throw std::runtime_error("queues need InitConfig and InitHostDS first");
+    const GPUDef& G = MGLOBAL::gpu_def[0];
+    MedusaQueue q;
+    q.of_edges = edges;
+    q.capacity = capacity >= 0 ? capacity
+                               : edges ? G.edgeArray.size : std::max<long long>(G.vertexArray.size, G.edgeArray.size);
+    q.d_items = dev_alloc_copy<int>(nullptr, q.capacity > 0 ? (size_t)q.capacity : 1);
+    MEDUSA_HIP_OK(hipMalloc((void**)&q.d_count, 128));
+    queue_set_count(q.d_count, 0ull);
+    return q;
+}
+
+template <bool kExpand, bool kEdges, bool kFromQueue, class O>
+inline void run_queue_op(O op, const MedusaQueue* in, MedusaQueue* out, const char* what) {
+    if (!MGLOBAL::gpu_def) throw std::runtime_error(std::string(what) + ": InitConfig/InitDeviceDS first");
+    if (kFromQueue && (!in || !in->d_items || in->of_edges != kEdges))
+        throw std::runtime_error(std::string(what) + ": input must be a " + (kEdges ? "edge" : "vertex") + " queue");
+    if (out && (!out->d_items || out->of_edges != kExpand))
+        throw std::runtime_error(std::string(what) + ": output must be " + (kExpand ? "an edge" : "a vertex") + " queue");
+    if (in && out && in->d_items == out->d_items)
+        throw std::runtime_error(std::string(what) + ": input and output must be different queues");
+    const GPUDef& G = MGLOBAL::gpu_def[0];
+    const long long in_n = kFromQueue ? in->capacity : (kEdges ? G.edgeArray.size : G.vertexArray.size);
+    int* out_items = out ? out->d_items : nullptr;
+    unsigned long long* out_count = out ? out->d_count : nullptr;
+    const long long out_cap = out ? out->capacity : 0;
+    if (out) queue_set_count(out->d_count, 0ull);
+    const int* in_items = kFromQueue ? in->d_items : nullptr;
+    const unsigned long long* in_count = kFromQueue ? in->d_count : nullptr;
+    if constexpr (kExpand)
+        hipLaunchKernelGGL((queueExpandWrapper<kEdges, kFromQueue, O>), dim3(grid_size()), dim3(256), 0, 0,
+                           MGLOBAL::super_step, op, in_items, in_count, in_n, out_items, out_count, out_cap);
+    else
+        hipLaunchKernelGGL((queueFilterWrapper<kEdges, kFromQueue, O>), dim3(grid_size()), dim3(256), 0, 0,
+                           MGLOBAL::super_step, op, in_items, in_count, in_n, out_items, out_count, out_cap);
+    check_launch(what);
+}
+}  // namespace medusa_api
+
+// Queue capacity in items: by default m for an edge queue and max(n, m) for a vertex queue
+// (one item per edge fits).  An edge queue written by X2EQ holds the activated vertices'
+// out-edges, which exceed m when vertices are activated more than once: give such a queue
+// a larger capacity.  InitHostDS first.
+inline MedusaQueue CreateVertexQueue(long long capacity = -1) { return medusa_api::make_queue(false, capacity); }
+inline MedusaQueue CreateEdgeQueue(long long capacity = -1) { return medusa_api::make_queue(true, capacity); }
+inline void FreeQueue(MedusaQueue& q) {
+    MEDUSA_HIP_OK(hipDeviceSynchronize());
+    if (q.d_items) (void)hipFree(q.d_items);
+    if (q.d_count) (void)hipFree(q.d_count);
+    q = MedusaQueue{};
+}
+// Number of items; throws if the last operation that wrote the queue overflowed it.
+inline long long QueueSize(const MedusaQueue& q) {
+    unsigned long long c = 0;
+    MEDUSA_HIP_OK(hipMemcpy(&c, q.d_count, sizeof(c), hipMemcpyDeviceToHost));
+    if (c > (unsigned long long)q.capacity)
+        throw std::runtime_error("queue overflow: " + std::to_string(c) + " items for capacity " +
+                                 std::to_string(q.capacity));
+    return (long long)c;
+}
+// Replaces the queue's contents with ids[0, count) (vertex ids or CSR edge indices).
+inline void QueueAssign(MedusaQueue& q, const int* ids, long long count) {
+    if (count < 0 || count > q.capacity) throw std::runtime_error("QueueAssign: count exceeds the queue's capacity");
+    const int limit = q.of_edges ? MGLOBAL::gpu_def[0].edgeArray.size : MGLOBAL::gpu_def[0].vertexArray.size;
+    for (long long i = 0; i < count; ++i)
+        if (ids[i] < 0 || ids[i] >= limit) throw std::runtime_error("QueueAssign: id out of range");
+    if (count) MEDUSA_HIP_OK(hipMemcpy(q.d_items, ids, (size_t)count * sizeof(int), hipMemcpyHostToDevice));
+    medusa_api::queue_set_count(q.d_count, (unsigned long long)count);
+}
+inline std::vector<int> QueueItems(const MedusaQueue& q) {
+    std::vector<int> v((size_t)QueueSize(q));
+    if (!v.empty()) MEDUSA_HIP_OK(hipMemcpy(v.data(), q.d_items, v.size() * sizeof(int), hipMemcpyDeviceToHost));
+    return v;
+}
+
+template <QAPIType q>
+struct QFunctorHolder {};
+// Run(op, in, out) for queue inputs, Run(op, out) for the whole vertex or edge array.
+#define MDS_QUEUE_OP(Q, EXPAND, EDGES)                                                                    \
+    template <>                                                                                           \
+    struct QFunctorHolder<Q> {                                                                            \
+        template <class OP>                                                                               \
+        static void Run(OP op, const MedusaQueue& in, MedusaQueue& out) {                                 \
+            medusa_api::run_queue_op<EXPAND, EDGES, true>(op, &in, &out, #Q);                              \
+        }                                                                                                 \
+    };
+#define MDS_ARRAY_OP(Q, EXPAND, EDGES)                                                                    \
+    template <>                                                                                           \
+    struct QFunctorHolder<Q> {                                                                            \
+        template <class OP>                                                                               \
+        static void Run(OP op, MedusaQueue& out) {                                                        \
+            medusa_api::run_queue_op<EXPAND, EDGES, false>(op, nullptr, &out, #Q);                         \
+        }                                                                                                 \
+    };
+MDS_QUEUE_OP(EQ2EQ, true, true)
+MDS_QUEUE_OP(EQ2VQ, false, true)
+MDS_QUEUE_OP(VQ2EQ, true, false)
+MDS_QUEUE_OP(VQ2VQ, false, false)
+MDS_ARRAY_OP(VA2EQ, true, false)
+MDS_ARRAY_OP(VA2AQ, false, false)
+MDS_ARRAY_OP(EA2EQ, true, true)
+MDS_ARRAY_OP(EA2AQ, false, true)
+#undef MDS_QUEUE_OP
+#undef MDS_ARRAY_OP
+template <>
+struct QFunctorHolder<EQConsume> {
+    template <class OP>
+    static void Run(OP op, const MedusaQueue& in) {
+        medusa_api::run_queue_op<false, true, true>(op, &in, nullptr, "EQConsume");
+    }
+};
+template <>
+struct QFunctorHolder<VQConsume> {
+    template <class OP>
+    static void Run(OP op, const MedusaQueue& in) {
+        medusa_api::run_queue_op<false, false, true>(op, &in, nullptr, "VQConsume");
     }
 };
 

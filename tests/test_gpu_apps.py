@@ -152,3 +152,64 @@ def test_edgelist_and_message_functors_app(tmp_path):
     want_res[np.bincount(col, minlength=n) == 0] = -1
     assert np.array_equal(wsum, want_wsum)
     assert np.array_equal(result, want_res)
+
+
+def _sections(path, count):
+    a = np.fromfile(path, np.int32)
+    out, i = [], 0
+    for _ in range(count):
+        k = int(a[i])
+        out.append(a[i + 1:i + 1 + k])
+        i += 1 + k
+    assert i == len(a)
+    return out
+
+
+@pytest.mark.parametrize("seed,hub", [(11, 0), (12, 20000)])
+def test_queue_operations_app(tmp_path, oracle, seed, hub):
+    """QFunctorHolder<...>::Run for every QAPIType: a BFS built from VQ2EQ + EQ2EQ (a functor
+    claiming the destination's level) must give the oracle's levels, and each other operation
+    must produce exactly the multiset of items its definition gives (order is unspecified).
+    `hub` adds a vertex with that many out-edges, so one activated vertex fills many blocks'
+    worth of queue entries."""
+    from conftest import csr_from_edges
+    rng = np.random.default_rng(seed)
+    n, m = 5000, 40000
+    edges = np.stack([rng.integers(0, n, m), rng.integers(0, n, m)], 1)
+    if hub:
+        edges = np.concatenate([edges, np.stack([np.full(hub, 17), rng.integers(0, n, hub)], 1)])
+    w = rng.integers(0, 50, len(edges)).astype(np.uint32)
+    rp, col, wc = csr_from_edges(n, edges, w)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col, wc)
+    out = tmp_path / "q.bin"
+    src, light = 17, 3
+    run("queues_app", g, out, src, light)
+    (levels, reached, even, mark, va2eq, ea2aq, ea2eq, eq2vq, hits, steps) = _sections(out, 10)
+
+    want = oracle.bfs_queue(rp, col, src)
+    assert np.array_equal(levels, want)
+    deg = np.diff(rp.astype(np.int64))
+    esrc = np.repeat(np.arange(n), deg)
+    with_edges = want[(want >= 0) & (deg > 0)]
+    assert int(steps[0]) == (int(with_edges.max()) + 1 if len(with_edges) else 0)
+
+    def edges_of(vs):
+        vs = np.asarray(vs, np.int64)
+        if len(vs) == 0:
+            return np.zeros(0, np.int64)
+        return np.concatenate([np.arange(rp[v], rp[v + 1], dtype=np.int64) for v in vs])
+
+    srt = np.sort
+    assert np.array_equal(srt(reached), np.flatnonzero(want >= 0))
+    ev = np.flatnonzero((want >= 0) & (want % 2 == 0))
+    assert np.array_equal(srt(even), ev)
+    assert np.array_equal(mark, np.isin(np.arange(n), ev).astype(np.int32))
+    assert np.array_equal(srt(va2eq), srt(edges_of(np.flatnonzero(want == 1))))
+    lt = np.flatnonzero(wc < light)
+    assert np.array_equal(srt(ea2aq), srt(col[lt].astype(np.int64)))
+    want_ea2eq = edges_of(col[lt])
+    assert np.array_equal(srt(ea2eq), srt(want_ea2eq))
+    odd = want_ea2eq[wc[want_ea2eq] % 2 == 1]
+    assert np.array_equal(srt(eq2vq), srt(col[odd].astype(np.int64)))
+    assert np.array_equal(hits, np.bincount(esrc[want_ea2eq], minlength=n).astype(np.int32))
