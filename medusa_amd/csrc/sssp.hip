@@ -91,11 +91,15 @@ __device__ __forceinline__ void stage_init(Stage& st) {
     __syncthreads();
 }
 
-// One thread's appends: v to the near queue (as its slices) and/or to the far queue.
+// Out-degree of v, read by the caller before it waits on v's stamp atomic so that the two
+// round trips overlap.
+__device__ __forceinline__ uint32_t out_degree(const Sinks& k, uint32_t v) { return k.row_ptr[v + 1] - k.row_ptr[v]; }
+
+// One thread's appends: v (out-degree deg) to the near queue (as its slices) and/or to the
+// far queue.
 __device__ __forceinline__ void stage_put(Stage& st, const Sinks& k, bool to_near, bool to_far, uint32_t v,
-                                          unsigned long long& ndeg) {
+                                          uint32_t deg, unsigned long long& ndeg) {
     if (to_near) {
-        const uint32_t deg = k.row_ptr[v + 1] - k.row_ptr[v];
         ndeg += deg;
         const uint32_t cnt = slices_of(deg);
         if (cnt > kStageSlices) {
@@ -184,7 +188,7 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
         for (uint32_t e0 = 0; e0 < total; e0 += 256) {  // uniform per block
             const uint32_t e = e0 + t;
             bool to_near = false, to_far = false;
-            uint32_t v = 0;
+            uint32_t v = 0, vdeg = 0;
             if (e < total) {
                 uint32_t lo = 0;
 #pragma unroll
@@ -199,11 +203,15 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
                     // keeps one near entry per vertex
                     const uint32_t nd = (uint32_t)nd64;
                     atomicMin(&dist[v], nd);
-                    if (nd < thr) to_near = atomicMax(&stamp[v], epoch) < epoch;
-                    else to_far = true;
+                    if (nd < thr) {
+                        vdeg = out_degree(k, v);
+                        to_near = atomicMax(&stamp[v], epoch) < epoch;
+                    } else {
+                        to_far = true;
+                    }
                 }
             }
-            stage_put(st, k, to_near, to_far, v, ndeg);
+            stage_put(st, k, to_near, to_far, v, vdeg, ndeg);
             stage_flush(st, k, false);
         }
         __syncthreads();  // s_beg / s_off / s_du / s_wave are rewritten by the next chunk
@@ -223,12 +231,13 @@ __device__ __forceinline__ uint32_t split_block(const uint32_t* __restrict__ far
     for (uint32_t i0 = blockIdx.x * 256; i0 < nin; i0 += gridDim.x * 256) {  // uniform per block
         const uint32_t i = i0 + threadIdx.x;
         bool to_near = false, keep = false;
-        uint32_t v = 0;
+        uint32_t v = 0, vdeg = 0;
         if (i < nin) {
             v = rescan ? i : far_in[i];
             const uint32_t d = dist[v];
             if (d != 0xFFFFFFFFu && d >= old_thr) {
                 if (d < thr) {
+                    vdeg = out_degree(k, v);
                     to_near = atomicMax(&stamp[v], epoch) < epoch;
                 } else {
                     keep = atomicMax(&stamp[v], epoch) < epoch;
@@ -236,7 +245,7 @@ __device__ __forceinline__ uint32_t split_block(const uint32_t* __restrict__ far
                 }
             }
         }
-        stage_put(st, k, to_near, keep, v, ndeg);
+        stage_put(st, k, to_near, keep, v, vdeg, ndeg);
         stage_flush(st, k, false);
     }
     return fmin;
