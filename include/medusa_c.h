@@ -5,6 +5,11 @@
  * (MEDUSA_OK == 0, negative on error) and never exits the process; the text
  * of the last error on the calling thread is returned by medusa_last_error().
  *
+ * Threads: calls on one graph handle are serialised by a lock inside the handle, so
+ * several host threads may share a handle; different handles run independently.  A
+ * handle must not be destroyed while another thread uses it, and a policy must not be
+ * trained while a run uses it.
+ *
  * Output pointers (levels, dist, rank) may point to host memory or to device
  * memory of the graph's GPU; the library detects which and copies the final
  * result there.  Graph inputs are host arrays that are uploaded once by

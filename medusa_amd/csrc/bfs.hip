@@ -693,7 +693,7 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
 
 static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, int32_t* levels,
                       const medusa_opts* opts, medusa_stats* stats) {
-    enter(g);
+    const auto lock = enter(g);
     MDS_CHECK(levels, MEDUSA_ERR_ARG, "bfs: null output");
     MDS_CHECK(sources && count >= 1, MEDUSA_ERR_ARG, "bfs: need at least one source");
     const uint32_t n = g->n;

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -191,6 +192,8 @@ using BfsCtlStorage = CtlStorage;
 }  // namespace medusa
 
 struct medusa_graph {
+    // serialises calls on this handle (recursive: the autotuner calls the BFS entry points)
+    mutable std::recursive_mutex mu;
     uint32_t n = 0;
     uint64_t m = 0;
     bool weighted = false;
@@ -227,8 +230,8 @@ struct medusa_graph {
 
 namespace medusa {
 
-// Makes g's device current and returns it.
-void enter(const medusa_graph* g);
+// Checks g, takes its lock for the rest of the call and makes its device current.
+[[nodiscard]] std::unique_lock<std::recursive_mutex> enter(const medusa_graph* g);
 // Grows the timing-event pool to at least `count` events.
 void ensure_events(medusa_graph* g, size_t count);
 // Copies `bytes` from device pointer `src` to `dst`, which may be host or device memory.

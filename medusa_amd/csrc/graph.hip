@@ -30,9 +30,11 @@ void fail(int code, const char* fmt, ...) {
     throw Error(code, buf);
 }
 
-void enter(const medusa_graph* g) {
+std::unique_lock<std::recursive_mutex> enter(const medusa_graph* g) {
     MDS_CHECK(g != nullptr, MEDUSA_ERR_ARG, "null graph handle");
+    std::unique_lock<std::recursive_mutex> lock(g->mu);
     MDS_HIP(hipSetDevice(g->device));
+    return lock;
 }
 
 void ensure_events(medusa_graph* g, size_t count) {
@@ -273,7 +275,7 @@ extern "C" int medusa_graph_create(uint32_t n, uint64_t m, const uint32_t* row_p
 
 extern "C" int medusa_graph_set_batch(medusa_graph* g, uint32_t count, const uint32_t* graph_begin) {
     return guard([&] {
-        enter(g);
+        const auto lock = enter(g);
         MDS_CHECK(count >= 1 && graph_begin, MEDUSA_ERR_ARG, "set_batch: need count >= 1 and graph_begin");
         MDS_CHECK(graph_begin[0] == 0 && graph_begin[count] == g->n, MEDUSA_ERR_ARG,
                   "set_batch: graph_begin must start at 0 and end at n=%u", g->n);
@@ -293,6 +295,7 @@ extern "C" int medusa_graph_batch(const medusa_graph* g, uint32_t* count, uint32
                                   uint32_t capacity) {
     return guard([&] {
         MDS_CHECK(g && count, MEDUSA_ERR_ARG, "null argument");
+        const std::lock_guard<std::recursive_mutex> lock(g->mu);
         *count = (uint32_t)g->parts.size() - 1;
         MDS_CHECK(graph_begin || capacity == 0, MEDUSA_ERR_ARG, "null graph_begin");
         const size_t k = std::min<size_t>(capacity, g->parts.size());
@@ -340,6 +343,7 @@ extern "C" int medusa_graph_device_arrays(const medusa_graph* g, const uint32_t*
 extern "C" int medusa_trace_count(const medusa_graph* g, uint32_t* count) {
     return guard([&] {
         MDS_CHECK(g && count, MEDUSA_ERR_ARG, "null argument");
+        const std::lock_guard<std::recursive_mutex> lock(g->mu);
         *count = (uint32_t)g->trace.size();
     });
 }
@@ -347,6 +351,7 @@ extern "C" int medusa_trace_count(const medusa_graph* g, uint32_t* count) {
 extern "C" int medusa_trace_read(const medusa_graph* g, double* out, uint32_t capacity) {
     return guard([&] {
         MDS_CHECK(g && (out || capacity == 0), MEDUSA_ERR_ARG, "null argument");
+        const std::lock_guard<std::recursive_mutex> lock(g->mu);
         const size_t k = std::min<size_t>(capacity, g->trace.size());
         for (size_t i = 0; i < k; ++i)
             for (int f = 0; f < MEDUSA_TRACE_FIELDS; ++f) out[i * MEDUSA_TRACE_FIELDS + f] = g->trace[i].f[f];

@@ -642,7 +642,7 @@ using namespace medusa;
 extern "C" int medusa_mbfs(medusa_graph* g, const uint32_t* sources, uint32_t k, uint8_t* levels,
                            const medusa_opts* opts, medusa_stats* stats) {
     return guard([&] {
-        enter(g);
+        const auto lock = enter(g);
         MDS_CHECK(levels && sources, MEDUSA_ERR_ARG, "mbfs: null argument");
         MDS_CHECK(k >= 1 && k <= 64, MEDUSA_ERR_ARG, "mbfs: k must be in [1, 64]");
         for (uint32_t s = 0; s < k; ++s)

@@ -542,7 +542,7 @@ using namespace medusa;
 extern "C" int medusa_pagerank(medusa_graph* g, float damping, uint32_t steps, float* rank, const medusa_opts* opts,
                                medusa_stats* stats) {
     return guard([&] {
-        enter(g);
+        const auto lock = enter(g);
         MDS_CHECK(rank, MEDUSA_ERR_ARG, "pagerank: null output");
         MDS_CHECK(damping >= 0.f && damping < 1.f, MEDUSA_ERR_ARG, "pagerank: damping must be in [0,1)");
         MDS_CHECK(g->n < (1u << 24), MEDUSA_ERR_ARG, "pagerank: n must be < 2^24 (exact in FP32)");

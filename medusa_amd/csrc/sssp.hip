@@ -750,7 +750,7 @@ using namespace medusa;
 extern "C" int medusa_sssp(medusa_graph* g, uint32_t source, uint32_t* dist, const medusa_opts* opts,
                            medusa_stats* stats) {
     return guard([&] {
-        enter(g);
+        const auto lock = enter(g);
         MDS_CHECK(dist, MEDUSA_ERR_ARG, "sssp: null output");
         MDS_CHECK(source < g->n, MEDUSA_ERR_ARG, "sssp: source %u out of range (n=%u)", source, g->n);
         MDS_CHECK(g->weighted, MEDUSA_ERR_STATE, "sssp: graph has no weights");

@@ -30,7 +30,7 @@ extern "C" int medusa_mbfs(medusa_graph* g, const uint32_t* sources, uint32_t k,
 extern "C" int medusa_autotune(medusa_graph* g, uint32_t algo, const uint32_t* sources, uint32_t count,
                                uint32_t reps, medusa_tuning* result) {
     return guard([&] {
-        enter(g);
+        const auto lock = enter(g);
         MDS_CHECK(algo == MEDUSA_TUNE_BFS || algo == MEDUSA_TUNE_MBFS, MEDUSA_ERR_ARG, "autotune: unknown algo %u",
                   algo);
         if (reps == 0) {  // forget the tuned shape

@@ -286,3 +286,40 @@ def test_tiny_graphs_every_engine(oracle, case):
                 assert np.array_equal(G.mbfs(srcs, timing=timing)[0], mwant)
         r, _ = G.pagerank(0.85, 4)
         assert np.array_equal(r.view(np.uint32), oracle.pagerank_f32(rp, col, 0.85, 4).view(np.uint32))
+
+
+def test_one_handle_from_several_threads(oracle):
+    """Calls on one handle from several host threads are serialised by the handle's lock:
+    every thread gets its own source's exact levels, distances and ranks."""
+    import threading
+    n = 20000
+    rp, col, _ = random_graph(n, 8 * n, 21)
+    w = np.random.default_rng(21).integers(1, 1001, len(col)).astype(np.uint32)
+    sources = [0, 17, 999, 4321, 12345, 19999]
+    want_lv = {s: oracle.bfs_queue(rp, col, s) for s in sources}
+    want_d = {s: oracle.sssp_dijkstra(rp, col, w, s) for s in sources}
+    errors = []
+    with md.Graph(rp, col, w) as G:
+        want_pr = G.pagerank(steps=10)[0]
+
+        def work(s):
+            try:
+                for _ in range(3):
+                    lv, _ = G.bfs(s)
+                    d, _ = G.sssp(s)
+                    pr, _ = G.pagerank(steps=10)
+                    mb, _ = G.mbfs([s, (s + 1) % n])
+                    if not (np.array_equal(lv, want_lv[s]) and np.array_equal(d, want_d[s])
+                            and np.array_equal(pr, want_pr) and np.array_equal(mb[0], np.where(
+                                want_lv[s] < 0, 0xFF, want_lv[s]).astype(np.uint8))):
+                        errors.append(s)
+            except Exception as e:  # reported below
+                errors.append((s, repr(e)))
+
+        threads = [threading.Thread(target=work, args=(s,)) for s in sources]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in threads)
+    assert errors == []
