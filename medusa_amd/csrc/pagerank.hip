@@ -545,7 +545,6 @@ extern "C" int medusa_pagerank(medusa_graph* g, float damping, uint32_t steps, f
         const auto lock = enter(g);
         MDS_CHECK(rank, MEDUSA_ERR_ARG, "pagerank: null output");
         MDS_CHECK(damping >= 0.f && damping < 1.f, MEDUSA_ERR_ARG, "pagerank: damping must be in [0,1)");
-        MDS_CHECK(g->n < (1u << 24), MEDUSA_ERR_ARG, "pagerank: n must be < 2^24 (exact in FP32)");
         medusa_opts o{};
         if (opts) o = *opts;
         medusa_stats st{};

@@ -133,3 +133,20 @@ def test_chung_lu_config(oracle):
         assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 20)))
         l1 = np.abs(r.astype(np.float64) - oracle.pagerank_f64(rp, col, 0.85, 20)).sum()
         assert l1 <= L1_TOL, l1
+
+
+def test_pagerank_more_than_2_24_vertices(oracle):
+    """n above 2^24, where n and 1/n are no longer exact in FP32: the ranks stay bit-identical
+    to the FP32 restatement (which rounds n the same way) and still sum to 1."""
+    n = (1 << 24) + 4099
+    m = 2 * n
+    rng = np.random.default_rng(24)
+    src = np.sort(rng.integers(0, n, m))
+    col = rng.integers(0, n, m).astype(np.uint32)
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum(np.bincount(src, minlength=n))
+    with md.Graph(rp, col) as G:
+        r, st = G.pagerank(0.85, 5)
+    assert st.supersteps == 5
+    assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 5)))
+    assert abs(r.astype(np.float64).sum() - 1.0) < 1e-4
