@@ -544,14 +544,14 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
     static_assert(sizeof(BfsCtl) <= sizeof(BfsCtlStorage), "control block storage");
     FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
 
-    // capture (once per graph, threshold and policy buffer) the batch of supersteps
-    GraphCache& gc = g->bfs_graph;
-    const uintptr_t key[4] = {(uintptr_t)g->level.p, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
-                              (uintptr_t)g->bitmap.p};
-    if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
+    // capture (once per graph, threshold, policy buffer and slot count) a batch of supersteps
+    auto captured = [&](GraphCache& gc, uint32_t slots) -> hipGraphExec_t {
+        const uintptr_t key[4] = {(uintptr_t)g->level.p, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
+                                  (uintptr_t)g->bitmap.p ^ ((uintptr_t)slots << 48)};
+        if (gc.exec && std::equal(key, key + 4, gc.key)) return gc.exec;
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (uint32_t k = 0; k < kBfsBatch; ++k) {
+        for (uint32_t k = 0; k < slots; ++k) {
             hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)ctl,
                                (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
                                g->cptr.p, g->csrc.p, (const unsigned long long*)g->bitmap.p, g->level.p, n, ctr);
@@ -566,7 +566,10 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
         if (graph) (void)hipGraphDestroy(graph);
         MDS_HIP(e);
         std::copy(key, key + 4, gc.key);
-    }
+        return gc.exec;
+    };
+    const uint32_t head = head_slots(g->bfs_last_steps, kBfsBatch);
+    const hipGraphExec_t head_exec = captured(head == kBfsBatch ? g->bfs_graph : g->bfs_head, head);
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     bfs_seed(g, nsrc, single, large);
@@ -575,13 +578,16 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
     MDS_LAUNCHED();
     BfsCtl h{};
     for (uint32_t batch = 0;; ++batch) {
-        MDS_HIP(hipGraphLaunch(gc.exec, s));
+        // the tail batch is captured after the timed region has started only on a handle's
+        // first runs; later runs find it cached
+        MDS_HIP(hipGraphLaunch(batch == 0 ? head_exec : captured(g->bfs_graph, kBfsBatch), s));
         MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
         MDS_HIP(hipStreamSynchronize(s));
         std::memcpy(&h, g->h_ctr_raw.p, sizeof(BfsCtl));
         if (h.done) break;
         MDS_CHECK(batch < (n + kBfsBatch) / kBfsBatch + 2, MEDUSA_ERR_INTERNAL, "bfs: superstep loop did not end");
     }
+    g->bfs_last_steps = h.supersteps;
     MDS_HIP(hipEventRecord(g->ev_end, s));
     MDS_HIP(hipStreamSynchronize(s));
     float ms = 0;

@@ -183,6 +183,14 @@ struct GraphCache {
     ~GraphCache() { reset(); }
 };
 
+// Slots in the first replay of a device-driven loop: the previous run's superstep count on
+// the same handle (capped), or the regular batch when there is none.  Slots after the
+// loop's end return at once, so the choice changes only the time, never the result.
+constexpr uint32_t kMaxHeadSlots = 64;
+inline uint32_t head_slots(uint32_t last_steps, uint32_t batch) {
+    return last_steps ? (last_steps < kMaxHeadSlots ? last_steps : kMaxHeadSlots) : batch;
+}
+
 // 256 bytes of device memory for a device-driven loop's control block.
 struct CtlStorage {
     unsigned long long w[32];
@@ -222,6 +230,10 @@ struct medusa_graph {
     // device-driven superstep loops: control blocks and captured superstep batches
     medusa::DBuf<medusa::CtlStorage> bfs_ctl, mbfs_ctl, sssp_ctl;
     medusa::GraphCache bfs_graph, mbfs_graph, sssp_graph;
+    // first replay of the BFS / MBFS loops: as many slots as the previous run on this handle
+    // took supersteps, so a repeated traversal ends after one replay with no idle slots
+    medusa::GraphCache bfs_head, mbfs_head;
+    uint32_t bfs_last_steps = 0, mbfs_last_steps = 0;
     std::unique_ptr<medusa::PrLayout> pr;
     std::vector<medusa::TraceRec> trace;
     std::vector<hipEvent_t> events;  // timing pool

@@ -516,27 +516,29 @@ static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t*
         MDS_LAUNCHED();
     };
 
-    // capture (once per graph, launch shape and policy buffer) a batch of supersteps
-    GraphCache& gc = g->mbfs_graph;
-    if (!host_driven) {
+    // capture (once per graph, launch shape, policy buffer and slot count) a batch of supersteps
+    auto captured = [&](GraphCache& gc, uint32_t slots) -> hipGraphExec_t {
         const uintptr_t key[4] = {(uintptr_t)masks.p, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
-                                  (uintptr_t)lvbuf.p ^ (uintptr_t)n};
-        if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
-            gc.reset();
-            MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            for (uint32_t b = 0; b < kMbfsBatch; ++b) {
-                expand();
-                update(ctl, 0);
-                hipLaunchKernelGGL(k_mbfs_decide, dim3(1), dim3(256), 0, s, ctl, C, policy, grid, 0u);
-            }
-            hipGraph_t graph = nullptr;
-            hipError_t e = hipStreamEndCapture(s, &graph);
-            if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
-            if (graph) (void)hipGraphDestroy(graph);
-            MDS_HIP(e);
-            std::copy(key, key + 4, gc.key);
+                                  ((uintptr_t)lvbuf.p ^ (uintptr_t)n) ^ ((uintptr_t)slots << 48)};
+        if (gc.exec && std::equal(key, key + 4, gc.key)) return gc.exec;
+        gc.reset();
+        MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (uint32_t b = 0; b < slots; ++b) {
+            expand();
+            update(ctl, 0);
+            hipLaunchKernelGGL(k_mbfs_decide, dim3(1), dim3(256), 0, s, ctl, C, policy, grid, 0u);
         }
-    }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        MDS_HIP(e);
+        std::copy(key, key + 4, gc.key);
+        return gc.exec;
+    };
+    const uint32_t head = head_slots(g->mbfs_last_steps, kMbfsBatch);
+    hipGraphExec_t head_exec = nullptr;
+    if (!host_driven) head_exec = captured(head == kMbfsBatch ? g->mbfs_graph : g->mbfs_head, head);
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_mbfs_fill, dim3(grid_for(std::max<uint64_t>(n, (level_bytes + 3) / 4), 256, 8192)),
@@ -555,7 +557,7 @@ static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t*
         hipLaunchKernelGGL(k_mbfs_decide, dim3(1), dim3(256), 0, s, ctl, C, policy, grid, 1u);
         MDS_LAUNCHED();
         for (uint32_t batch = 0;; ++batch) {
-            MDS_HIP(hipGraphLaunch(gc.exec, s));
+            MDS_HIP(hipGraphLaunch(batch == 0 ? head_exec : captured(g->mbfs_graph, kMbfsBatch), s));
             MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(MbfsCtl), hipMemcpyDeviceToHost, s));
             MDS_HIP(hipStreamSynchronize(s));
             std::memcpy(&h, g->h_ctr_raw.p, sizeof(MbfsCtl));
@@ -563,6 +565,7 @@ static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t*
             MDS_CHECK(batch < (n + kMbfsBatch) / kMbfsBatch + 2, MEDUSA_ERR_INTERNAL,
                       "mbfs: superstep loop did not end");
         }
+        g->mbfs_last_steps = h.supersteps;
     } else {
         Sched sched(o, Sched::kMbfs);
         const bool timing = o.timing != 0;
