@@ -228,8 +228,8 @@ def _emit(a, metric_cfg, value, ms_per_step, roofline, cpu, extra):
 
 def _train_direction_policy(md, wl, np):
     """Push/pull cost model trained on the GPU from push-only and pull-only traced BFS runs
-    on smaller graphs (RMAT scale 14-17, Erdos-Renyi, Chung-Lu); never the measured graph."""
-    graphs = [wl.rmat_graph(scale=sc, edge_factor=16, seed=10 + sc) for sc in (14, 15, 16, 17)]
+    on smaller graphs (RMAT scale 14-19, Erdos-Renyi, Chung-Lu); never the measured graph."""
+    graphs = [wl.rmat_graph(scale=sc, edge_factor=16, seed=10 + sc) for sc in (14, 15, 16, 17, 18, 19)]
     graphs += [wl.er_graph(2 ** 15, 2 ** 19, seed=21), wl.chung_lu_graph(200_000, 420_000, seed=22),
                wl.chung_lu_graph(1_200_000, 2_500_000, seed=23)]
     Xs, ys = [], []

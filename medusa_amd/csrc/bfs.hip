@@ -376,6 +376,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
                                                     FrontierCounters* __restrict__ ctr,
                                                     const float* __restrict__ policy, uint32_t grid) {
     __shared__ MlpShared s_mlp;
+    if (policy) mlp_stage(policy, s_mlp.P);  // overlaps the loads below
     if (ctl->done) return;
     const uint32_t n_new = ctr->n_new;
     const uint32_t n_small = ctr->n_small;
@@ -414,7 +415,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
         } else if (c.mode == 3 && policy) {
             const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                      (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
-            policy_direction_costs(rec, policy, s_mlp);
+            policy_direction_costs(rec, policy, s_mlp, true);
             // the heuristic's choice stands unless the policy predicts a clear gain
             const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlpha
                                      : !((double)c.n_f < (double)c.n / Sched::kBeta);

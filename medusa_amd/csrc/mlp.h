@@ -37,15 +37,41 @@ uint32_t policy_kind(const medusa_policy* p);
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 namespace medusa {
+// Copies the parameters into LDS (no barrier).  With 256 threads every thread issues all
+// of its loads before its first store, so the copy costs one memory round trip instead of
+// one per element it moves.  A kernel may call it first thing, so that the copy overlaps
+// its other loads, and then evaluate with staged = true.
+constexpr int kStageBlock = 256;
+constexpr int kStagePer = (int)((kMlpParams + kStageBlock - 1) / kStageBlock);
+__device__ inline void mlp_stage(const float* __restrict__ P, float* sP) {
+    const int t = threadIdx.x;
+    if (blockDim.x == kStageBlock) {
+        float r[kStagePer];
+#pragma unroll
+        for (int j = 0; j < kStagePer; ++j) {
+            const int i = t + j * kStageBlock;
+            r[j] = i < (int)kMlpParams ? P[i] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < kStagePer; ++j) {
+            const int i = t + j * kStageBlock;
+            if (i < (int)kMlpParams) sP[i] = r[j];
+        }
+        return;
+    }
+    for (int i = t; i < (int)kMlpParams; i += blockDim.x) sP[i] = P[i];
+}
+
 // A workgroup evaluates the MLP on `rows` input rows held in LDS (x[rows][16]) with the
-// parameters staged once into LDS (sP[kMlpParams]).  Hidden units of all rows are computed
-// in parallel, one (row, unit) per thread; out[r] is written for every row.  The FP32
-// operations and their order per row are those of mlp_eval_host, so host and device pick
-// the same action.  Every thread of the block must call it; out[] is valid afterwards.
+// parameters staged once into LDS (sP[kMlpParams]; copied here unless `staged`).  Hidden
+// units of all rows are computed in parallel, one (row, unit) per thread; out[r] is written
+// for every row.  The FP32 operations and their order per row are those of mlp_eval_host,
+// so host and device pick the same action.  Every thread of the block must call it; out[]
+// is valid afterwards.
 __device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, const float* x, int rows, float* s_a1,
-                                     float* s_a2, float* out) {
+                                     float* s_a2, float* out, bool staged = false) {
     const int t = threadIdx.x, nt = blockDim.x;
-    for (int i = t; i < (int)kMlpParams; i += nt) sP[i] = P[i];
+    if (!staged) mlp_stage(P, sP);
     __syncthreads();
     const float* W1 = sP;
     const float* b1 = W1 + kMlpH * kMlpIn;

@@ -14,17 +14,26 @@ __host__ __device__ inline float lg1(double x) { return (float)log2(1.0 + (x > 0
 
 // Direction rows (record = step, n_f, m_f, n_u, m_u, n, m, ...): frontier and unvisited
 // vertex/edge shares on log scales, mean degree, step, size, and the candidate action
-// (0 push, 1 pull) with its interactions.
-__host__ __device__ inline void direction_row(const double* r, double action, float* x) {
-    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
+// (0 push, 1 pull) with its interactions.  The seven logarithms do not depend on the
+// action: direction_log(r, i) is logarithm i (of n, m, n_f, m_f, n_u, m_u, m/n) and
+// direction_fill builds a row from them, so a workgroup can compute them in parallel.
+constexpr int kDirectionLogs = 7;
+__host__ __device__ inline float direction_log(const double* r, int i) {
     const double n = r[5] > 1.0 ? r[5] : 1.0, m = r[6] > 1.0 ? r[6] : 1.0;
-    const float ln = lg1(n), lm = lg1(m);
+    if (i == 0) return lg1(n);
+    if (i == 1) return lg1(m);
+    if (i == 6) return lg1(m / n);
+    return lg1(r[i - 1]);  // i = 2..5: n_f, m_f, n_u, m_u
+}
+__host__ __device__ inline void direction_fill(const double* r, const float* L, double action, float* x) {
+    for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
+    const float ln = L[0], lm = L[1];
     const float a = action != 0.0 ? 1.f : 0.f;
-    x[0] = lg1(r[1]) / ln;
-    x[1] = lg1(r[2]) / lm;
-    x[2] = lg1(r[3]) / ln;
-    x[3] = lg1(r[4]) / lm;
-    x[4] = lg1(m / n) / 8.f;
+    x[0] = L[2] / ln;
+    x[1] = L[3] / lm;
+    x[2] = L[4] / ln;
+    x[3] = L[5] / lm;
+    x[4] = L[6] / 8.f;
     x[5] = (float)(r[0] < 64.0 ? r[0] : 64.0) / 64.f;
     x[6] = ln / 32.f;
     x[8] = a;
@@ -33,6 +42,11 @@ __host__ __device__ inline void direction_row(const double* r, double action, fl
     x[11] = a * x[2];
     x[12] = a * x[3];
     x[13] = 1.f - a;
+}
+__host__ __device__ inline void direction_row(const double* r, double action, float* x) {
+    float L[kDirectionLogs];
+    for (int i = 0; i < kDirectionLogs; ++i) L[i] = direction_log(r, i);
+    direction_fill(r, L, action, x);
 }
 
 // Delta rows (record = phase, far entries, mean weight, threshold, 0, n, m, ...): pending far
@@ -87,10 +101,20 @@ struct MlpShared {
 };
 
 // Predicted costs of push (out[0]) and pull (out[1]) for the superstep described by `rec`.
-// Every thread of the block calls it.
-__device__ inline void policy_direction_costs(const double* rec, const float* __restrict__ policy, MlpShared& m) {
-    if (threadIdx.x < 2) direction_row(rec, (double)threadIdx.x, m.x + threadIdx.x * MEDUSA_POLICY_INPUTS);
-    mlp_eval_rows(policy, m.P, m.x, 2, m.a1, m.a2, m.out);  // its first barrier publishes the rows
+// Every thread of the block calls it; `staged`: the kernel already ran mlp_stage into m.P.
+// The logarithms are computed by seven threads at once (in double, a long dependent chain
+// each); the rows are the same as direction_row's.
+__device__ inline void policy_direction_costs(const double* rec, const float* __restrict__ policy, MlpShared& m,
+                                              bool staged = false) {
+    float* L = m.out;  // scratch until the network writes its outputs
+    if (threadIdx.x < kDirectionLogs) L[threadIdx.x] = direction_log(rec, threadIdx.x);
+    __syncthreads();
+    float x[MEDUSA_POLICY_INPUTS];
+    if (threadIdx.x < 2) {
+        direction_fill(rec, L, (double)threadIdx.x, x);
+        for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) m.x[threadIdx.x * MEDUSA_POLICY_INPUTS + i] = x[i];
+    }
+    mlp_eval_rows(policy, m.P, m.x, 2, m.a1, m.a2, m.out, staged);  // its first barrier publishes the rows
 }
 #endif
 
