@@ -165,6 +165,7 @@ __device__ __forceinline__ void push_large(const uint32_t* __restrict__ q, uint3
 }
 
 constexpr uint32_t kPullLane = 32;  // in-degree up to which one lane scans a vertex alone
+constexpr uint32_t kPullGroup = 4;  // in-edges a lane probes per round
 
 // Bottom-up: an unvisited vertex joins if any in-neighbour is in the frontier bitmap.  A
 // lane scans a vertex of in-degree <= kPullLane alone; above that the whole wave scans it
@@ -188,13 +189,26 @@ __device__ __forceinline__ void pull_all(uint32_t n, const uint32_t* __restrict_
         }
         const bool big = todo && e - b > kPullLane;
         if (todo && !big) {
-            for (uint32_t k = b; k < e; ++k) {
-                const uint32_t u = csrc[k];
-                ++probes;
-                if ((bitmap[u >> 6] >> (u & 63)) & 1ull) {
+            // kPullGroup in-edges per round: their source and bitmap loads are issued together,
+            // so a round costs two memory round trips instead of two per in-edge; probes still
+            // count the in-edges up to and including the first hit
+            for (uint32_t k0 = b; k0 < e; k0 += kPullGroup) {
+                uint32_t u[kPullGroup];
+                unsigned long long word[kPullGroup];
+#pragma unroll
+                for (uint32_t j = 0; j < kPullGroup; ++j) u[j] = k0 + j < e ? csrc[k0 + j] : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j < kPullGroup; ++j) word[j] = k0 + j < e ? bitmap[u[j] >> 6] : 0ull;
+                uint32_t hit = kPullGroup;
+#pragma unroll
+                for (uint32_t j = kPullGroup; j-- > 0;)
+                    if ((word[j] >> (u[j] & 63)) & 1ull) hit = j;
+                if (hit < kPullGroup) {
+                    probes += hit + 1;
                     level[v] = next;
                     break;
                 }
+                probes += min(e - k0, kPullGroup);
             }
         }
         for (unsigned long long bm = __ballot(big); bm; bm &= bm - 1) {
