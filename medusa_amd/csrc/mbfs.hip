@@ -159,6 +159,7 @@ __device__ __forceinline__ void mbfs_push_large(const uint32_t* __restrict__ q, 
 }
 
 constexpr uint32_t kMbfsPullLane = 32;  // in-degree up to which one lane scans a vertex alone
+constexpr uint32_t kMbfsPullGroup = 4;  // in-edges a lane probes per round
 
 // Bottom-up: every vertex not yet visited by all sources ORs its in-neighbours' frontier
 // masks, stopping once every missing bit is found.  Vertices above kMbfsPullLane in-degree
@@ -184,11 +185,24 @@ __device__ __forceinline__ void mbfs_pull_all(uint32_t n, unsigned long long ful
         }
         const bool big = e - b > kMbfsPullLane;
         if (!big && b < e) {
+            // kMbfsPullGroup in-edges per round, their loads issued together; the masks are
+            // then merged one by one with the same early exit and probe count as a plain loop
             unsigned long long acc = 0;
-            for (uint32_t k = b; k < e; ++k) {
-                acc |= frontier[csrc[k]];
-                ++probes;
-                if ((acc | vis) == full) break;
+            bool all = false;
+            for (uint32_t k0 = b; k0 < e && !all; k0 += kMbfsPullGroup) {
+                uint32_t u[kMbfsPullGroup];
+                unsigned long long f[kMbfsPullGroup];
+#pragma unroll
+                for (uint32_t j = 0; j < kMbfsPullGroup; ++j) u[j] = k0 + j < e ? csrc[k0 + j] : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j < kMbfsPullGroup; ++j) f[j] = k0 + j < e ? frontier[u[j]] : 0ull;
+#pragma unroll
+                for (uint32_t j = 0; j < kMbfsPullGroup; ++j) {
+                    if (all || k0 + j >= e) continue;
+                    acc |= f[j];
+                    ++probes;
+                    all = (acc | vis) == full;
+                }
             }
             acc &= ~vis;
             if (acc) next[v] = acc;
