@@ -391,14 +391,14 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
                                                     const float* __restrict__ policy, uint32_t grid) {
     __shared__ MlpShared s_mlp;
     if (policy) mlp_stage(policy, s_mlp.P);  // overlaps the loads below
-    if (ctl->done) return;
-    const uint32_t n_new = ctr->n_new;
-    const uint32_t n_small = ctr->n_small;
-    const uint32_t n_large = ctr->n_large;
-    const unsigned long long deg_sum = ctr->deg_sum;
-    const unsigned long long probes = ctr->probes;
-    const unsigned long long deg_large = ctr->deg_large;
-    BfsCtl c = *ctl;
+    const uint32_t n_new = load_now(&ctr->n_new);
+    const uint32_t n_small = load_now(&ctr->n_small);
+    const uint32_t n_large = load_now(&ctr->n_large);
+    const unsigned long long deg_sum = load_now(&ctr->deg_sum);
+    const unsigned long long probes = load_now(&ctr->probes);
+    const unsigned long long deg_large = load_now(&ctr->deg_large);
+    BfsCtl c = load_now_block(ctl);
+    if (c.done) return;
     // byte model of the superstep just finished (same as the host-driven driver): push =
     // 12 per frontier vertex + 8 per out-edge + 8 per claim; pull = 2 bitmap passes (n/4) +
     // 8 per unvisited vertex + 8 per probed in-edge + 4 per claim

@@ -90,6 +90,26 @@ struct HBuf {
 // counters that share a line serialise on that line.
 #define MDS_COUNTER(type, name) alignas(128) type name
 
+#if defined(__HIPCC__)
+// A load issued where it stands: relaxed atomic loads are not sunk into the branches that
+// use them, so a one-workgroup decide kernel pays one memory round trip for all its inputs.
+template <class T>
+__device__ __forceinline__ T load_now(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A whole control block, 8 bytes at a time, the same way.
+template <class T>
+__device__ __forceinline__ T load_now_block(const T* p) {
+    static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 8, "control blocks are 8-byte aligned");
+    T out;
+    auto* o = reinterpret_cast<unsigned long long*>(&out);
+    const auto* q = reinterpret_cast<const unsigned long long*>(p);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 8; ++i) o[i] = load_now(q + i);
+    return out;
+}
+#endif
+
 // Frontier counters written by the compaction kernels, read back each superstep.
 struct FrontierCounters {
     MDS_COUNTER(uint32_t, n_small);              // frontier vertices with 1 <= outdeg <= large (wave queue)

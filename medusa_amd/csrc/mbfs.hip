@@ -414,9 +414,18 @@ __global__ __launch_bounds__(256) void k_mbfs_decide(MbfsCtl* __restrict__ ctl, 
                                                      const float* __restrict__ policy, uint32_t grid, uint32_t first) {
     __shared__ MlpShared s_mlp;
     if (policy) mlp_stage(policy, s_mlp.P);  // overlaps the loads below
-    if (ctl->done) return;
-    const MbfsCounters k = *ctr;
-    MbfsCtl c = *ctl;
+    MbfsCounters k;
+    k.n_small = load_now(&ctr->n_small);
+    k.n_large = load_now(&ctr->n_large);
+    k.n_active = load_now(&ctr->n_active);
+    k.saturated = load_now(&ctr->saturated);
+    k.active_deg = load_now(&ctr->active_deg);
+    k.new_bits = load_now(&ctr->new_bits);
+    k.reached_edges = load_now(&ctr->reached_edges);
+    k.probes = load_now(&ctr->probes);
+    k.deg_large = load_now(&ctr->deg_large);
+    MbfsCtl c = load_now_block(ctl);
+    if (c.done) return;
     const unsigned long long n = c.n;
     if (!first) {
         // byte model of the superstep just finished: push = queue + offsets + mask per active

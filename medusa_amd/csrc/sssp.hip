@@ -392,9 +392,14 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
 __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
                                                      uint32_t far_cap, const float* __restrict__ policy) {
     __shared__ MlpShared s_mlp;
-    if (ctl->done) return;
-    const SsspDevCounters k = *ctr;
-    SsspCtl c = *ctl;
+    SsspDevCounters k;
+    k.n_near = load_now(&ctr->n_near);
+    k.n_far = load_now(&ctr->n_far);
+    k.far_out = load_now(&ctr->far_out);
+    k.far_min = load_now(&ctr->far_min);
+    k.relax = load_now(&ctr->relax);
+    SsspCtl c = load_now_block(ctl);
+    if (c.done) return;
     uint32_t n_far;
     if (c.split_pending) {
         // after a split
