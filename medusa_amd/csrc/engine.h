@@ -97,6 +97,9 @@ template <class T>
 __device__ __forceinline__ T load_now(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Pins a uniform value in a scalar register at this point of the program, so the load that
+// produced it is issued here with the loads around it rather than sunk into a later branch.
+__device__ __forceinline__ void keep_here(uint32_t x) { asm volatile("" ::"s"(x)); }
 // A whole control block, 8 bytes at a time, the same way.
 template <class T>
 __device__ __forceinline__ T load_now_block(const T* p) {

@@ -361,29 +361,42 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
                                                    uint32_t* dist, uint32_t* __restrict__ stamp,
                                                    SsspDevCounters* __restrict__ ctr) {
     __shared__ Stage st;
-    if (ctl->done) return;
-    const uint32_t epoch = ctl->epoch, thr = ctl->thr;
+    const SsspCtl c = *ctl;
+    // every field this kernel reads, loaded in one round trip
+    keep_here(c.done);
+    keep_here(c.split_pending);
+    keep_here(c.n_near);
+    keep_here(c.epoch);
+    keep_here(c.thr);
+    keep_here(c.old_thr);
+    keep_here(c.near_sel);
+    keep_here(c.far_sel);
+    keep_here(c.overflow);
+    keep_here(c.n_far);
+    keep_here((uint32_t)c.n);
+    if (c.done) return;
+    const uint32_t epoch = c.epoch, thr = c.thr;
     unsigned long long ndeg = 0;
-    if (ctl->split_pending) {
-        const bool rescan = ctl->overflow != 0;
-        const uint32_t nin = rescan ? (uint32_t)ctl->n : ctl->n_far;
-        const Sinks k{ctl->near_sel ? near1 : near0, &ctr->n_near, near_cap, ctl->far_sel ? far0 : far1,
+    if (c.split_pending) {
+        const bool rescan = c.overflow != 0;
+        const uint32_t nin = rescan ? (uint32_t)c.n : c.n_far;
+        const Sinks k{c.near_sel ? near1 : near0, &ctr->n_near, near_cap, c.far_sel ? far0 : far1,
                       &ctr->far_out, far_cap, row_ptr};
         stage_init(st);
-        uint32_t fmin = split_block(ctl->far_sel ? far1 : far0, nin, rescan, dist, stamp, epoch, ctl->old_thr, thr,
+        uint32_t fmin = split_block(c.far_sel ? far1 : far0, nin, rescan, dist, stamp, epoch, c.old_thr, thr,
                                     st, k, ndeg);
         stage_flush(st, k, true);
         fmin = wave_min_u32(fmin);
         if ((threadIdx.x & (kWave - 1)) == 0 && fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
         return;
     }
-    const uint32_t N = ctl->n_near;
+    const uint32_t N = c.n_near;
     if (N == 0) return;
-    const Sinks k{ctl->near_sel ? near0 : near1, &ctr->n_near, near_cap, ctl->far_sel ? far1 : far0, &ctr->n_far,
+    const Sinks k{c.near_sel ? near0 : near1, &ctr->n_near, near_cap, c.far_sel ? far1 : far0, &ctr->n_far,
                   far_cap, row_ptr};
     stage_init(st);
     const unsigned long long relax =
-        relax_block(ctl->near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st, k, ndeg);
+        relax_block(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st, k, ndeg);
     stage_flush(st, k, true);
     if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
 }
