@@ -431,13 +431,13 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
                                                      (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
             policy_direction_costs(rec, policy, s_mlp, true);
             // the heuristic's choice stands unless the policy predicts a clear gain
-            const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlpha
+            const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs
                                      : !((double)c.n_f < (double)c.n / Sched::kBeta);
             pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain);
         } else {
             // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
             if (!pull)
-                pull = (double)c.m_f > (double)c.m_u / Sched::kAlpha;
+                pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
             else
                 pull = !((double)c.n_f < (double)c.n / Sched::kBeta);
         }
@@ -488,9 +488,9 @@ __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ c
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
         policy_direction_costs(rec, policy, s_mlp);
-        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlpha);
+        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
     } else if (c.mode == 0) {
-        pull = (double)c.m_f > (double)c.m_u / Sched::kAlpha;
+        pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
     }
     c.pull = pull ? 1u : 0u;
     c.supersteps = 1;

@@ -400,11 +400,11 @@ __device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restr
         const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                  (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
         policy_direction_costs(rec, policy, m, staged);
-        const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlpha
+        const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaMbfs
                                  : !((double)c.n_f < (double)c.n / Sched::kBeta);
         return policy_pull(m.out[0], m.out[1], plain);
     }
-    if (!pull) return (double)c.m_f > (double)c.m_u / Sched::kAlpha;
+    if (!pull) return (double)c.m_f > (double)c.m_u / Sched::kAlphaMbfs;
     return !((double)c.n_f < (double)c.n / Sched::kBeta);
 }
 

@@ -131,8 +131,13 @@ struct Frontier {
 
 struct Sched {
     enum Algo { kBfs, kSssp, kMbfs };
-    // Direction-optimising thresholds (Beamer, Asanovic, Patterson, SC'12).
-    static constexpr double kAlpha = 14.0, kBeta = 24.0;
+    // Direction-optimising thresholds (Beamer, Asanovic, Patterson, SC'12): pull once the
+    // frontier's out-edges exceed the unvisited out-edges / alpha, push again once the frontier
+    // falls below n / beta.  The paper's alpha = 14 suits the 64-source mask engine; for the
+    // single-source engine a pull superstep costs relatively more on this GPU, and per-superstep
+    // push and pull times measured on Erdos-Renyi, RMAT, Chung-Lu and lattice graphs put the
+    // best switch at alpha = 3.
+    static constexpr double kAlphaBfs = 3.0, kAlphaMbfs = 14.0, kBeta = 24.0;
     const medusa_opts& o;
     Algo algo;
     Sched(const medusa_opts& opts, Algo a) : o(opts), algo(a) {}
@@ -140,7 +145,8 @@ struct Sched {
     // Plain engine: switch to pull when the frontier's out-edges exceed the
     // unvisited out-edges / alpha; back to push when the frontier shrinks below n / beta.
     bool heuristic_pull(const Frontier& f, bool cur_pull) const {
-        if (!cur_pull) return (double)f.m_f > (double)f.m_u / kAlpha;
+        const double alpha = algo == kMbfs ? kAlphaMbfs : kAlphaBfs;
+        if (!cur_pull) return (double)f.m_f > (double)f.m_u / alpha;
         return !((double)f.n_f < (double)f.n / kBeta);
     }
     // Direction for the next superstep: the learned policy's cheaper action when one is
