@@ -390,7 +390,6 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
                                                     FrontierCounters* __restrict__ ctr,
                                                     const float* __restrict__ policy, uint32_t grid) {
     __shared__ MlpShared s_mlp;
-    if (policy) mlp_stage(policy, s_mlp.P);  // overlaps the loads below
     const uint32_t n_new = load_now(&ctr->n_new);
     const uint32_t n_small = load_now(&ctr->n_small);
     const uint32_t n_large = load_now(&ctr->n_large);
@@ -426,14 +425,16 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
             pull = false;
         } else if (c.mode == 2) {
             pull = true;
-        } else if (c.mode == 3 && policy) {
+        } else if (c.mode == 3 && policy &&
+                   bfs_policy_consulted(pull, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
+                                        Sched::kAlphaBfs, Sched::kBeta)) {
             const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                      (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
-            policy_direction_costs(rec, policy, s_mlp, true);
+            policy_direction_costs(rec, policy, s_mlp);
             // the heuristic's choice stands unless the policy predicts a clear gain
             const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs
                                      : !((double)c.n_f < (double)c.n / Sched::kBeta);
-            pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain);
+            pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain, kBfsDirectionMargin);
         } else {
             // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
             if (!pull)
@@ -484,12 +485,15 @@ __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ c
     bool pull = false;
     if (c.mode == 2) {
         pull = true;
-    } else if (c.mode == 3 && policy) {
+    } else if (c.mode == 3 && policy &&
+               bfs_policy_consulted(false, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
+                                    Sched::kAlphaBfs, Sched::kBeta)) {
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
         policy_direction_costs(rec, policy, s_mlp);
-        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
-    } else if (c.mode == 0) {
+        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs,
+                           kBfsDirectionMargin);
+    } else if (c.mode == 0 || c.mode == 3) {  // heuristic, or the policy far from its switching point
         pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
     }
     c.pull = pull ? 1u : 0u;

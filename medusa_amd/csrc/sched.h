@@ -129,6 +129,17 @@ struct Frontier {
     uint64_t m_u = 0;  // their out-edges
 };
 
+// Single-source BFS consults the learned direction policy only near the heuristic's switching
+// point, where its decision ratio (alpha * m_f / m_u while pushing, beta * n_f / n while
+// pulling) lies within a factor kPolicyBand of 1; farther out the heuristic's choice is taken
+// without evaluating the network, which saves its cost on most supersteps.
+constexpr double kPolicyBand = 8.0;
+__host__ __device__ inline bool bfs_policy_consulted(bool cur_pull, double n_f, double m_f, double m_u, double n,
+                                                     double alpha, double beta) {
+    const double num = cur_pull ? beta * n_f : alpha * m_f, den = cur_pull ? n : m_u;
+    return num * kPolicyBand > den && num < den * kPolicyBand;
+}
+
 struct Sched {
     enum Algo { kBfs, kSssp, kMbfs };
     // Direction-optimising thresholds (Beamer, Asanovic, Patterson, SC'12): pull once the
@@ -153,6 +164,9 @@ struct Sched {
     // given (and of the right kind), else the heuristic.
     bool bfs_pull(const Frontier& f, bool cur_pull) const {
         if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_DIRECTION) return heuristic_pull(f, cur_pull);
+        if (algo == kBfs && !bfs_policy_consulted(cur_pull, (double)f.n_f, (double)f.m_f, (double)f.m_u,
+                                                  (double)f.n, kAlphaBfs, kBeta))
+            return heuristic_pull(f, cur_pull);
         double rec[MEDUSA_TRACE_FIELDS];
         to_record(f, cur_pull ? 1.0 : 0.0, 0.0, rec);
         float x[MEDUSA_POLICY_INPUTS];
@@ -160,7 +174,8 @@ struct Sched {
         const float c_push = mlp_eval_host(policy_params(o.policy), x);
         policy_row(MEDUSA_POLICY_DIRECTION, rec, 1.0, x);
         const float c_pull = mlp_eval_host(policy_params(o.policy), x);
-        return policy_pull(c_push, c_pull, heuristic_pull(f, cur_pull));
+        return policy_pull(c_push, c_pull, heuristic_pull(f, cur_pull),
+                           algo == kMbfs ? kDirectionMargin : kBfsDirectionMargin);
     }
     // SSSP bucket width at a threshold advance: the learned policy picks among
     // wmean * 2^k (k = -2..8) the width with the lowest predicted time per unit of distance,
