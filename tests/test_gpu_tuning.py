@@ -9,7 +9,7 @@ import pytest
 
 import medusa_amd as md
 from medusa_amd import workloads as wl
-from conftest import random_graph
+from conftest import csr_from_edges, random_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -135,3 +135,19 @@ def test_bad_grid_is_an_error():
             G.mbfs([0], grid=70000)
         with pytest.raises(md.MedusaError):
             G.autotune(7, [0])
+
+
+def test_first_replay_sized_from_previous_run(oracle):
+    """The device loops size their first replay from the previous run on the handle: a long
+    traversal after a short one (more slots than predicted) and a short one after a long one
+    (idle slots) must both give exact levels."""
+    n = 300
+    path = [(i, i + 1) for i in range(199)]              # 200-vertex path: 200 supersteps
+    star = [(200, j) for j in range(201, 300)]           # star: 2 supersteps
+    rp, col, _ = csr_from_edges(n, path + star)
+    with md.Graph(rp, col) as G:
+        for src in (200, 0, 200, 0, 150, 200):
+            lv, st = G.bfs(src)
+            assert np.array_equal(lv, oracle.bfs_queue(rp, col, src)), src
+            out, _ = G.mbfs([src, 100])
+            assert np.array_equal(out, oracle.mbfs(rp, col, np.array([src, 100], np.uint32))), src
