@@ -166,6 +166,7 @@ struct PrLayout {
     DBuf<uint32_t> d_part_begin;  // G+1 range starts (internal ids = original ids at range level)
     uint32_t total_wgs = 0;
     DBuf<PrRun> d_runs;
+    DBuf<uint32_t> d_wg_run;  // run of every workgroup of the superstep launch (total_wgs)
     DBuf<uint32_t> ell;     // ELL slabs of the kPrLane runs
     DBuf<uint32_t> perm;    // internal -> original
     DBuf<uint32_t> iperm;   // original -> internal

@@ -176,6 +176,12 @@ void build_pr_layout(medusa_graph* g) {
     hipStream_t s = g->stream;
     L->ell.upload(ell.data(), ell.size(), s);
     L->d_runs.upload(L->runs.data(), L->runs.size(), s);
+    std::vector<uint32_t> wg_run(std::max<uint32_t>(L->total_wgs, 1), 0u);
+    for (uint32_t r = 0; r < (uint32_t)L->runs.size(); ++r) {
+        const uint32_t end = r + 1 < L->runs.size() ? L->runs[r + 1].wg_begin : L->total_wgs;
+        for (uint32_t w = L->runs[r].wg_begin; w < end; ++w) wg_run[w] = r;
+    }
+    L->d_wg_run.upload(wg_run.data(), wg_run.size(), s);
     if (!L->blocks.empty()) L->d_blocks.upload(L->blocks.data(), L->blocks.size(), s);
     L->d_parts.upload(L->parts.data(), L->parts.size(), s);
     L->d_part_begin.upload(pb.data(), pb.size(), s);

@@ -140,7 +140,7 @@ __device__ __forceinline__ float lane_vertex_sum(uint32_t deg, const uint32_t* _
 // One superstep for every vertex.  kFromMsg: values come from the message
 // array in slot order (push variant) instead of contrib[src] (pull variant).
 template <bool kFromMsg>
-__global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs, uint32_t nruns,
+__global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs, const uint32_t* __restrict__ wg_run,
                                                  const uint32_t* __restrict__ ell, const uint32_t* __restrict__ cptr,
                                                  const uint32_t* __restrict__ crow,
                                                  const float* __restrict__ vals,
@@ -148,13 +148,7 @@ __global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs,
                                                  const float* __restrict__ base_p, float d,
                                                  float* __restrict__ rank_out, float* __restrict__ contrib_out) {
     const uint32_t wg = blockIdx.x;
-    uint32_t lo = 0, hi = nruns;  // last run with wg_begin <= wg
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (runs[mid].wg_begin <= wg) lo = mid;
-        else hi = mid;
-    }
-    const PrRun R = runs[lo];
+    const PrRun R = runs[wg_run[wg]];  // a table lookup, not a search at every workgroup's start
     const uint32_t local = wg - R.wg_begin;
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const float base = base_p[R.part];
@@ -440,11 +434,11 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
             if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 1], s));
             if (step_push)
                 hipLaunchKernelGGL(k_pr_main<true>, dim3(L.total_wgs), dim3(256), 0, s, (const PrRun*)L.d_runs.p,
-                                   (uint32_t)L.runs.size(), (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p,
+                                   (const uint32_t*)L.d_wg_run.p, (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p,
                                    (const float*)L.msg.p, L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             else
                 hipLaunchKernelGGL(k_pr_main<false>, dim3(L.total_wgs), dim3(256), 0, s, (const PrRun*)L.d_runs.p,
-                                   (uint32_t)L.runs.size(), (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p, c_in,
+                                   (const uint32_t*)L.d_wg_run.p, (const uint32_t*)L.ell.p, L.cptr.p, L.crow.p, c_in,
                                    L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             MDS_LAUNCHED();
             if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 2], s));
