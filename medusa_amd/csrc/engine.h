@@ -189,6 +189,20 @@ struct PrLayout {
     }
 };
 
+// Layout for pull supersteps over 64-bit masks (MBFS): vertices with in-edges in in-degree-
+// descending order, cut into runs as in PrLayout (equal in-degree 1..64: a lane per vertex
+// over an ELL slab of ORIGINAL source ids, so the k-th sources of 64 vertices are one coalesced
+// load; in-degree > 64: a wave per vertex over the CSC segment).  Vertices without in-edges
+// never join through a pull and are left out.
+struct PullLayout {
+    std::vector<PrRun> runs;
+    uint32_t total_wgs = 0;  // work items of 256 lane-vertices or 4 wave-vertices
+    DBuf<PrRun> d_runs;
+    DBuf<uint32_t> wg_run;   // run of every work item
+    DBuf<uint32_t> perm;     // sorted position -> original vertex id
+    DBuf<uint32_t> ell;      // ELL slabs (original source ids)
+};
+
 struct TraceRec {
     double f[MEDUSA_TRACE_FIELDS];
 };
@@ -259,6 +273,7 @@ struct medusa_graph {
     medusa::GraphCache bfs_head, mbfs_head;
     uint32_t bfs_last_steps = 0, mbfs_last_steps = 0;
     std::unique_ptr<medusa::PrLayout> pr;
+    std::unique_ptr<medusa::PullLayout> pull;  // built on the first MBFS run
     std::vector<medusa::TraceRec> trace;
     std::vector<hipEvent_t> events;  // timing pool
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -282,6 +297,7 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_wg, uint32_t cap = 8192) {
     return (uint32_t)g;
 }
 void build_pr_layout(medusa_graph* g);
+void build_pull_layout(medusa_graph* g);
 
 constexpr size_t kCounterWords = 256;  // 2 KiB
 template <class T>

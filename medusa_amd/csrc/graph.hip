@@ -67,6 +67,62 @@ void copy_out(medusa_graph* g, void* dst, const void* src, size_t bytes) {
 
 // Internal id order for PageRank: inside every packed graph's id range,
 // (dangling last, in-degree descending, original id).
+void build_pull_layout(medusa_graph* g) {
+    if (g->pull) return;
+    const uint32_t n = g->n;
+    auto L = std::make_unique<PullLayout>();
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    for (uint32_t v = 0; v < n; ++v)
+        if (g->h_cptr[v + 1] > g->h_cptr[v]) order.push_back(v);
+    auto indeg = [&](uint32_t v) { return g->h_cptr[v + 1] - g->h_cptr[v]; };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return indeg(a) > indeg(b); });
+    const uint32_t cnt = (uint32_t)order.size();
+    uint32_t wg = 0;
+    uint64_t ell_size = 0;
+    for (uint32_t p = 0; p < cnt;) {
+        const uint32_t deg = indeg(order[p]);
+        const bool lane = deg <= kPrLaneMax;
+        uint32_t q = p + 1;
+        while (q < cnt && (lane ? indeg(order[q]) == deg : indeg(order[q]) > kPrLaneMax)) ++q;
+        PrRun r{};
+        r.begin = p;
+        r.count = q - p;
+        r.kind = lane ? kPrLane : kPrWave;
+        r.wg_begin = wg;
+        if (lane) {
+            r.deg = deg;
+            r.ell_stride = (r.count + 63u) & ~63u;
+            r.ell_base = ell_size;
+            ell_size += (uint64_t)deg * r.ell_stride;
+        }
+        wg += (r.count + (lane ? 255u : 3u)) / (lane ? 256u : 4u);
+        L->runs.push_back(r);
+        p = q;
+    }
+    L->total_wgs = wg;
+    std::vector<uint32_t> ell(std::max<uint64_t>(ell_size, 1), 0u);
+    for (const PrRun& r : L->runs) {
+        if (r.kind != kPrLane) continue;
+        for (uint32_t i = 0; i < r.count; ++i) {
+            const uint32_t b = g->h_cptr[order[r.begin + i]];
+            for (uint32_t k = 0; k < r.deg; ++k) ell[r.ell_base + (uint64_t)k * r.ell_stride + i] = g->h_csrc[b + k];
+        }
+    }
+    std::vector<uint32_t> wg_run(std::max<uint32_t>(wg, 1), 0u);
+    for (uint32_t r = 0; r < (uint32_t)L->runs.size(); ++r) {
+        const uint32_t end = r + 1 < L->runs.size() ? L->runs[r + 1].wg_begin : wg;
+        for (uint32_t w = L->runs[r].wg_begin; w < end; ++w) wg_run[w] = r;
+    }
+    hipStream_t s = g->stream;
+    if (!L->runs.empty()) L->d_runs.upload(L->runs.data(), L->runs.size(), s);
+    L->wg_run.upload(wg_run.data(), wg_run.size(), s);
+    if (cnt) L->perm.upload(order.data(), cnt, s);
+    L->ell.upload(ell.data(), ell.size(), s);
+    MDS_HIP(hipStreamSynchronize(s));
+    g->pull = std::move(L);
+}
+
 void build_pr_layout(medusa_graph* g) {
     if (g->pr) return;
     const uint32_t n = g->n;

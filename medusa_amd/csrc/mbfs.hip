@@ -158,59 +158,34 @@ __device__ __forceinline__ void mbfs_push_large(const uint32_t* __restrict__ q, 
     }
 }
 
-constexpr uint32_t kMbfsPullLane = 32;  // in-degree up to which one lane scans a vertex alone
 constexpr uint32_t kMbfsPullGroup = 4;  // in-edges a lane probes per round
 
-// Bottom-up: every vertex not yet visited by all sources ORs its in-neighbours' frontier
-// masks, stopping once every missing bit is found.  Vertices above kMbfsPullLane in-degree
-// are scanned by the whole wave, 64 in-edges per step with an OR reduction (their probe
-// count then includes the rest of the last 64-edge step).
-__device__ __forceinline__ void mbfs_pull_all(uint32_t n, unsigned long long full, const uint32_t* __restrict__ cptr,
-                                              const uint32_t* __restrict__ csrc,
+// Bottom-up over the pull layout: a vertex missing some sources ORs its in-neighbours'
+// frontier masks until every missing bit is found.  Work item w of the layout is 256
+// lane-vertices of one equal-in-degree run (their k-th sources are one coalesced ELL load;
+// kMbfsPullGroup in-edges per round, loads issued together, merged one by one with the same
+// early exit and probe count as a plain loop) or 4 wave-vertices of in-degree > 64 (the wave
+// scans 64 in-edges per step and ORs them by shuffles).  Workgroups stride over the items.
+// `probes` counts in-edges examined up to and including the last one needed.
+__device__ __forceinline__ void mbfs_pull_ell(unsigned long long full, const PrRun* __restrict__ runs,
+                                              const uint32_t* __restrict__ wg_run, uint32_t items,
+                                              const uint32_t* __restrict__ perm, const uint32_t* __restrict__ ell,
+                                              const uint32_t* __restrict__ cptr, const uint32_t* __restrict__ csrc,
                                               const unsigned long long* __restrict__ frontier,
                                               const unsigned long long* __restrict__ visited,
                                               unsigned long long* __restrict__ next, MbfsCounters* __restrict__ ctr) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     unsigned long long probes = 0;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t v = base + lane;  // uniform loop: every lane of the wave iterates together
-        uint32_t b = 0, e = 0;
-        unsigned long long vis = full;
-        if (v < n) {
-            vis = visited[v];
-            if (vis != full) {
-                b = cptr[v];
-                e = cptr[v + 1];
-            }
-        }
-        const bool big = e - b > kMbfsPullLane;
-        if (!big && b < e) {
-            // kMbfsPullGroup in-edges per round, their loads issued together; the masks are
-            // then merged one by one with the same early exit and probe count as a plain loop
-            unsigned long long acc = 0;
-            bool all = false;
-            for (uint32_t k0 = b; k0 < e && !all; k0 += kMbfsPullGroup) {
-                uint32_t u[kMbfsPullGroup];
-                unsigned long long f[kMbfsPullGroup];
-#pragma unroll
-                for (uint32_t j = 0; j < kMbfsPullGroup; ++j) u[j] = k0 + j < e ? csrc[k0 + j] : 0u;
-#pragma unroll
-                for (uint32_t j = 0; j < kMbfsPullGroup; ++j) f[j] = k0 + j < e ? frontier[u[j]] : 0ull;
-#pragma unroll
-                for (uint32_t j = 0; j < kMbfsPullGroup; ++j) {
-                    if (all || k0 + j >= e) continue;
-                    acc |= f[j];
-                    ++probes;
-                    all = (acc | vis) == full;
-                }
-            }
-            acc &= ~vis;
-            if (acc) next[v] = acc;
-        }
-        for (unsigned long long bm = __ballot(big); bm; bm &= bm - 1) {
-            const uint32_t owner = __builtin_ctzll(bm);
-            const uint32_t vb = __shfl(b, owner, kWave), ve = __shfl(e, owner, kWave);
-            const unsigned long long ovis = __shfl(vis, owner, kWave);
+    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {  // uniform per block
+        const PrRun R = runs[wg_run[w]];
+        const uint32_t local = w - R.wg_begin;
+        if (R.kind == kPrWave) {
+            const uint32_t p = R.begin + local * 4 + wave;
+            if (p >= R.begin + R.count) continue;  // uniform across the wave
+            const uint32_t v = perm[p];
+            const unsigned long long vis = visited[v];
+            if (vis == full) continue;
+            const uint32_t vb = cptr[v], ve = cptr[v + 1];
             unsigned long long acc = 0;
             for (uint32_t k0 = vb; k0 < ve; k0 += kWave) {
                 const uint32_t k = k0 + lane;
@@ -219,11 +194,38 @@ __device__ __forceinline__ void mbfs_pull_all(uint32_t n, unsigned long long ful
                 for (int off = kWave / 2; off >= 1; off >>= 1) x |= __shfl_xor(x, off, kWave);
                 acc |= x;
                 if (lane == 0) probes += min(ve - k0, (uint32_t)kWave);
-                if ((acc | ovis) == full) break;
+                if ((acc | vis) == full) break;
             }
-            acc &= ~ovis;
-            if (lane == 0 && acc) next[base + owner] = acc;
+            acc &= ~vis;
+            if (lane == 0 && acc) next[v] = acc;
+            continue;
         }
+        const uint32_t i = local * 256 + threadIdx.x;
+        if (i >= R.count) continue;
+        const uint32_t v = perm[R.begin + i];
+        const unsigned long long vis = visited[v];
+        if (vis == full) continue;
+        const uint32_t* col = ell + R.ell_base + i;
+        const uint32_t deg = R.deg, st = R.ell_stride;
+        unsigned long long acc = 0;
+        bool all = false;
+        for (uint32_t k0 = 0; k0 < deg && !all; k0 += kMbfsPullGroup) {
+            uint32_t u[kMbfsPullGroup];
+            unsigned long long f[kMbfsPullGroup];
+#pragma unroll
+            for (uint32_t j = 0; j < kMbfsPullGroup; ++j) u[j] = k0 + j < deg ? col[(size_t)(k0 + j) * st] : 0u;
+#pragma unroll
+            for (uint32_t j = 0; j < kMbfsPullGroup; ++j) f[j] = k0 + j < deg ? frontier[u[j]] : 0ull;
+#pragma unroll
+            for (uint32_t j = 0; j < kMbfsPullGroup; ++j) {
+                if (all || k0 + j >= deg) continue;
+                acc |= f[j];
+                ++probes;
+                all = (acc | vis) == full;
+            }
+        }
+        acc &= ~vis;
+        if (acc) next[v] = acc;
     }
     // one atomic per workgroup (every wave of the block reaches this point)
     __shared__ unsigned long long s_probe[4];
@@ -244,11 +246,13 @@ __global__ __launch_bounds__(256) void k_mbfs_expand(const MbfsCtl* __restrict__
                                                      const uint32_t* __restrict__ csrc,
                                                      const unsigned long long* __restrict__ frontier,
                                                      const unsigned long long* __restrict__ visited,
-                                                     unsigned long long* __restrict__ next, uint32_t n,
-                                                     MbfsCounters* __restrict__ ctr) {
+                                                     unsigned long long* __restrict__ next,
+                                                     const PrRun* __restrict__ runs, const uint32_t* __restrict__ wg_run,
+                                                     uint32_t items, const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ ell, MbfsCounters* __restrict__ ctr) {
     if (ctl->done) return;
     if (ctl->pull) {
-        mbfs_pull_all(n, ctl->full, cptr, csrc, frontier, visited, next, ctr);
+        mbfs_pull_ell(ctl->full, runs, wg_run, items, perm, ell, cptr, csrc, frontier, visited, next, ctr);
         return;
     }
     const uint32_t hub = ctl->hub_wgs;
@@ -528,11 +532,15 @@ static void mbfs_run(medusa_graph* g, const uint32_t* srcs, uint32_t k, uint8_t*
     MDS_HIP(hipMemcpyAsync(g->q_aux.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
     g->trace.clear();
 
+    build_pull_layout(g);
+    const PullLayout& PL = *g->pull;
     auto expand = [&]() {
         hipLaunchKernelGGL(k_mbfs_expand, dim3(grid), dim3(256), 0, s, (const MbfsCtl*)ctl,
                            (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
                            g->cptr.p, g->csrc.p, (const unsigned long long*)frontier,
-                           (const unsigned long long*)visited, next, n, C);
+                           (const unsigned long long*)visited, next, (const PrRun*)PL.d_runs.p,
+                           (const uint32_t*)PL.wg_run.p, PL.total_wgs, (const uint32_t*)PL.perm.p,
+                           (const uint32_t*)PL.ell.p, C);
         MDS_LAUNCHED();
     };
     auto update = [&](const MbfsCtl* c, int32_t lv) {
