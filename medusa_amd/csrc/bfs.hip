@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
             // the heuristic's choice stands unless the policy predicts a clear gain
             const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs
                                      : !((double)c.n_f < (double)c.n / Sched::kBeta);
-            pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain, kBfsDirectionMargin);
+            pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain);
         } else {
             // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
             if (!pull)
@@ -527,8 +527,7 @@ __global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ c
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
         policy_direction_costs(rec, policy, s_mlp);
-        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs,
-                           kBfsDirectionMargin);
+        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
     } else if (c.mode == 0 || c.mode == 3) {  // heuristic, or the policy far from its switching point
         pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
     }

@@ -400,7 +400,9 @@ __device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restr
                                bool staged) {
     if (c.mode == 1) return false;
     if (c.mode == 2) return true;
-    if (c.mode == 3 && policy) {
+    if (c.mode == 3 && policy &&
+        bfs_policy_consulted(pull, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n, Sched::kAlphaMbfs,
+                             Sched::kBeta)) {
         const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                  (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
         policy_direction_costs(rec, policy, m, staged);
@@ -417,7 +419,6 @@ __device__ bool mbfs_direction(const MbfsCtl& c, bool pull, const float* __restr
 __global__ __launch_bounds__(256) void k_mbfs_decide(MbfsCtl* __restrict__ ctl, MbfsCounters* __restrict__ ctr,
                                                      const float* __restrict__ policy, uint32_t grid, uint32_t first) {
     __shared__ MlpShared s_mlp;
-    if (policy) mlp_stage(policy, s_mlp.P);  // overlaps the loads below
     MbfsCounters k;
     k.n_small = load_now(&ctr->n_small);
     k.n_large = load_now(&ctr->n_large);
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void k_mbfs_decide(MbfsCtl* __restrict__ ctl, 
     c.n_active = k.n_active;
     c.hub_wgs = mbfs_hub_wgs(grid, k.n_small, k.n_large, k.deg_large, k.active_deg);
     if (k.n_active == 0) c.done = 1;
-    const bool pull = c.done ? false : mbfs_direction(c, first ? false : c.pull != 0, policy, s_mlp, true);
+    const bool pull = c.done ? false : mbfs_direction(c, first ? false : c.pull != 0, policy, s_mlp, false);
     c.pull = pull ? 1u : 0u;
     __syncthreads();  // every wave has read the control block and counters
     if (threadIdx.x == 0) {

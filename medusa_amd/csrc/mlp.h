@@ -15,19 +15,16 @@ constexpr uint32_t kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH +
 constexpr float kMlpSlope = 1.0f / 64.0f;
 // The policy overrides the plain engine's choice only when it predicts the other action to
 // be cheaper by more than a margin (log2 units of cost); otherwise the plain engine's choice
-// stands, so an uncertain model falls back to it.  Direction: 0.5 (about 41 %) for the
-// 64-source mask engine, 1.0 (2x) for single-source BFS, whose tuned heuristic is within a few
-// per cent of the best choice per superstep on the graphs measured; bucket width and kernel
-// variant: 0.25 (about 19 %).
+// stands, so an uncertain model falls back to it.  Direction: 1.0 (2x), because the tuned
+// direction heuristics are close to the best choice per superstep on the graphs measured;
+// bucket width and kernel variant: 0.25 (about 19 %).
 constexpr float kPolicyMargin = 0.25f;
-constexpr float kDirectionMargin = 0.5f;
-constexpr float kBfsDirectionMargin = 1.0f;
+constexpr float kDirectionMargin = 1.0f;
 
 // Direction from the two predicted costs and the plain engine's choice.
-inline __host__ __device__ bool policy_pull(float c_push, float c_pull, bool plain_pull,
-                                            float margin = kDirectionMargin) {
-    if (c_pull + margin < c_push) return true;
-    if (c_push + margin < c_pull) return false;
+inline __host__ __device__ bool policy_pull(float c_push, float c_pull, bool plain_pull) {
+    if (c_pull + kDirectionMargin < c_push) return true;
+    if (c_push + kDirectionMargin < c_pull) return false;
     return plain_pull;
 }
 

@@ -129,7 +129,7 @@ struct Frontier {
     uint64_t m_u = 0;  // their out-edges
 };
 
-// Single-source BFS consults the learned direction policy only near the heuristic's switching
+// BFS and MBFS consult the learned direction policy only near the heuristic's switching
 // point, where its decision ratio (alpha * m_f / m_u while pushing, beta * n_f / n while
 // pulling) lies within a factor kPolicyBand of 1; farther out the heuristic's choice is taken
 // without evaluating the network, which saves its cost on most supersteps.
@@ -144,11 +144,12 @@ struct Sched {
     enum Algo { kBfs, kSssp, kMbfs };
     // Direction-optimising thresholds (Beamer, Asanovic, Patterson, SC'12): pull once the
     // frontier's out-edges exceed the unvisited out-edges / alpha, push again once the frontier
-    // falls below n / beta.  The paper's alpha = 14 suits the 64-source mask engine; for the
-    // single-source engine a pull superstep costs relatively more on this GPU, and per-superstep
-    // push and pull times measured on Erdos-Renyi, RMAT, Chung-Lu and lattice graphs put the
-    // best switch at alpha = 3.
-    static constexpr double kAlphaBfs = 3.0, kAlphaMbfs = 14.0, kBeta = 24.0;
+    // falls below n / beta.  The paper uses alpha = 14.  Set from per-superstep push and pull
+    // times measured on Erdos-Renyi, RMAT, Chung-Lu and lattice graphs: alpha = 3 for the
+    // single-source engine, whose pull costs relatively more on this GPU, and alpha = 100 for
+    // the 64-source mask engine, whose pull (over coalesced ELL slabs, reading nearly every
+    // in-edge) beats the push much earlier.
+    static constexpr double kAlphaBfs = 3.0, kAlphaMbfs = 100.0, kBeta = 24.0;
     const medusa_opts& o;
     Algo algo;
     Sched(const medusa_opts& opts, Algo a) : o(opts), algo(a) {}
@@ -164,8 +165,8 @@ struct Sched {
     // given (and of the right kind), else the heuristic.
     bool bfs_pull(const Frontier& f, bool cur_pull) const {
         if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_DIRECTION) return heuristic_pull(f, cur_pull);
-        if (algo == kBfs && !bfs_policy_consulted(cur_pull, (double)f.n_f, (double)f.m_f, (double)f.m_u,
-                                                  (double)f.n, kAlphaBfs, kBeta))
+        if (!bfs_policy_consulted(cur_pull, (double)f.n_f, (double)f.m_f, (double)f.m_u, (double)f.n,
+                                  algo == kMbfs ? kAlphaMbfs : kAlphaBfs, kBeta))
             return heuristic_pull(f, cur_pull);
         double rec[MEDUSA_TRACE_FIELDS];
         to_record(f, cur_pull ? 1.0 : 0.0, 0.0, rec);
@@ -174,8 +175,7 @@ struct Sched {
         const float c_push = mlp_eval_host(policy_params(o.policy), x);
         policy_row(MEDUSA_POLICY_DIRECTION, rec, 1.0, x);
         const float c_pull = mlp_eval_host(policy_params(o.policy), x);
-        return policy_pull(c_push, c_pull, heuristic_pull(f, cur_pull),
-                           algo == kMbfs ? kDirectionMargin : kBfsDirectionMargin);
+        return policy_pull(c_push, c_pull, heuristic_pull(f, cur_pull));
     }
     // SSSP bucket width at a threshold advance: the learned policy picks among
     // wmean * 2^k (k = -2..8) the width with the lowest predicted time per unit of distance,
