@@ -1,7 +1,8 @@
 """Pins the CPU oracle: golden fixtures, closed forms, invariants and brute force.
 
-The reference ships no tests or golden vectors (SURVEY: none exist), so every
-oracle function is checked here against something other than its own formula.
+The reference ships no tests or golden vectors (its datasets are not in the repository
+either, .MISSING_LARGE_BLOBS), so every oracle function is checked here against something
+other than its own formula.
 """
 import json
 import pathlib
