@@ -16,6 +16,8 @@
 //          `large` out-degree get split over whole workgroups;
 //   pull:  every unvisited vertex scans its in-edges for a parent in the
 //          frontier bitmap and stops at the first hit;
+//   direction: Beamer's rule with thresholds measured for this GPU (Sched), or the
+//          learned policy near the rule's switching point;
 //   compaction: one pass builds the next frontier bitmap (ballot per 64 vertices)
 //          and the small/large queues (ballot + in-workgroup prefix scan, one
 //          atomic per workgroup per queue) and counts the frontier's out-edges.
@@ -25,9 +27,10 @@
 // Two drivers run the same kernels:
 //   device-driven (default): a one-workgroup kernel after each compaction decides the
 //     next superstep (direction by the heuristic or the learned policy, queue sizes, the
-//     termination vote) into a control block in HBM; kBfsBatch supersteps of
-//     [expand, compact, decide] are captured once as a hipGraph and replayed until the
-//     control block says done, so the host waits once per batch, not per superstep;
+//     termination vote) into a control block in HBM; batches of [expand, compact, decide]
+//     slots are captured once as hipGraphs and replayed until the control block says
+//     done, so the host waits once per batch, not per superstep (the first batch has as
+//     many slots as the previous run on the handle had supersteps, then kBfsBatch);
 //   host-driven (timing or trace requested): the host reads the counters after every
 //     superstep, times the expansion with hipEvents and logs the trace record.
 

@@ -11,16 +11,18 @@
 //         up to `large` out-degree go through the wave-cooperative queue, hubs above it
 //         are split over whole workgroups (the threshold is autotunable);
 //   pull: every vertex not yet visited by all sources ORs the frontier masks of its
-//         in-neighbours, stopping early once every missing bit has been found;
+//         in-neighbours, stopping early once every missing bit has been found; vertices are
+//         visited in in-degree order over ELL slabs of source ids (the pull layout), so the
+//         index loads are coalesced;
 //   update: new = next & ~visited; visited |= new; frontier = new; level[s][v] =
 //         step+1 for each new bit s; the two active queues and the counters are rebuilt
 //         with ballot + in-workgroup prefix scan.  Seeding is the same update at level 0
 //         after the sources' bits are placed in next[].
 //
 // Drivers as for BFS: device-driven (a one-workgroup decide kernel after each update
-// writes the next superstep into a control block; kMbfsBatch supersteps are captured as
-// one hipGraph and replayed until done) and host-driven (timing or tracing) over the
-// same kernels.
+// writes the next superstep into a control block; batches of supersteps are captured as
+// hipGraphs and replayed until done, the first batch sized by the previous run on the
+// handle) and host-driven (timing or tracing) over the same kernels.
 
 #include <algorithm>
 #include <cstring>
