@@ -323,3 +323,22 @@ def test_one_handle_from_several_threads(oracle):
             t.join(timeout=120)
         assert not any(t.is_alive() for t in threads)
     assert errors == []
+
+
+@pytest.mark.slow
+def test_rmat22_four_times_the_configs(oracle):
+    """RMAT scale 22 (4,194,304 vertices, 67,108,864 arcs), four times the RMAT-20 configs:
+    BFS, SSSP and 8-source MBFS stay bit-exact against the oracle."""
+    rp, col = wl.rmat_graph(scale=22, edge_factor=16, seed=22)
+    w = wl.weighted(rp, col, seed=23)
+    src = wl.max_outdeg_vertex(rp)
+    want = oracle.bfs_queue(rp, col, src)
+    with md.Graph(rp, col, w) as G:
+        for d in _dirs():
+            lv, _ = G.bfs(src, direction=d)
+            assert np.array_equal(lv, want), d
+        dist, _ = G.sssp(src)
+        assert np.array_equal(dist, oracle.sssp_dijkstra(rp, col, w, src))
+        srcs = wl.mbfs_sources(rp, 8, seed=24)
+        out, _ = G.mbfs(srcs)
+        assert np.array_equal(out, oracle.mbfs(rp, col, srcs))
