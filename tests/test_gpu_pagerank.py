@@ -150,3 +150,16 @@ def test_pagerank_more_than_2_24_vertices(oracle):
     assert st.supersteps == 5
     assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 5)))
     assert abs(r.astype(np.float64).sum() - 1.0) < 1e-4
+
+
+@pytest.mark.slow
+def test_pagerank_rmat22(oracle):
+    """RMAT scale 22 (4,194,304 vertices, 67,108,864 arcs): 10 supersteps bit-identical to
+    the FP32 restatement, within L1 1e-6 of FP64."""
+    rp, col = wl.rmat_graph(scale=22, edge_factor=16, seed=22)
+    with md.Graph(rp, col) as G:
+        r, st = G.pagerank(0.85, 10, use_graph=1)
+    assert st.supersteps == 10
+    assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 10)))
+    l1 = np.abs(r.astype(np.float64) - oracle.pagerank_f64(rp, col, 0.85, 10)).sum()
+    assert l1 <= L1_TOL, l1
