@@ -282,9 +282,9 @@ def _train_delta_policy(md, wl, np, family):
             for d in (250, 500, 1000, 2000, 4000, 8000, 16000):
                 H.sssp(src, delta=d, trace=1)
                 for r in H.trace():
-                    if r[7] > 0 and r[8] > 0:
+                    if r[4] > 0 and r[7] > 0 and r[8] > 0:  # time per settled vertex, in log2 ns
                         Xs.append(md.policy_features(md.POLICY_DELTA, r, r[7]))
-                        ys.append(np.log2(r[8] * 1e3 / r[7] + 1e-3))
+                        ys.append(np.log2(r[8] * 1e6 / r[4]))
     pol = md.Policy(md.POLICY_DELTA, seed=3)
     pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
     return pol
@@ -376,8 +376,11 @@ def run_workload(a, md, wl, torch, np):
                  "gteps_plain": round(a.steps * m_r / wp / 1e9, 4), "supersteps_plain": rp_[-1].supersteps,
                  "learned_over_plain": round(wp / wall, 3),
                  "distances_identical_to_plain": bool(np.array_equal(learned, out.cpu().numpy()))}
-        for name, d in (("bellman_ford", 0xFFFFFFFF), ("delta_1000", 1000), ("delta_8000", 8000)):
-            wt_, r_ = _timed(lambda: G.sssp(src, out=out, delta=d)[1], max(1, a.steps // 2), 1, torch)
+        # fixed widths, and the learned widths with one hop per superstep (no local rounds)
+        for name, kw in (("bellman_ford", {"delta": 0xFFFFFFFF}), ("delta_1000", {"delta": 1000}),
+                         ("delta_8000", {"delta": 8000}),
+                         ("learned_one_hop", {"policy": pol, "local_rounds": 0xFFFFFFFF})):
+            wt_, r_ = _timed(lambda: G.sssp(src, out=out, **kw)[1], max(1, a.steps // 2), 1, torch)
             extra[f"gteps_{name}"] = round(max(1, a.steps // 2) * m_r / wt_ / 1e9, 4)
             extra[f"supersteps_{name}"] = r_[-1].supersteps
         # the measured (learned) run, host-driven so that every relaxation launch is timed

@@ -70,7 +70,11 @@ typedef struct medusa_opts {
     uint32_t use_graph;    /* nonzero: run the fixed superstep sequence as one captured hipGraph (PageRank) */
     uint32_t grid;         /* workgroups of the BFS/MBFS frontier-expansion launch; 0 = default (or tuned) */
     uint32_t far_limit;    /* SSSP: far-queue entries kept before it is rebuilt from the distances; 0 = m+n */
-    uint32_t reserved[5];
+    uint32_t local_rounds; /* SSSP: while the near queue is short, each workgroup relaxes
+                              up to this many more rounds of the vertices it lowered itself in the same
+                              superstep; 0 = default (16 when the mean out-degree is at most 4,
+                              else none), 0xFFFFFFFF = none, at most 64 */
+    uint32_t reserved[4];
 } medusa_opts;
 
 /* Per-run statistics.  Reference: printf of wall time / steps (Algorithm/main.cu:316,355,
@@ -192,7 +196,10 @@ int medusa_autotune(medusa_graph* g, uint32_t algo, const uint32_t* sources, uin
  * and per-operator timers SSSP/UserOperators.h:198-262.  Each record has
  * MEDUSA_TRACE_FIELDS doubles: step, frontier vertices, frontier out-edge sum,
  * unvisited vertices, unvisited out-edge sum, n, m, action, kernel ms (BFS/MBFS);
- * phase, far entries, mean weight, threshold, 0, n, m, width, phase ms (SSSP);
+ * phase, far entries, mean weight, threshold, vertices settled in the phase (final
+ * distance from this phase's threshold up to the next one's), n, m, width, phase ms (SSSP; without
+ * timing the device-driven loop marks each phase on the GPU clock, with timing the
+ * host-driven loop times it on the host);
  * step, dangling vertices, vertices of in-degree > 64, vertices of in-degree 1..64, 0, n, m,
  * variant, superstep ms (PageRank). */
 #define MEDUSA_TRACE_FIELDS 9

@@ -59,7 +59,8 @@ class Opts(ctypes.Structure):
         ("use_graph", ctypes.c_uint32),
         ("grid", ctypes.c_uint32),
         ("far_limit", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 5),
+        ("local_rounds", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 

@@ -210,7 +210,7 @@ struct TraceRec {
 // An instantiated hipGraph of captured supersteps and the buffers it was captured with.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    uintptr_t key[4] = {0, 0, 0, 0};
+    uintptr_t key[5] = {0, 0, 0, 0, 0};
     GraphCache() = default;
     GraphCache(const GraphCache&) = delete;
     GraphCache& operator=(const GraphCache&) = delete;
@@ -261,6 +261,7 @@ struct medusa_graph {
     medusa::DBuf<uint32_t> dist, q_small, q_large, q_aux, stamp;
     medusa::DBuf<unsigned long long> bitmap;
     medusa::DBuf<unsigned long long> near_q;  // SSSP near queues of (slice << 32 | vertex) entries
+    medusa::DBuf<unsigned long long> sssp_phases;  // SSSP device-loop phase marks (trace on)
     // per-superstep counters (device) and their pinned host mirror; each engine
     // views the 256 bytes as its own counter struct
     medusa::DBuf<unsigned long long> ctr_raw;

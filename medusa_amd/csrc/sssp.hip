@@ -19,6 +19,12 @@
 // kSlice of its out-edges, so a hub's relaxation spreads over as many workgroups as it
 // has slices.  A workgroup relaxes 256 entries at a time through a block-wide prefix scan
 // of their degrees, and stages its queue appends in LDS (one atomic per flush).
+//
+// While the near queue is short (a thin frontier), a workgroup keeps the vertices it lowered
+// below the threshold in LDS and relaxes them itself in further rounds of the same launch,
+// so the frontier advances several hops per superstep.  This stays exact: a vertex kept
+// here is relaxed after this workgroup's own atomicMin on it, and any later, lower offer
+// from another workgroup puts it in the near queue by stamp as usual.
 
 #include <algorithm>
 #include <chrono>
@@ -155,17 +161,98 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_wave
     return off + incl - x;
 }
 
+// Vertices a workgroup lowered below the threshold and relaxes again itself, in the same
+// launch, instead of handing them to the next superstep (two lists, one per round).
+constexpr uint32_t kLocal = 512;
+constexpr uint32_t kLocalQueue = 32768;  // near entries up to which the local rounds run
+constexpr uint32_t kLocalRounds = 16;     // default rounds on graphs of mean out-degree <= kLocalMeanDeg
+constexpr uint32_t kLocalMeanDeg = 4;
+
+struct LocalList {
+    uint32_t v[2][kLocal];
+    uint32_t n[2];
+};
+
+// Relaxes one tile of at most 256 entries with the whole block: thread t brings the edge
+// range [beg, beg + deg) of its entry and the entry's distance du.  The block shares the
+// out-edges through a prefix scan of the degrees and every thread binary-searches the
+// owner of its edge.  A lowered vertex below the threshold goes to `loc_out` when one is
+// given (one slice, room left), else to the near queue once per superstep (by stamp);
+// one at or above the threshold goes to the far queue.  Returns (in every thread) the
+// edges relaxed; ends with a barrier.
+__device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint32_t du,
+                                               const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
+                                               uint32_t* dist, uint32_t* __restrict__ stamp, uint32_t epoch,
+                                               uint32_t thr, Stage& st, const Sinks& k, unsigned long long& ndeg,
+                                               uint32_t* loc_out, uint32_t* loc_cnt) {
+    __shared__ uint32_t s_beg[256], s_off[256], s_du[256], s_wave[4];
+    const uint32_t t = threadIdx.x;
+    uint32_t total;
+    const uint32_t excl = block_excl_scan(deg, s_wave, total);
+    s_beg[t] = beg;
+    s_off[t] = excl;
+    s_du[t] = du;
+    __syncthreads();
+    for (uint32_t e0 = 0; e0 < total; e0 += 256) {  // uniform per block
+        const uint32_t e = e0 + t;
+        bool to_near = false, to_far = false;
+        uint32_t v = 0, vdeg = 0;
+        if (e < total) {
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = 128; step; step >>= 1)
+                if (s_off[lo + step] <= e) lo += step;
+            const uint32_t x = s_beg[lo] + (e - s_off[lo]);
+            v = col[x];
+            const unsigned long long nd64 = (unsigned long long)s_du[lo] + w[x];
+            if (nd64 < dist[v]) {
+                // dist[v] falls this superstep (to nd or below), so v needs relaxing at its
+                // new distance whoever wins: the min needs no return value, and the stamp
+                // keeps one near entry per vertex
+                const uint32_t nd = (uint32_t)nd64;
+                atomicMin(&dist[v], nd);
+                if (nd < thr) {
+                    vdeg = out_degree(k, v);
+                    // kept here: this block relaxes v after its own atomicMin, so at nd or
+                    // below; a later, lower offer from another block stamps v itself
+                    bool kept = false;
+                    if (loc_out && vdeg <= kSlice) {
+                        if (vdeg == 0) {
+                            kept = true;
+                        } else {
+                            const uint32_t pos = atomicAdd(loc_cnt, 1u);
+                            if (pos < kLocal) {
+                                loc_out[pos] = v;
+                                kept = true;
+                            }
+                        }
+                    }
+                    if (!kept) to_near = atomicMax(&stamp[v], epoch) < epoch;
+                } else {
+                    to_far = true;
+                }
+            }
+        }
+        stage_put(st, k, to_near, to_far, v, vdeg, ndeg);
+        stage_flush(st, k, false);
+    }
+    __syncthreads();  // s_beg / s_off / s_du / s_wave and the local lists are reused by the next tile
+    return total;
+}
+
 // Relaxes near entries q[0, qn) with the whole block: workgroup b takes the 256 entries of
-// chunks b, b + grid, ...; the block shares their out-edges through a prefix scan of the
-// entry degrees and every thread binary-searches the owner of its edge.  Returns (in every
-// thread) the edges relaxed by this block.
+// chunks b, b + grid, ....  With a local list and rounds > 0, the block then runs up to
+// `rounds` further tiles over the vertices it lowered itself (the last round hands its
+// lowered vertices to the near queue).  Returns (in every thread) the edges relaxed by
+// this block.
 __device__ __forceinline__ unsigned long long relax_block(const unsigned long long* __restrict__ q, uint32_t qn,
                                                           const uint32_t* __restrict__ col,
                                                           const uint32_t* __restrict__ w, uint32_t* dist,
                                                           uint32_t* __restrict__ stamp, uint32_t epoch, uint32_t thr,
-                                                          Stage& st, const Sinks& k, unsigned long long& ndeg) {
-    __shared__ uint32_t s_beg[256], s_off[256], s_du[256], s_wave[4];
+                                                          Stage& st, const Sinks& k, unsigned long long& ndeg,
+                                                          LocalList* loc = nullptr, uint32_t rounds = 0) {
     const uint32_t t = threadIdx.x;
+    if (!loc) rounds = 0;
     unsigned long long relax = 0;
     for (uint32_t chunk = blockIdx.x; (uint64_t)chunk * 256 < qn; chunk += gridDim.x) {  // uniform per block
         const uint64_t i = (uint64_t)chunk * 256 + t;
@@ -178,43 +265,29 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
             deg = min(e - beg, kSlice);
             du = dist[u];
         }
-        uint32_t total;
-        const uint32_t excl = block_excl_scan(deg, s_wave, total);
-        s_beg[t] = beg;
-        s_off[t] = excl;
-        s_du[t] = du;
+        relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg,
+                            rounds ? loc->v[0] : nullptr, rounds ? &loc->n[0] : nullptr);
+    }
+    for (uint32_t r = 0; r < rounds; ++r) {
+        // list r & 1 is complete: the last tile ended with a barrier
+        const uint32_t cur = r & 1, nxt = cur ^ 1;
+        const uint32_t cnt = min(loc->n[cur], kLocal);
+        if (cnt == 0) break;  // uniform: every thread reads the same LDS word
+        if (t == 0) loc->n[nxt] = 0;
         __syncthreads();
-        relax += total;
-        for (uint32_t e0 = 0; e0 < total; e0 += 256) {  // uniform per block
-            const uint32_t e = e0 + t;
-            bool to_near = false, to_far = false;
-            uint32_t v = 0, vdeg = 0;
-            if (e < total) {
-                uint32_t lo = 0;
-#pragma unroll
-                for (uint32_t step = 128; step; step >>= 1)
-                    if (s_off[lo + step] <= e) lo += step;
-                const uint32_t x = s_beg[lo] + (e - s_off[lo]);
-                v = col[x];
-                const unsigned long long nd64 = (unsigned long long)s_du[lo] + w[x];
-                if (nd64 < dist[v]) {
-                    // dist[v] falls this superstep (to nd or below), so v needs relaxing at its
-                    // new distance whoever wins: the min needs no return value, and the stamp
-                    // keeps one near entry per vertex
-                    const uint32_t nd = (uint32_t)nd64;
-                    atomicMin(&dist[v], nd);
-                    if (nd < thr) {
-                        vdeg = out_degree(k, v);
-                        to_near = atomicMax(&stamp[v], epoch) < epoch;
-                    } else {
-                        to_far = true;
-                    }
-                }
+        const bool more = r + 1 < rounds;
+        for (uint32_t i0 = 0; i0 < cnt; i0 += 256) {  // uniform per block
+            const uint32_t i = i0 + t;
+            uint32_t beg = 0, deg = 0, du = 0;
+            if (i < cnt) {
+                const uint32_t u = loc->v[cur][i];
+                beg = k.row_ptr[u];
+                deg = k.row_ptr[u + 1] - beg;  // at most kSlice: larger vertices are never kept
+                du = dist[u];
             }
-            stage_put(st, k, to_near, to_far, v, vdeg, ndeg);
-            stage_flush(st, k, false);
+            relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg,
+                                more ? loc->v[nxt] : nullptr, more ? &loc->n[nxt] : nullptr);
         }
-        __syncthreads();  // s_beg / s_off / s_du / s_wave are rewritten by the next chunk
     }
     return relax;
 }
@@ -264,12 +337,14 @@ __global__ __launch_bounds__(256) void k_sssp_relax(const unsigned long long* __
                                                     uint32_t* dist, uint32_t* __restrict__ stamp, uint32_t epoch,
                                                     uint32_t thr, unsigned long long* __restrict__ near_next,
                                                     uint32_t near_cap, uint32_t* __restrict__ far_q, uint32_t far_cap,
-                                                    SsspCounters* __restrict__ ctr) {
+                                                    SsspCounters* __restrict__ ctr, uint32_t rounds) {
     __shared__ Stage st;
-    stage_init(st);
+    __shared__ LocalList s_loc;
+    if (threadIdx.x == 0) s_loc.n[0] = 0;
+    stage_init(st);  // its barrier publishes s_loc.n[0]
     const Sinks k{near_next, &ctr->n_near, near_cap, far_q, &ctr->n_far, far_cap, row_ptr};
     unsigned long long ndeg = 0;
-    const unsigned long long relax = relax_block(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg);
+    const unsigned long long relax = relax_block(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg, &s_loc, rounds);
     stage_flush(st, k, true);
     ndeg = wave_sum_u64(ndeg);
     if ((threadIdx.x & (kWave - 1)) == 0 && ndeg) atomicAdd(&ctr->near_deg, ndeg);
@@ -303,7 +378,9 @@ __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__
 // Each slot of the captured loop is two kernels:
 //   work:   if a split is pending, move far entries below the new threshold to the near queue
 //           (a full rescan of the distances if the far queue overflowed); otherwise relax the
-//           current near queue;
+//           current near queue, and, while that queue is short (at most kLocalQueue entries),
+//           let each workgroup relax up to `local_cfg` more rounds of the vertices it lowered
+//           itself, so a thin frontier (a road-like graph) advances several hops per superstep;
 //   decide: one workgroup folds the counters into the control block; after a relaxation
 //           that left the near queue empty the phase ends: it picks the next bucket width
 //           (default, fixed, or the learned policy evaluated on the GPU), raises the
@@ -316,10 +393,25 @@ constexpr uint32_t kSsspGrid = 2048;
 
 struct SsspCtl {
     uint32_t done, epoch, thr, old_thr, delta, n_near, n_far, split_pending;
-    uint32_t overflow, phase, supersteps, spare, mode, wmean, dflt_delta, near_sel;
-    uint32_t far_sel, pad[3];
+    uint32_t overflow, phase, supersteps, local_rounds, mode, wmean, dflt_delta, near_sel;
+    uint32_t far_sel, local_cfg, trace, pol_far, pol_width, pad;
     unsigned long long n, m, relaxed, kernel_bytes;
 };
+
+// Phase marks of the device-driven loop when tracing: mark i (3 words) holds the features
+// the width of phase i was chosen on (phase index as the policy saw it, far-queue length,
+// threshold before the advance), the width, and the wall clock at the advance.  The mark
+// after the last phase holds only the end time.  The host turns consecutive marks into
+// phase durations, so the delta policy learns from the loop it steers.
+constexpr uint32_t kMaxPhaseMarks = 1u << 16;
+
+__device__ __forceinline__ void phase_mark(unsigned long long* marks, uint32_t i, uint32_t phase, uint32_t n_far,
+                                           uint32_t thr, uint32_t delta) {
+    if (i >= kMaxPhaseMarks) return;
+    marks[3 * i] = ((unsigned long long)n_far << 32) | phase;
+    marks[3 * i + 1] = ((unsigned long long)delta << 32) | thr;
+    marks[3 * i + 2] = (unsigned long long)wall_clock64();
+}
 
 struct SsspDevCounters {
     MDS_COUNTER(uint32_t, n_near);   // near entries written by this slot's work kernel
@@ -331,10 +423,12 @@ struct SsspDevCounters {
 
 // Bucket width at a threshold advance: the learned policy's cheapest width among
 // wmean * 2^k (k = -2..8) if it beats the default width by the policy margin, else the
-// default.  All threads of the block call it and get the same value.
-__device__ uint32_t sssp_pick_delta(const SsspCtl& c, uint32_t n_far, const float* __restrict__ policy,
-                                    MlpShared& m) {
+// default.  The choice is kept while the far queue stays within a factor 2 of its length at
+// the last evaluation, so a run of hundreds of short phases evaluates the network a few
+// dozen times.  All threads of the block call it and get the same value.
+__device__ uint32_t sssp_pick_delta(SsspCtl& c, uint32_t n_far, const float* __restrict__ policy, MlpShared& m) {
     if (c.mode != 2 || !policy) return c.dflt_delta;
+    if (c.pol_width && (unsigned long long)n_far <= 2ull * c.pol_far && 2ull * n_far >= c.pol_far) return c.pol_width;
     const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.phase, (double)n_far, (double)c.wmean, (double)c.thr, 0.0,
                                              (double)c.n, (double)c.m, 0.0, 0.0};
     // row 0: the default width; rows 1..11: k = -2..8
@@ -344,7 +438,9 @@ __device__ uint32_t sssp_pick_delta(const SsspCtl& c, uint32_t n_far, const floa
     int best = 1;
     for (int r = 2; r < 12; ++r)
         if (m.out[r] < m.out[best]) best = r;
-    return m.out[best] + kPolicyMargin < m.out[0] ? (uint32_t)width(best) : c.dflt_delta;
+    c.pol_far = n_far;
+    c.pol_width = m.out[best] + kPolicyMargin < m.out[0] ? (uint32_t)width(best) : c.dflt_delta;
+    return c.pol_width;
 }
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
@@ -361,9 +457,11 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
                                                    uint32_t* dist, uint32_t* __restrict__ stamp,
                                                    SsspDevCounters* __restrict__ ctr) {
     __shared__ Stage st;
+    __shared__ LocalList s_loc;
     const SsspCtl c = *ctl;
     // every field this kernel reads, loaded in one round trip
     keep_here(c.done);
+    keep_here(c.local_rounds);
     keep_here(c.split_pending);
     keep_here(c.n_near);
     keep_here(c.epoch);
@@ -394,16 +492,18 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
     if (N == 0) return;
     const Sinks k{c.near_sel ? near0 : near1, &ctr->n_near, near_cap, c.far_sel ? far1 : far0, &ctr->n_far,
                   far_cap, row_ptr};
-    stage_init(st);
-    const unsigned long long relax =
-        relax_block(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st, k, ndeg);
+    if (threadIdx.x == 0) s_loc.n[0] = 0;
+    stage_init(st);  // its barrier publishes s_loc.n[0]
+    const unsigned long long relax = relax_block(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st,
+                                                 k, ndeg, &s_loc, c.local_rounds);
     stage_flush(st, k, true);
     if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
 }
 
 // One workgroup after the work kernel (its counters are visible at the kernel boundary).
 __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
-                                                     uint32_t far_cap, const float* __restrict__ policy) {
+                                                     uint32_t far_cap, const float* __restrict__ policy,
+                                                     unsigned long long* __restrict__ marks) {
     __shared__ MlpShared s_mlp;
     SsspDevCounters k;
     k.n_near = load_now(&ctr->n_near);
@@ -414,6 +514,8 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
     SsspCtl c = load_now_block(ctl);
     if (c.done) return;
     uint32_t n_far;
+    // phase mark to write (trace on): index, then its fields; ~0u = none
+    uint32_t mark = 0xFFFFFFFFu, mark_phase = 0, mark_far = 0, mark_thr = 0, mark_delta = 0;
     if (c.split_pending) {
         // after a split
         c.far_sel ^= 1u;
@@ -426,6 +528,7 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
             c.split_pending = 0;
         } else if (n_far == 0 && !c.overflow) {
             c.done = 1;
+            mark = c.phase + 1;  // end time
         } else {  // empty bucket: restart from the smallest far distance
             const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
             c.old_thr = c.thr;
@@ -452,8 +555,14 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
         if (k.n_near == 0) {
             if (n_far == 0 && !c.overflow) {
                 c.done = 1;
+                mark = c.phase + 1;  // end time
             } else {
                 const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
+                mark = c.phase + 1;
+                mark_phase = c.phase;  // as sssp_pick_delta saw it
+                mark_far = n_far;
+                mark_thr = c.thr;
+                mark_delta = d;
                 c.old_thr = c.thr;
                 c.delta = d;
                 c.thr = sat_add(c.thr, d);
@@ -462,8 +571,10 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
             }
         }
     }
+    c.local_rounds = c.n_near <= kLocalQueue ? c.local_cfg : 0u;
     __syncthreads();  // every wave has read the control block and counters
     if (threadIdx.x == 0) {
+        if (c.trace && mark != 0xFFFFFFFFu) phase_mark(marks, mark, mark_phase, mark_far, mark_thr, mark_delta);
         *ctl = c;
         ctr->n_near = 0;
         ctr->n_far = n_far;
@@ -474,8 +585,14 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
 }
 
 __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, unsigned long long m, uint32_t thr,
-                                uint32_t mode, uint32_t wmean, uint32_t dflt, uint32_t src_slices) {
+                                uint32_t mode, uint32_t wmean, uint32_t dflt, uint32_t src_slices,
+                                uint32_t local_cfg, uint32_t trace, uint32_t delta0, unsigned long long* marks) {
     SsspCtl z{};
+    z.trace = trace;
+    // phase 0: one source vertex, threshold 0 before the first advance
+    if (trace) phase_mark(marks, 0, 0, 1, 0, delta0);
+    z.local_cfg = local_cfg;
+    z.local_rounds = src_slices <= kLocalQueue ? local_cfg : 0u;
     z.epoch = 1;
     z.thr = thr;
     z.delta = thr;
@@ -491,6 +608,16 @@ __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, un
     c->far_out = 0;
     c->far_min = 0xFFFFFFFFu;
     c->relax = 0;
+}
+
+// Local rounds per superstep (while the near queue is short): the caller's count, or by
+// default kLocalRounds on sparse graphs (mean out-degree <= kLocalMeanDeg, road-like: thin
+// frontiers, thousands of supersteps) and none on denser ones, where the extra rounds cost
+// more than the supersteps they save.
+static uint32_t sssp_local_cfg(const medusa_graph* g, const medusa_opts& o) {
+    if (o.local_rounds == 0xFFFFFFFFu) return 0;
+    if (o.local_rounds) return o.local_rounds;
+    return g->m <= (uint64_t)kLocalMeanDeg * g->n ? kLocalRounds : 0u;
 }
 
 // Far-queue capacity: m + n entries, or the caller's smaller limit (a full queue is rebuilt
@@ -509,6 +636,51 @@ static void sssp_widths(const medusa_graph* g, uint32_t& wmean, uint32_t& dflt) 
     for (uint64_t e = 0; e < m; e += wstride, ++wcnt) wsum += g->h_w[e];
     wmean = (uint32_t)std::max<uint64_t>(1, wcnt ? wsum / wcnt : 1);
     dflt = (uint32_t)std::min<uint64_t>(0xFFFFFFFEull, 2ull * wmean);
+}
+
+// Field 4 of every phase record: the vertices the phase settled, i.e. those whose final
+// distance lies between the phase's starting threshold and the next phase's (the last phase
+// takes the rest).  Time per settled vertex is what a bucket width should minimise: a narrow
+// width pays the per-phase cost over few vertices, a wide one relaxes vertices many times.
+static void sssp_trace_settled(medusa_graph* g) {
+    if (g->trace.empty()) return;
+    std::vector<uint32_t> d(g->n);
+    MDS_HIP(hipMemcpyAsync(d.data(), g->dist.p, (size_t)g->n * 4, hipMemcpyDeviceToHost, g->stream));
+    MDS_HIP(hipStreamSynchronize(g->stream));
+    std::vector<double> lo(g->trace.size());
+    for (size_t i = 0; i < lo.size(); ++i) lo[i] = g->trace[i].f[3];
+    std::vector<uint64_t> cnt(lo.size(), 0);
+    for (uint32_t x : d) {
+        if (x == 0xFFFFFFFFu) continue;
+        const size_t k = (size_t)(std::upper_bound(lo.begin(), lo.end(), (double)x) - lo.begin());
+        if (k) ++cnt[k - 1];
+    }
+    for (size_t i = 0; i < lo.size(); ++i) g->trace[i].f[4] = (double)cnt[i];
+}
+
+// Trace records of a device-driven run from its phase marks: one per phase, with the
+// features and width of the mark and the time to the next mark.
+static void sssp_device_trace(medusa_graph* g, uint32_t phases, uint32_t wmean) {
+    const uint32_t nmarks = std::min<uint32_t>(phases + 2, kMaxPhaseMarks);
+    std::vector<unsigned long long> h(3ull * nmarks);
+    MDS_HIP(hipMemcpyAsync(h.data(), g->sssp_phases.p, h.size() * 8, hipMemcpyDeviceToHost, g->stream));
+    MDS_HIP(hipStreamSynchronize(g->stream));
+    int khz = 0;
+    MDS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device));
+    MDS_CHECK(khz > 0, MEDUSA_ERR_HIP, "sssp: no wall clock rate");
+    g->trace.clear();
+    for (uint32_t i = 0; i + 1 < nmarks; ++i) {
+        Frontier f{};
+        f.step = (uint32_t)h[3 * i];
+        f.n_f = (uint32_t)(h[3 * i] >> 32);
+        f.n = g->n;
+        f.m = g->m;
+        const uint32_t thr = (uint32_t)h[3 * i + 1], delta = (uint32_t)(h[3 * i + 1] >> 32);
+        const double ms = (double)(h[3 * (i + 1) + 2] - h[3 * i + 2]) / (double)khz;
+        TraceRec r;
+        Sched::to_sssp_record(f, wmean, thr, (double)delta, ms, r.f);
+        g->trace.push_back(r);
+    }
 }
 
 static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const medusa_opts& o, medusa_stats& st) {
@@ -541,31 +713,33 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     const uint32_t src_slices = slices_of(g->h_row_ptr[src + 1] - g->h_row_ptr[src]);
 
     GraphCache& gc = g->sssp_graph;
-    const uintptr_t key[4] = {(uintptr_t)g->dist.p ^ (uintptr_t)g->near_q.p, (uintptr_t)policy, (uintptr_t)g->q_aux.p,
-                              (uintptr_t)g->w.p};
-    if (!gc.exec || !std::equal(key, key + 4, gc.key)) {
+    unsigned long long* marks = g->sssp_phases.p;
+    const uintptr_t key[5] = {(uintptr_t)g->dist.p ^ (uintptr_t)g->near_q.p, (uintptr_t)policy, (uintptr_t)g->q_aux.p,
+                              (uintptr_t)g->w.p, (uintptr_t)marks};
+    if (!gc.exec || !std::equal(key, key + 5, gc.key)) {
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (uint32_t k = 0; k < kSsspBatch; ++k) {
             hipLaunchKernelGGL(k_sssp_work, dim3(kSsspGrid), dim3(256), 0, s, (const SsspCtl*)ctl, near0, near1,
                                (uint32_t)near_cap, far0, far1, (uint32_t)far_cap, g->row_ptr.p, g->col.p, g->w.p,
                                g->dist.p, g->stamp.p, ctr);
-            hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap, policy);
+            hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap, policy, marks);
         }
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
         if (graph) (void)hipGraphDestroy(graph);
         MDS_HIP(e);
-        std::copy(key, key + 4, gc.key);
+        std::copy(key, key + 5, gc.key);
     }
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_sssp_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->dist.p, g->stamp.p, src,
                        src_slices, near0);
     MDS_LAUNCHED();
+    const uint32_t local_cfg = sssp_local_cfg(g, o);
     hipLaunchKernelGGL(k_sssp_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, thr0, mode, wmean,
-                       width, src_slices);
+                       width, src_slices, local_cfg, o.trace ? 1u : 0u, d0, marks);
     MDS_LAUNCHED();
     SsspCtl h{};
     // every replay either relaxes a non-empty queue or raises the threshold
@@ -589,6 +763,7 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     st.alg_bytes = h.kernel_bytes + 8ull * n;
     st.reserved[0] = (uint32_t)std::min<uint64_t>(h.relaxed, 0xFFFFFFFFull);
     st.reserved[1] = h.phase;
+    if (o.trace) sssp_device_trace(g, h.phase, wmean);
     copy_out(g, out, g->dist.p, (size_t)n * 4);
 }
 
@@ -626,6 +801,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     uint64_t relaxed = 0;
     const bool timing = o.timing != 0;
     if (timing) ensure_events(g, 2);
+    const uint32_t local_cfg = sssp_local_cfg(g, o);
     Frontier f{};
     f.n = n;
     f.m = m;
@@ -693,7 +869,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         // 256 entries per workgroup chunk; the far queue keeps growing at far_a[n_far..)
         hipLaunchKernelGGL(k_sssp_relax, dim3(grid_for(n_near, 256, kSsspGrid)), dim3(256), 0, s, near_cur, n_near,
                            g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, epoch, thr, near_nxt,
-                           (uint32_t)near_cap, far_a, (uint32_t)far_cap, C);
+                           (uint32_t)near_cap, far_a, (uint32_t)far_cap, C, n_near <= kLocalQueue ? local_cfg : 0u);
         MDS_LAUNCHED();
         if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
         const SsspCounters c = read_counters<SsspCounters>(g);
@@ -774,15 +950,21 @@ extern "C" int medusa_sssp(medusa_graph* g, uint32_t source, uint32_t* dist, con
         MDS_CHECK(g->weighted, MEDUSA_ERR_STATE, "sssp: graph has no weights");
         medusa_opts o{};
         if (opts) o = *opts;
+        MDS_CHECK(o.local_rounds <= 64 || o.local_rounds == 0xFFFFFFFFu, MEDUSA_ERR_ARG,
+                  "sssp: local_rounds %u above 64", o.local_rounds);
         const uint32_t n = g->n;
         const size_t far_cap = sssp_far_alloc(g);
         g->dist.ensure(n);
         g->stamp.ensure(n);
         g->near_q.ensure(2 * near_capacity(n, g->m));  // near queues A and B ((vertex, slice) entries)
+        g->sssp_phases.ensure(3ull * kMaxPhaseMarks);     // phase marks of a traced device-driven run
         g->q_aux.ensure(2 * far_cap);                   // far queues A and B
         medusa_stats st{};
-        if (o.timing || o.trace) sssp_run_host(g, source, dist, o, st);
+        // timing: host-driven, every relaxation launch timed; otherwise device-driven (traced
+        // by phase marks when trace is on)
+        if (o.timing) sssp_run_host(g, source, dist, o, st);
         else sssp_run_device(g, source, dist, o, st);
+        if (o.trace) sssp_trace_settled(g);
         unsigned long long* red = reinterpret_cast<unsigned long long*>(g->sssp_ctl.p) + 30;
         if (!g->sssp_ctl.p) {
             g->sssp_ctl.ensure(1);

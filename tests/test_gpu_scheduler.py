@@ -123,10 +123,10 @@ def _delta_samples(G, src, deltas):
         G.sssp(src, delta=d, trace=1)
         tr = G.trace()
         for r in tr:
-            if r[7] <= 0 or r[8] <= 0:
+            if r[4] <= 0 or r[7] <= 0 or r[8] <= 0:
                 continue
             Xs.append(md.policy_features(md.POLICY_DELTA, r, r[7]))
-            ys.append(np.log2(r[8] * 1e3 / r[7] + 1e-3))  # time per unit of threshold advanced
+            ys.append(np.log2(r[8] * 1e6 / r[4]))  # time per settled vertex, in log2 ns
     return Xs, ys
 
 
@@ -146,12 +146,34 @@ def test_learned_delta_sssp(oracle):
     with md.Graph(rp, col, w) as G:
         d_plain, st_plain = G.sssp(src)
         d_learn, st_learn = G.sssp(src, policy=pol)
-        d_host, _ = G.sssp(src, policy=pol, trace=1)
+        d_host, _ = G.sssp(src, policy=pol, timing=1)
         assert np.array_equal(d_plain, want)
         assert np.array_equal(d_learn, want)
         assert np.array_equal(d_host, want)
         print(f"[sssp 700x700] plain: {st_plain.supersteps} supersteps {st_plain.device_ms:.2f} ms; "
               f"learned: {st_learn.supersteps} supersteps {st_learn.device_ms:.2f} ms", flush=True)
+
+
+def test_sssp_device_trace_marks_every_phase(oracle):
+    """A traced SSSP run without timing stays device-driven and reads one record per phase
+    back from the GPU's phase marks: phase 0 plus one per threshold advance, the width used,
+    rising thresholds, durations that fit inside the run, and settled counts that cover
+    every reached vertex once."""
+    rp, col, w = wl.lattice_graph(200, 200, seed=31)
+    src = wl.lattice_source(200, 200)
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        for lr in (0, 0xFFFFFFFF):
+            d, st = G.sssp(src, delta=500, trace=1, local_rounds=lr)
+            tr = G.trace()
+            assert np.array_equal(d, want)
+            assert len(tr) == st.reserved[1] + 1
+            assert (tr[:, 7] == 500).all()
+            assert tr[0, 3] == 0 and (np.diff(tr[:, 3]) > 0).all()
+            assert (tr[:, 8] > 0).all()
+            assert tr[:, 8].sum() <= st.device_ms * 1.05 + 0.05
+            assert (tr[:, 5] == len(rp) - 1).all() and (tr[:, 6] == len(col)).all()
+            assert tr[:, 4].sum() == st.reached  # every reached vertex settles in exactly one phase
 
 
 def test_device_and_host_drivers_agree(oracle):

@@ -246,6 +246,47 @@ def test_sssp_zero_weights_duplicates_and_self_loops(oracle):
                 assert np.array_equal(d, want), (delta, timing)
 
 
+@pytest.mark.parametrize("case", ["lattice", "random", "zero_weights", "hubs"])
+def test_sssp_local_rounds(oracle, case):
+    """Workgroups relaxing the vertices they lowered themselves for up to 1, 3 or 64 further
+    rounds, or none, in both loop drivers: distances stay Dijkstra's.  The cases cover a
+    thin frontier (lattice), a dense random graph, zero weights with duplicate arcs and self
+    loops (offers that tie), and hubs wider than one near-queue slice (never kept locally)."""
+    rng = np.random.default_rng(21)
+    if case == "lattice":
+        rp, col, w = wl.lattice_graph(120, 150, seed=9)
+        src = wl.lattice_source(120, 150)
+    elif case == "random":
+        rp, col, _ = random_graph(6000, 60000, 21)
+        w = rng.integers(1, 1001, len(col)).astype(np.uint32)
+        src = 3
+    elif case == "zero_weights":
+        n, m = 3000, 9000
+        e = rng.integers(0, n, (m, 2))
+        e[:300, 0] = e[:300, 1]
+        e[300:900] = e[900:1500]
+        w = rng.integers(0, 20, m).astype(np.uint32)
+        w[::3] = 0
+        rp, col, w = csr_from_edges(n, e, w)
+        src = 5
+    else:
+        n = 20000
+        hub = np.stack([np.zeros(3000, np.int64), rng.integers(1, n, 3000)], 1)
+        rest = np.stack([rng.integers(0, n, 40000), rng.integers(0, n, 40000)], 1)
+        rp, col, _ = csr_from_edges(n, np.concatenate([hub, rest]))
+        w = rng.integers(1, 1001, len(col)).astype(np.uint32)
+        src = 0
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        for lr in (1, 3, 64, U32MAX, 0):
+            for delta in ((0, 300, U32MAX) if case == "lattice" else (0, 1, 300, U32MAX)):
+                for timing in (0, 1):
+                    d, _ = G.sssp(src, delta=delta, timing=timing, local_rounds=lr)
+                    assert np.array_equal(d, want), (lr, delta, timing)
+        with pytest.raises(md.MedusaError):
+            G.sssp(src, local_rounds=65)
+
+
 def test_sssp_hub_source_slices(oracle):
     """A source whose out-degree spans many near-queue slices, plus hubs inside the graph."""
     rng = np.random.default_rng(12)
