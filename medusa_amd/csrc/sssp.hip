@@ -162,29 +162,33 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_wave
 }
 
 // Vertices a workgroup lowered below the threshold and relaxes again itself, in the same
-// launch, instead of handing them to the next superstep (two lists, one per round).
+// launch, instead of handing them to the next superstep (two lists, one per round).  An
+// entry carries what the next round needs (edge range and the distance this workgroup
+// wrote), so a round starts without a global read.
 constexpr uint32_t kLocal = 512;
 constexpr uint32_t kLocalQueue = 32768;  // near entries up to which the local rounds run
 constexpr uint32_t kLocalRounds = 16;     // default rounds on graphs of mean out-degree <= kLocalMeanDeg
 constexpr uint32_t kLocalMeanDeg = 4;
 
 struct LocalList {
-    uint32_t v[2][kLocal];
+    uint32_t beg[2][kLocal];  // row_ptr[v]
+    uint32_t du[2][kLocal];   // the distance this workgroup offered v
+    uint8_t deg[2][kLocal];   // out-degree, at most kSlice
     uint32_t n[2];
 };
 
 // Relaxes one tile of at most 256 entries with the whole block: thread t brings the edge
 // range [beg, beg + deg) of its entry and the entry's distance du.  The block shares the
 // out-edges through a prefix scan of the degrees and every thread binary-searches the
-// owner of its edge.  A lowered vertex below the threshold goes to `loc_out` when one is
-// given (one slice, room left), else to the near queue once per superstep (by stamp);
-// one at or above the threshold goes to the far queue.  Returns (in every thread) the
-// edges relaxed; ends with a barrier.
+// owner of its edge.  A lowered vertex below the threshold goes to side `side` of the local
+// list when one is given (side < 2, one slice, room left), else to the near queue once per
+// superstep (by stamp); one at or above the threshold goes to the far queue.  Returns (in
+// every thread) the edges relaxed; ends with a barrier.
 __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint32_t du,
                                                const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
                                                uint32_t* dist, uint32_t* __restrict__ stamp, uint32_t epoch,
                                                uint32_t thr, Stage& st, const Sinks& k, unsigned long long& ndeg,
-                                               uint32_t* loc_out, uint32_t* loc_cnt) {
+                                               LocalList* loc, uint32_t side) {
     __shared__ uint32_t s_beg[256], s_off[256], s_du[256], s_wave[4];
     const uint32_t t = threadIdx.x;
     uint32_t total;
@@ -212,17 +216,21 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
                 const uint32_t nd = (uint32_t)nd64;
                 atomicMin(&dist[v], nd);
                 if (nd < thr) {
-                    vdeg = out_degree(k, v);
-                    // kept here: this block relaxes v after its own atomicMin, so at nd or
-                    // below; a later, lower offer from another block stamps v itself
+                    const uint32_t vb = k.row_ptr[v];
+                    vdeg = k.row_ptr[v + 1] - vb;
+                    // kept here: this block relaxes v at nd, the value it wrote; if the
+                    // atomicMin found a lower one, that one's writer relaxes v, and a later,
+                    // lower offer from another block stamps v itself
                     bool kept = false;
-                    if (loc_out && vdeg <= kSlice) {
+                    if (loc && side < 2 && vdeg <= kSlice) {
                         if (vdeg == 0) {
                             kept = true;
                         } else {
-                            const uint32_t pos = atomicAdd(loc_cnt, 1u);
+                            const uint32_t pos = atomicAdd(&loc->n[side], 1u);
                             if (pos < kLocal) {
-                                loc_out[pos] = v;
+                                loc->beg[side][pos] = vb;
+                                loc->du[side][pos] = nd;
+                                loc->deg[side][pos] = (uint8_t)vdeg;
                                 kept = true;
                             }
                         }
@@ -265,8 +273,7 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
             deg = min(e - beg, kSlice);
             du = dist[u];
         }
-        relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg,
-                            rounds ? loc->v[0] : nullptr, rounds ? &loc->n[0] : nullptr);
+        relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, rounds ? 0u : 2u);
     }
     for (uint32_t r = 0; r < rounds; ++r) {
         // list r & 1 is complete: the last tile ended with a barrier
@@ -280,13 +287,11 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
             const uint32_t i = i0 + t;
             uint32_t beg = 0, deg = 0, du = 0;
             if (i < cnt) {
-                const uint32_t u = loc->v[cur][i];
-                beg = k.row_ptr[u];
-                deg = k.row_ptr[u + 1] - beg;  // at most kSlice: larger vertices are never kept
-                du = dist[u];
+                beg = loc->beg[cur][i];
+                deg = loc->deg[cur][i];
+                du = loc->du[cur][i];
             }
-            relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg,
-                                more ? loc->v[nxt] : nullptr, more ? &loc->n[nxt] : nullptr);
+            relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, more ? nxt : 2u);
         }
     }
     return relax;
@@ -330,7 +335,9 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
     return x;
 }
 
-// One relaxation superstep over the near queue (host-driven loop).
+// One relaxation superstep over the near queue (host-driven loop); kLoc: with the local list
+// (its LDS is allocated only in that variant).
+template <bool kLoc>
 __global__ __launch_bounds__(256) void k_sssp_relax(const unsigned long long* __restrict__ q, uint32_t qn,
                                                     const uint32_t* __restrict__ row_ptr,
                                                     const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
@@ -339,12 +346,17 @@ __global__ __launch_bounds__(256) void k_sssp_relax(const unsigned long long* __
                                                     uint32_t near_cap, uint32_t* __restrict__ far_q, uint32_t far_cap,
                                                     SsspCounters* __restrict__ ctr, uint32_t rounds) {
     __shared__ Stage st;
-    __shared__ LocalList s_loc;
-    if (threadIdx.x == 0) s_loc.n[0] = 0;
+    LocalList* loc = nullptr;
+    if constexpr (kLoc) {
+        __shared__ LocalList s_loc;
+        if (threadIdx.x == 0) s_loc.n[0] = 0;
+        loc = &s_loc;
+    }
     stage_init(st);  // its barrier publishes s_loc.n[0]
     const Sinks k{near_next, &ctr->n_near, near_cap, far_q, &ctr->n_far, far_cap, row_ptr};
     unsigned long long ndeg = 0;
-    const unsigned long long relax = relax_block(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg, &s_loc, rounds);
+    const unsigned long long relax =
+        relax_block(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, kLoc ? rounds : 0u);
     stage_flush(st, k, true);
     ndeg = wave_sum_u64(ndeg);
     if ((threadIdx.x & (kWave - 1)) == 0 && ndeg) atomicAdd(&ctr->near_deg, ndeg);
@@ -448,6 +460,7 @@ __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     return s >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
 }
 
+template <bool kLoc>
 __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ ctl,
                                                    unsigned long long* __restrict__ near0,
                                                    unsigned long long* __restrict__ near1, uint32_t near_cap,
@@ -457,7 +470,11 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
                                                    uint32_t* dist, uint32_t* __restrict__ stamp,
                                                    SsspDevCounters* __restrict__ ctr) {
     __shared__ Stage st;
-    __shared__ LocalList s_loc;
+    LocalList* loc = nullptr;
+    if constexpr (kLoc) {
+        __shared__ LocalList s_loc;
+        loc = &s_loc;
+    }
     const SsspCtl c = *ctl;
     // every field this kernel reads, loaded in one round trip
     keep_here(c.done);
@@ -492,10 +509,10 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
     if (N == 0) return;
     const Sinks k{c.near_sel ? near0 : near1, &ctr->n_near, near_cap, c.far_sel ? far1 : far0, &ctr->n_far,
                   far_cap, row_ptr};
-    if (threadIdx.x == 0) s_loc.n[0] = 0;
-    stage_init(st);  // its barrier publishes s_loc.n[0]
+    if (loc && threadIdx.x == 0) loc->n[0] = 0;
+    stage_init(st);  // its barrier publishes loc->n[0]
     const unsigned long long relax = relax_block(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st,
-                                                 k, ndeg, &s_loc, c.local_rounds);
+                                                 k, ndeg, loc, kLoc ? c.local_rounds : 0u);
     stage_flush(st, k, true);
     if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
 }
@@ -717,15 +734,17 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
 
     GraphCache& gc = g->sssp_graph;
     unsigned long long* marks = g->sssp_phases.p;
+    const uint32_t local_cfg = sssp_local_cfg(g, o);
+    // the marks buffer is 256-byte aligned, so its low bit can carry the kernel variant
     const uintptr_t key[5] = {(uintptr_t)g->dist.p ^ (uintptr_t)g->near_q.p, (uintptr_t)policy, (uintptr_t)g->q_aux.p,
-                              (uintptr_t)g->w.p, (uintptr_t)marks};
+                              (uintptr_t)g->w.p, (uintptr_t)marks | (local_cfg ? 1u : 0u)};
     if (!gc.exec || !std::equal(key, key + 5, gc.key)) {
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (uint32_t k = 0; k < kSsspBatch; ++k) {
-            hipLaunchKernelGGL(k_sssp_work, dim3(kSsspGrid), dim3(256), 0, s, (const SsspCtl*)ctl, near0, near1,
-                               (uint32_t)near_cap, far0, far1, (uint32_t)far_cap, g->row_ptr.p, g->col.p, g->w.p,
-                               g->dist.p, g->stamp.p, ctr);
+            hipLaunchKernelGGL(local_cfg ? k_sssp_work<true> : k_sssp_work<false>, dim3(kSsspGrid), dim3(256), 0, s,
+                               (const SsspCtl*)ctl, near0, near1, (uint32_t)near_cap, far0, far1, (uint32_t)far_cap,
+                               g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, ctr);
             hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap, policy, marks);
         }
         hipGraph_t graph = nullptr;
@@ -740,7 +759,6 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     hipLaunchKernelGGL(k_sssp_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->dist.p, g->stamp.p, src,
                        src_slices, near0);
     MDS_LAUNCHED();
-    const uint32_t local_cfg = sssp_local_cfg(g, o);
     hipLaunchKernelGGL(k_sssp_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, thr0, mode, wmean,
                        width, src_slices, local_cfg, o.trace ? 1u : 0u, d0, marks);
     MDS_LAUNCHED();
@@ -866,9 +884,10 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         MDS_LAUNCHED();
         if (timing) MDS_HIP(hipEventRecord(g->events[0], s));  // brackets the relaxation launch alone
         // 256 entries per workgroup chunk; the far queue keeps growing at far_a[n_far..)
-        hipLaunchKernelGGL(k_sssp_relax, dim3(grid_for(n_near, 256, kSsspGrid)), dim3(256), 0, s, near_cur, n_near,
-                           g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, epoch, thr, near_nxt,
-                           (uint32_t)near_cap, far_a, (uint32_t)far_cap, C, n_near <= kLocalQueue ? local_cfg : 0u);
+        const uint32_t rounds = n_near <= kLocalQueue ? local_cfg : 0u;
+        hipLaunchKernelGGL(rounds ? k_sssp_relax<true> : k_sssp_relax<false>, dim3(grid_for(n_near, 256, kSsspGrid)),
+                           dim3(256), 0, s, near_cur, n_near, g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p,
+                           epoch, thr, near_nxt, (uint32_t)near_cap, far_a, (uint32_t)far_cap, C, rounds);
         MDS_LAUNCHED();
         if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
         const SsspCounters c = read_counters<SsspCounters>(g);
