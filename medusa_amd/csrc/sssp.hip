@@ -183,7 +183,11 @@ struct LocalList {
 // owner of its edge.  A lowered vertex below the threshold goes to side `side` of the local
 // list when one is given (side < 2, one slice, room left), else to the near queue once per
 // superstep (by stamp); one at or above the threshold goes to the far queue.  Returns (in
-// every thread) the edges relaxed; ends with a barrier.
+// every thread) the edges relaxed; ends with a barrier.  kSpec: read the target's row
+// offsets together with its distance whenever the offer is below the threshold, so a
+// lowered vertex costs one round trip fewer (worth it on sparse graphs, where rounds are
+// latency-bound; on dense ones most offers lose and the extra reads cost bandwidth).
+template <bool kSpec>
 __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint32_t du,
                                                const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
                                                uint32_t* dist, uint32_t* __restrict__ stamp, uint32_t epoch,
@@ -209,6 +213,13 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
             const uint32_t x = s_beg[lo] + (e - s_off[lo]);
             v = col[x];
             const unsigned long long nd64 = (unsigned long long)s_du[lo] + w[x];
+            uint32_t vb = 0, ve = 0;
+            if constexpr (kSpec) {
+                if (nd64 < thr) {
+                    vb = k.row_ptr[v];
+                    ve = k.row_ptr[v + 1];
+                }
+            }
             if (nd64 < dist[v]) {
                 // dist[v] falls this superstep (to nd or below), so v needs relaxing at its
                 // new distance whoever wins: the min needs no return value, and the stamp
@@ -216,8 +227,11 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
                 const uint32_t nd = (uint32_t)nd64;
                 atomicMin(&dist[v], nd);
                 if (nd < thr) {
-                    const uint32_t vb = k.row_ptr[v];
-                    vdeg = k.row_ptr[v + 1] - vb;
+                    if constexpr (!kSpec) {
+                        vb = k.row_ptr[v];
+                        ve = k.row_ptr[v + 1];
+                    }
+                    vdeg = ve - vb;
                     // kept here: this block relaxes v at nd, the value it wrote; if the
                     // atomicMin found a lower one, that one's writer relaxes v, and a later,
                     // lower offer from another block stamps v itself
@@ -252,7 +266,8 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
 // chunks b, b + grid, ....  With a local list and rounds > 0, the block then runs up to
 // `rounds` further tiles over the vertices it lowered itself (the last round hands its
 // lowered vertices to the near queue).  Returns (in every thread) the edges relaxed by
-// this block.
+// this block.  kSpec as for relax_tile.
+template <bool kSpec>
 __device__ __forceinline__ unsigned long long relax_block(const unsigned long long* __restrict__ q, uint32_t qn,
                                                           const uint32_t* __restrict__ col,
                                                           const uint32_t* __restrict__ w, uint32_t* dist,
@@ -273,7 +288,7 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
             deg = min(e - beg, kSlice);
             du = dist[u];
         }
-        relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, rounds ? 0u : 2u);
+        relax += relax_tile<kSpec>(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, rounds ? 0u : 2u);
     }
     for (uint32_t r = 0; r < rounds; ++r) {
         // list r & 1 is complete: the last tile ended with a barrier
@@ -291,7 +306,8 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
                 deg = loc->deg[cur][i];
                 du = loc->du[cur][i];
             }
-            relax += relax_tile(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, more ? nxt : 2u);
+            relax += relax_tile<kSpec>(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc,
+                                       more ? nxt : 2u);
         }
     }
     return relax;
@@ -356,7 +372,7 @@ __global__ __launch_bounds__(256) void k_sssp_relax(const unsigned long long* __
     const Sinks k{near_next, &ctr->n_near, near_cap, far_q, &ctr->n_far, far_cap, row_ptr};
     unsigned long long ndeg = 0;
     const unsigned long long relax =
-        relax_block(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, kLoc ? rounds : 0u);
+        relax_block<kLoc>(q, qn, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, kLoc ? rounds : 0u);
     stage_flush(st, k, true);
     ndeg = wave_sum_u64(ndeg);
     if ((threadIdx.x & (kWave - 1)) == 0 && ndeg) atomicAdd(&ctr->near_deg, ndeg);
@@ -511,8 +527,8 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
                   far_cap, row_ptr};
     if (loc && threadIdx.x == 0) loc->n[0] = 0;
     stage_init(st);  // its barrier publishes loc->n[0]
-    const unsigned long long relax = relax_block(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch, thr, st,
-                                                 k, ndeg, loc, kLoc ? c.local_rounds : 0u);
+    const unsigned long long relax = relax_block<kLoc>(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch,
+                                                       thr, st, k, ndeg, loc, kLoc ? c.local_rounds : 0u);
     stage_flush(st, k, true);
     if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
 }
