@@ -662,8 +662,8 @@ static size_t sssp_far_cap(const medusa_graph* g, const medusa_opts& o) {
 }
 
 // Mean weight (sampled) and the default bucket width, a small multiple of it: 3x with local
-// rounds, which absorb the extra hops of a wider bucket (1400x1400 lattice: 17.5 ms at 3x,
-// 19.9 ms at 2x), else 2x.
+// rounds, which absorb the extra hops of a wider bucket (1400x1400 lattice: 16.9 ms at 3x,
+// 19.3 ms at 2x), else 2x.
 static void sssp_widths(const medusa_graph* g, const medusa_opts& o, uint32_t& wmean, uint32_t& dflt) {
     uint64_t wsum = 0, wcnt = 0;
     const uint64_t m = g->m;
