@@ -23,8 +23,9 @@
 // While the near queue is short (a thin frontier), a workgroup keeps the vertices it lowered
 // below the threshold in LDS and relaxes them itself in further rounds of the same launch,
 // so the frontier advances several hops per superstep.  This stays exact: a vertex kept
-// here is relaxed after this workgroup's own atomicMin on it, and any later, lower offer
-// from another workgroup puts it in the near queue by stamp as usual.
+// here is relaxed at the distance this workgroup wrote into it (if its atomicMin found a
+// lower one, that value's writer relaxes the vertex), and any later, lower offer from
+// another workgroup puts it in the near queue by stamp as usual.
 
 #include <algorithm>
 #include <chrono>
