@@ -266,11 +266,12 @@ def _train_mbfs_policy(md, wl, np):
 
 
 def _train_delta_policy(md, wl, np, family):
-    """Delta-width cost model trained on the GPU from traced SSSP runs of smaller graphs of the
-    same family (lattices, or RMAT scale 14-16 for RMAT), over a sweep of fixed widths."""
+    """Delta-width cost model trained on the GPU from traced SSSP runs of other graphs of the
+    same family (two lattices of other seeds, one of them the measured size, or RMAT scale 14-16
+    for RMAT), over a sweep of fixed widths."""
     if family == "sssp_lattice":
         graphs = [wl.lattice_graph(r, c, seed=sd) + (wl.lattice_source(r, c),)
-                  for r, c, sd in ((200, 200, 31), (400, 300, 32), (500, 500, 33))]
+                  for r, c, sd in ((1000, 1000, 32), (1400, 1400, 31))]
     else:
         graphs = []
         for sc in (14, 15, 16):
@@ -279,7 +280,7 @@ def _train_delta_policy(md, wl, np, family):
     Xs, ys = [], []
     for rp, col, wt, src in graphs:
         with md.Graph(rp, col, wt) as H:
-            for d in (250, 500, 1000, 2000, 4000, 8000, 16000):
+            for d in (250, 500, 1000, 2000, 4000, 8000, 16000, 32000):
                 H.sssp(src, delta=d, trace=1)
                 for r in H.trace():
                     if r[4] > 0 and r[7] > 0 and r[8] > 0:  # time per settled vertex, in log2 ns
@@ -372,7 +373,7 @@ def run_workload(a, md, wl, torch, np):
         wp, rp_ = _timed(lambda: G.sssp(src, out=out)[1], a.steps, 1, torch)
         extra = {"supersteps": res[-1].supersteps, "phases": res[-1].reserved[1], "edges_reached": m_r,
                  "policy": "delta width per threshold advance by the MLP, trained on the GPU from traced runs of "
-                           "smaller graphs of the same family (not this graph)",
+                           "other graphs of the same family (other seeds; not this graph)",
                  "gteps_plain": round(a.steps * m_r / wp / 1e9, 4), "supersteps_plain": rp_[-1].supersteps,
                  "learned_over_plain": round(wp / wall, 3),
                  "distances_identical_to_plain": bool(np.array_equal(learned, out.cpu().numpy()))}

@@ -72,7 +72,7 @@ typedef struct medusa_opts {
     uint32_t far_limit;    /* SSSP: far-queue entries kept before it is rebuilt from the distances; 0 = m+n */
     uint32_t local_rounds; /* SSSP: while the near queue is short, each workgroup relaxes
                               up to this many more rounds of the vertices it lowered itself in the same
-                              superstep; 0 = default (16 when the mean out-degree is at most 4,
+                              superstep; 0 = default (32 when the mean out-degree is at most 4,
                               else none), 0xFFFFFFFF = none, at most 64 */
     uint32_t reserved[4];
 } medusa_opts;

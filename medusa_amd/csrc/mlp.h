@@ -17,9 +17,11 @@ constexpr float kMlpSlope = 1.0f / 64.0f;
 // be cheaper by more than a margin (log2 units of cost); otherwise the plain engine's choice
 // stands, so an uncertain model falls back to it.  Direction: 1.0 (2x), because the tuned
 // direction heuristics are close to the best choice per superstep on the graphs measured;
-// bucket width and kernel variant: 0.25 (about 19 %).
+// bucket width: 1.0, because the width model, trained on smaller graphs, errs by more than
+// 19 % on large lattices; kernel variant: 0.25 (about 19 %).
 constexpr float kPolicyMargin = 0.25f;
 constexpr float kDirectionMargin = 1.0f;
+constexpr float kWidthMargin = 1.0f;
 
 // Direction from the two predicted costs and the plain engine's choice.
 inline __host__ __device__ bool policy_pull(float c_push, float c_pull, bool plain_pull) {

@@ -198,7 +198,7 @@ struct Sched {
                 pick = (uint32_t)d;
             }
         }
-        return best + kPolicyMargin < c_dflt ? pick : dflt;
+        return best + kWidthMargin < c_dflt ? pick : dflt;
     }
     // PageRank kernel variant for one superstep (record as in variant_row): the policy's
     // cheaper variant if it predicts a clear gain over `dflt`, else `dflt`.

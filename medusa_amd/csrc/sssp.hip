@@ -168,7 +168,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_wave
 // wrote), so a round starts without a global read.
 constexpr uint32_t kLocal = 512;
 constexpr uint32_t kLocalQueue = 32768;  // near entries up to which the local rounds run
-constexpr uint32_t kLocalRounds = 16;     // default rounds on graphs of mean out-degree <= kLocalMeanDeg
+constexpr uint32_t kLocalRounds = 32;     // default rounds on graphs of mean out-degree <= kLocalMeanDeg
 constexpr uint32_t kLocalMeanDeg = 4;
 
 struct LocalList {
@@ -184,7 +184,8 @@ struct LocalList {
 // owner of its edge.  A lowered vertex below the threshold goes to side `side` of the local
 // list when one is given (side < 2, one slice, room left), else to the near queue once per
 // superstep (by stamp); one at or above the threshold goes to the far queue.  Returns (in
-// every thread) the edges relaxed; ends with a barrier.  kSpec: read the target's row
+// every thread) the edges relaxed and lowers `fmin` to the smallest distance it appended to
+// the far queue; ends with a barrier.  kSpec: read the target's row
 // offsets together with its distance whenever the offer is below the threshold, so a
 // lowered vertex costs one round trip fewer (worth it on sparse graphs, where rounds are
 // latency-bound; on dense ones most offers lose and the extra reads cost bandwidth).
@@ -193,7 +194,7 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
                                                const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
                                                uint32_t* dist, uint32_t* __restrict__ stamp, uint32_t epoch,
                                                uint32_t thr, Stage& st, const Sinks& k, unsigned long long& ndeg,
-                                               LocalList* loc, uint32_t side) {
+                                               LocalList* loc, uint32_t side, uint32_t& fmin) {
     __shared__ uint32_t s_beg[256], s_off[256], s_du[256], s_wave[4];
     const uint32_t t = threadIdx.x;
     uint32_t total;
@@ -253,6 +254,7 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
                     if (!kept) to_near = atomicMax(&stamp[v], epoch) < epoch;
                 } else {
                     to_far = true;
+                    fmin = min(fmin, nd);
                 }
             }
         }
@@ -263,34 +265,18 @@ __device__ __forceinline__ uint32_t relax_tile(uint32_t beg, uint32_t deg, uint3
     return total;
 }
 
-// Relaxes near entries q[0, qn) with the whole block: workgroup b takes the 256 entries of
-// chunks b, b + grid, ....  With a local list and rounds > 0, the block then runs up to
-// `rounds` further tiles over the vertices it lowered itself (the last round hands its
-// lowered vertices to the near queue).  Returns (in every thread) the edges relaxed by
-// this block.  kSpec as for relax_tile.
+// Up to `rounds` tiles over the vertices the block kept in side 0 of its local list (the
+// last round hands what it lowers to the near queue).  Returns (in every thread) the edges
+// relaxed.  The list's side 0 must be complete: the caller's last append precedes a barrier.
 template <bool kSpec>
-__device__ __forceinline__ unsigned long long relax_block(const unsigned long long* __restrict__ q, uint32_t qn,
-                                                          const uint32_t* __restrict__ col,
-                                                          const uint32_t* __restrict__ w, uint32_t* dist,
-                                                          uint32_t* __restrict__ stamp, uint32_t epoch, uint32_t thr,
-                                                          Stage& st, const Sinks& k, unsigned long long& ndeg,
-                                                          LocalList* loc = nullptr, uint32_t rounds = 0) {
+__device__ __forceinline__ unsigned long long local_rounds(LocalList* loc, uint32_t rounds,
+                                                           const uint32_t* __restrict__ col,
+                                                           const uint32_t* __restrict__ w, uint32_t* dist,
+                                                           uint32_t* __restrict__ stamp, uint32_t epoch, uint32_t thr,
+                                                           Stage& st, const Sinks& k, unsigned long long& ndeg,
+                                                           uint32_t& fmin) {
     const uint32_t t = threadIdx.x;
-    if (!loc) rounds = 0;
     unsigned long long relax = 0;
-    for (uint32_t chunk = blockIdx.x; (uint64_t)chunk * 256 < qn; chunk += gridDim.x) {  // uniform per block
-        const uint64_t i = (uint64_t)chunk * 256 + t;
-        uint32_t beg = 0, deg = 0, du = 0;
-        if (i < qn) {
-            const unsigned long long ent = q[i];
-            const uint32_t u = (uint32_t)ent, slice = (uint32_t)(ent >> 32);
-            const uint32_t b = k.row_ptr[u], e = k.row_ptr[u + 1];
-            beg = b + min(e - b, slice * kSlice);
-            deg = min(e - beg, kSlice);
-            du = dist[u];
-        }
-        relax += relax_tile<kSpec>(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc, rounds ? 0u : 2u);
-    }
     for (uint32_t r = 0; r < rounds; ++r) {
         // list r & 1 is complete: the last tile ended with a barrier
         const uint32_t cur = r & 1, nxt = cur ^ 1;
@@ -308,20 +294,55 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
                 du = loc->du[cur][i];
             }
             relax += relax_tile<kSpec>(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc,
-                                       more ? nxt : 2u);
+                                       more ? nxt : 2u, fmin);
         }
     }
+    return relax;
+}
+
+// Relaxes near entries q[0, qn) with the whole block: workgroup b takes the 256 entries of
+// chunks b, b + grid, ....  With a local list and rounds > 0, the block then runs up to
+// `rounds` further tiles over the vertices it lowered itself.  Returns (in every thread) the
+// edges relaxed by this block.  kSpec as for relax_tile.
+template <bool kSpec>
+__device__ __forceinline__ unsigned long long relax_block(const unsigned long long* __restrict__ q, uint32_t qn,
+                                                          const uint32_t* __restrict__ col,
+                                                          const uint32_t* __restrict__ w, uint32_t* dist,
+                                                          uint32_t* __restrict__ stamp, uint32_t epoch, uint32_t thr,
+                                                          Stage& st, const Sinks& k, unsigned long long& ndeg,
+                                                          LocalList* loc = nullptr, uint32_t rounds = 0) {
+    const uint32_t t = threadIdx.x;
+    if (!loc) rounds = 0;
+    unsigned long long relax = 0;
+    uint32_t fmin = 0xFFFFFFFFu;  // not needed after a relaxation superstep
+    for (uint32_t chunk = blockIdx.x; (uint64_t)chunk * 256 < qn; chunk += gridDim.x) {  // uniform per block
+        const uint64_t i = (uint64_t)chunk * 256 + t;
+        uint32_t beg = 0, deg = 0, du = 0;
+        if (i < qn) {
+            const unsigned long long ent = q[i];
+            const uint32_t u = (uint32_t)ent, slice = (uint32_t)(ent >> 32);
+            const uint32_t b = k.row_ptr[u], e = k.row_ptr[u + 1];
+            beg = b + min(e - b, slice * kSlice);
+            deg = min(e - beg, kSlice);
+            du = dist[u];
+        }
+        relax += relax_tile<kSpec>(beg, deg, du, col, w, dist, stamp, epoch, thr, st, k, ndeg, loc,
+                                   rounds ? 0u : 2u, fmin);
+    }
+    if (rounds) relax += local_rounds<kSpec>(loc, rounds, col, w, dist, stamp, epoch, thr, st, k, ndeg, fmin);
     return relax;
 }
 
 // Splits far entries (or, when `rescan`, every vertex) at a new threshold: distances in
 // [old_thr, thr) move to the near queue (once, by stamp), distances >= thr stay far
 // (deduplicated by stamp); below old_thr they were settled through the near queue.
-// Returns the smallest kept far distance of this thread.
+// Returns the smallest kept far distance of this thread.  With a local list, a vertex moved
+// to the near side (one slice) goes to side 0 of the list instead, for the block to relax in
+// the same launch; the stamp still marks it moved, so duplicates in the far queue move once.
 __device__ __forceinline__ uint32_t split_block(const uint32_t* __restrict__ far_in, uint32_t nin, bool rescan,
                                                 const uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
                                                 uint32_t epoch, uint32_t old_thr, uint32_t thr, Stage& st,
-                                                const Sinks& k, unsigned long long& ndeg) {
+                                                const Sinks& k, unsigned long long& ndeg, LocalList* loc = nullptr) {
     uint32_t fmin = 0xFFFFFFFFu;
     for (uint32_t i0 = blockIdx.x * 256; i0 < nin; i0 += gridDim.x * 256) {  // uniform per block
         const uint32_t i = i0 + threadIdx.x;
@@ -332,8 +353,22 @@ __device__ __forceinline__ uint32_t split_block(const uint32_t* __restrict__ far
             const uint32_t d = dist[v];
             if (d != 0xFFFFFFFFu && d >= old_thr) {
                 if (d < thr) {
-                    vdeg = out_degree(k, v);
+                    const uint32_t vb = k.row_ptr[v];
+                    vdeg = k.row_ptr[v + 1] - vb;
                     to_near = atomicMax(&stamp[v], epoch) < epoch;
+                    if (to_near && loc && vdeg <= kSlice) {
+                        if (vdeg == 0) {
+                            to_near = false;
+                        } else {
+                            const uint32_t pos = atomicAdd(&loc->n[0], 1u);
+                            if (pos < kLocal) {
+                                loc->beg[0][pos] = vb;
+                                loc->du[0][pos] = d;
+                                loc->deg[0][pos] = (uint8_t)vdeg;
+                                to_near = false;
+                            }
+                        }
+                    }
                 } else {
                     keep = atomicMax(&stamp[v], epoch) < epoch;
                     if (keep) fmin = min(fmin, d);
@@ -468,7 +503,7 @@ __device__ uint32_t sssp_pick_delta(SsspCtl& c, uint32_t n_far, const float* __r
     for (int r = 2; r < 12; ++r)
         if (m.out[r] < m.out[best]) best = r;
     c.pol_far = n_far;
-    c.pol_width = m.out[best] + kPolicyMargin < m.out[0] ? (uint32_t)width(best) : c.dflt_delta;
+    c.pol_width = m.out[best] + kWidthMargin < m.out[0] ? (uint32_t)width(best) : c.dflt_delta;
     return c.pol_width;
 }
 
@@ -514,9 +549,19 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
         const uint32_t nin = rescan ? (uint32_t)c.n : c.n_far;
         const Sinks k{c.near_sel ? near1 : near0, &ctr->n_near, near_cap, c.far_sel ? far0 : far1,
                       &ctr->far_out, far_cap, row_ptr};
-        stage_init(st);
+        LocalList* sl = kLoc && c.local_rounds ? loc : nullptr;
+        if (sl && threadIdx.x == 0) sl->n[0] = 0;
+        stage_init(st);  // its barrier publishes sl->n[0]
         uint32_t fmin = split_block(c.far_sel ? far1 : far0, nin, rescan, dist, stamp, epoch, c.old_thr, thr,
-                                    st, k, ndeg);
+                                    st, k, ndeg, sl);
+        if (sl) {
+            // the moved vertices this block kept are relaxed at once; their stamps use the next
+            // epoch, so a vertex this split kept far still enters the near queue if it is lowered
+            // below the threshold
+            const unsigned long long relax =
+                local_rounds<kLoc>(sl, c.local_rounds, col, w, dist, stamp, epoch + 1, thr, st, k, ndeg, fmin);
+            if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
+        }
         stage_flush(st, k, true);
         fmin = wave_min_u32(fmin);
         if ((threadIdx.x & (kWave - 1)) == 0 && fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
@@ -551,20 +596,37 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
     // phase mark to write (trace on): index, then its fields; ~0u = none
     uint32_t mark = 0xFFFFFFFFu, mark_phase = 0, mark_far = 0, mark_thr = 0, mark_delta = 0;
     if (c.split_pending) {
-        // after a split
+        // after a split (which may also have relaxed the vertices it moved, under epoch + 1)
         c.far_sel ^= 1u;
         n_far = min(k.far_out, far_cap);
         c.n_far = n_far;
         c.overflow = k.far_out > far_cap ? 1u : 0u;  // kept entries beyond the cap: rescan next time
-        c.epoch += 1;
+        c.epoch += 2;
+        if (k.relax) {
+            c.supersteps += 1;
+            c.relaxed += k.relax;
+            c.kernel_bytes += 16ull * k.relax + 8ull * k.n_near;
+        }
         c.n_near = k.n_near;
         if (k.n_near > 0) {
             c.split_pending = 0;
         } else if (n_far == 0 && !c.overflow) {
             c.done = 1;
             mark = c.phase + 1;  // end time
-        } else {  // empty bucket: restart from the smallest far distance
+        } else {
+            // the bucket is finished: either this slot relaxed the vertices it moved and
+            // lowered nothing below the threshold that is left for the near queue (a phase
+            // ends), or it moved nothing (an empty bucket).  Either way the threshold restarts
+            // from the smallest far distance.
             const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
+            if (k.relax) {
+                mark = c.phase + 1;
+                mark_phase = c.phase;  // as sssp_pick_delta saw it
+                mark_far = n_far;
+                mark_thr = c.thr;
+                mark_delta = d;
+                c.phase += 1;
+            }
             c.old_thr = c.thr;
             c.delta = d;
             c.thr = sat_add(k.far_min > c.thr ? k.far_min : c.thr, d);
@@ -662,16 +724,16 @@ static size_t sssp_far_cap(const medusa_graph* g, const medusa_opts& o) {
     return o.far_limit ? std::min<size_t>(o.far_limit, full) : full;
 }
 
-// Mean weight (sampled) and the default bucket width, a small multiple of it: 3x with local
-// rounds, which absorb the extra hops of a wider bucket (1400x1400 lattice: 16.9 ms at 3x,
-// 19.3 ms at 2x), else 2x.
+// Mean weight (sampled) and the default bucket width, a multiple of it: 8x with local rounds,
+// which absorb the extra hops of a wider bucket (1400x1400 lattice, 32 rounds: 8.0 ms at 8x,
+// 8.6 ms at 6x, 11.5 ms at 3x), else 2x.
 static void sssp_widths(const medusa_graph* g, const medusa_opts& o, uint32_t& wmean, uint32_t& dflt) {
     uint64_t wsum = 0, wcnt = 0;
     const uint64_t m = g->m;
     const uint64_t wstride = std::max<uint64_t>(1, m / 4096);
     for (uint64_t e = 0; e < m; e += wstride, ++wcnt) wsum += g->h_w[e];
     wmean = (uint32_t)std::max<uint64_t>(1, wcnt ? wsum / wcnt : 1);
-    const uint64_t mult = sssp_local_cfg(g, o) ? 3 : 2;
+    const uint64_t mult = sssp_local_cfg(g, o) ? 8 : 2;
     dflt = (uint32_t)std::min<uint64_t>(0xFFFFFFFEull, mult * wmean);
 }
 
