@@ -282,12 +282,11 @@ def _train_delta_policy(md, wl, np, family):
         with md.Graph(rp, col, wt) as H:
             for d in (250, 500, 1000, 2000, 4000, 8000, 16000, 32000):
                 H.sssp(src, delta=d, trace=1)
-                for r in H.trace():
-                    if r[4] > 0 and r[7] > 0 and r[8] > 0:  # time per settled vertex, in log2 ns
-                        Xs.append(md.policy_features(md.POLICY_DELTA, r, r[7]))
-                        ys.append(np.log2(r[8] * 1e6 / r[4]))
+                X, y = md.delta_samples(H.trace())  # cost to the end of the run per vertex left
+                Xs.append(X)
+                ys.append(y)
     pol = md.Policy(md.POLICY_DELTA, seed=3)
-    pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
+    pol.train(np.concatenate(Xs), np.concatenate(ys), iters=3000, lr=3e-3)
     return pol
 
 

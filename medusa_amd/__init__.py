@@ -410,6 +410,24 @@ def policy_features(kind, record, action):
     return row
 
 
+def delta_samples(trace):
+    """Training rows of the bucket-width policy from the trace of one fixed-width SSSP run.
+
+    Row i pairs phase i's features and width with the run's cost from that phase to its end per
+    vertex still to settle (log2 ns): a width that makes one phase cheap per vertex but leaves
+    costly phases behind it, such as a huge first bucket that relaxes most vertices many times,
+    is charged for those phases too.  Returns (X float32[k, 16], y float32[k])."""
+    tr = np.asarray(trace, np.float64).reshape(-1, TRACE_FIELDS)
+    ms_left = np.cumsum(tr[::-1, 8])[::-1]
+    settled_left = np.cumsum(tr[::-1, 4])[::-1]
+    keep = (tr[:, 7] > 0) & (ms_left > 0) & (settled_left > 0)
+    X = np.zeros((int(keep.sum()), POLICY_INPUTS), np.float32)
+    for j, r in enumerate(tr[keep]):
+        X[j] = policy_features(POLICY_DELTA, r, r[7])
+    y = np.log2(ms_left[keep] * 1e6 / settled_left[keep]).astype(np.float32)
+    return X, y
+
+
 class Policy:
     """Learned per-superstep scheduler (MLP cost model trained on the GPU)."""
 

@@ -121,12 +121,9 @@ def _delta_samples(G, src, deltas):
     Xs, ys = [], []
     for d in deltas:
         G.sssp(src, delta=d, trace=1)
-        tr = G.trace()
-        for r in tr:
-            if r[4] <= 0 or r[7] <= 0 or r[8] <= 0:
-                continue
-            Xs.append(md.policy_features(md.POLICY_DELTA, r, r[7]))
-            ys.append(np.log2(r[8] * 1e6 / r[4]))  # time per settled vertex, in log2 ns
+        X, y = md.delta_samples(G.trace())  # cost to the end of the run per vertex left
+        Xs += list(X)
+        ys += list(y)
     return Xs, ys
 
 

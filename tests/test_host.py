@@ -154,3 +154,22 @@ def test_pack_graphs_host_logic():
     assert list(w) == [5, 6, 7, 8, 9]
     with pytest.raises(md.MedusaError):
         md.pack_graphs([a, b[:2]])  # weights on some graphs only
+
+
+def test_delta_samples_charge_the_rest_of_the_run():
+    """Width-policy rows: each phase is charged the run's time from that phase to the end per
+    vertex still to settle.  Hand-worked trace: phase ms 4, 1, 1 and settled 2, 8, 10 give
+    6 ms / 20, 2 ms / 18 and 1 ms / 10 vertices; a record without a width is dropped."""
+    tr = np.zeros((4, md.TRACE_FIELDS))
+    tr[:, 5], tr[:, 6] = 1000, 3000          # n, m
+    tr[:, 2] = 100                           # mean weight
+    tr[:, 3] = [0, 400, 800, 1200]           # thresholds
+    tr[:, 4] = [2, 8, 10, 0]                 # settled
+    tr[:, 7] = [400, 400, 400, 0]            # widths (the last record has none)
+    tr[:, 8] = [4.0, 1.0, 1.0, 0.5]          # ms
+    X, y = md.delta_samples(tr)
+    assert X.shape == (3, 16) and y.dtype == np.float32
+    want = np.log2([6.5e6 / 20, 2.5e6 / 18, 1.5e6 / 10])
+    assert np.allclose(y, want, rtol=1e-6), (y, want)
+    for j in range(3):
+        assert np.array_equal(X[j], md.policy_features(md.POLICY_DELTA, tr[j], tr[j, 7]))
