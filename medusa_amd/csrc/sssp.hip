@@ -98,10 +98,6 @@ __device__ __forceinline__ void stage_init(Stage& st) {
     __syncthreads();
 }
 
-// Out-degree of v, read by the caller before it waits on v's stamp atomic so that the two
-// round trips overlap.
-__device__ __forceinline__ uint32_t out_degree(const Sinks& k, uint32_t v) { return k.row_ptr[v + 1] - k.row_ptr[v]; }
-
 // One thread's appends: v (out-degree deg) to the near queue (as its slices) and/or to the
 // far queue.
 __device__ __forceinline__ void stage_put(Stage& st, const Sinks& k, bool to_near, bool to_far, uint32_t v,
