@@ -287,6 +287,23 @@ def test_sssp_local_rounds(oracle, case):
             G.sssp(src, local_rounds=65)
 
 
+def test_sssp_sparse_power_law_local_rounds(oracle):
+    """A sparse skewed graph (Chung-Lu, mean out-degree about 2, so local rounds are on by
+    default) whose hubs span many near-queue slices and are never kept locally, and whose
+    frontier grows past the queue length up to which the rounds run: distances stay Dijkstra's
+    with the default rounds, with none, and with the widest bucket."""
+    rp, col = wl.chung_lu_graph(n=200_000, m=420_000, seed=17, max_degree=20_000)
+    w = wl.weighted(rp, col, seed=17)
+    src = wl.max_outdeg_vertex(rp)
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        for kw in ({}, {"local_rounds": U32MAX}, {"delta": U32MAX}, {"local_rounds": 64, "delta": 300},
+                   {"timing": 1}):
+            d, st = G.sssp(src, **kw)
+            assert np.array_equal(d, want), kw
+            assert st.reached == int((want != U32MAX).sum())
+
+
 def test_sssp_hub_source_slices(oracle):
     """A source whose out-degree spans many near-queue slices, plus hubs inside the graph."""
     rng = np.random.default_rng(12)
