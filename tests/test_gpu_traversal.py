@@ -214,15 +214,20 @@ def test_mbfs_rmat20_config(oracle):
 @pytest.mark.parametrize("far_limit", [1, 7, 64])
 def test_sssp_far_queue_limit_rebuilds(oracle, far_limit):
     """A far queue capped far below its need is rebuilt from the distances at every split;
-    distances stay exact on both drivers and for every bucket width."""
+    distances stay exact on both drivers, for every bucket width, and with local rounds (whose
+    split slot relaxes the vertices it moves, also after a rescan) on a random graph and a
+    lattice."""
     rp, col, _ = random_graph(6000, 36000, 31)
     w = np.random.default_rng(31).integers(1, 1001, len(col)).astype(np.uint32)
-    want = oracle.sssp_dijkstra(rp, col, w, 7)
-    with md.Graph(rp, col, w) as G:
-        for delta in (0, 50, 400):
-            for timing in (0, 1):
-                d, _ = G.sssp(7, delta=delta, far_limit=far_limit, timing=timing)
-                assert np.array_equal(d, want), (delta, timing)
+    lrp, lcol, lw = wl.lattice_graph(60, 70, seed=13)
+    for rp, col, w, src in ((rp, col, w, 7), (lrp, lcol, lw, wl.lattice_source(60, 70))):
+        want = oracle.sssp_dijkstra(rp, col, w, src)
+        with md.Graph(rp, col, w) as G:
+            for lr in (0, 32, U32MAX):
+                for delta in (0, 50, 400):
+                    for timing in (0, 1):
+                        d, _ = G.sssp(src, delta=delta, far_limit=far_limit, timing=timing, local_rounds=lr)
+                        assert np.array_equal(d, want), (lr, delta, timing)
 
 
 def test_sssp_zero_weights_duplicates_and_self_loops(oracle):
