@@ -34,22 +34,25 @@ def parse():
                    choices=["pagerank_rmat20", "bfs_er", "bfs_rmat20", "sssp_lattice", "mbfs_rmat20",
                             "chung_lu", "chung_lu_batch", "sssp_rmat20"],
                    help="pagerank_rmat20 is the headline line; the others measure the remaining configurations")
+    p.add_argument("--variant", default="pull", choices=["pull", "push", "binned"],
+                   help="PageRank superstep kernel variant (all give bit-identical ranks)")
     p.add_argument("--profile", action="store_true",
                    help="short run for hardware counters: 3 PageRank runs (non-graph launches) and one "
                         "256 MiB device copy as the byte-count calibration; prints no JSON")
     return p.parse_args()
 
 
-def profile_pass(G, out, torch):
-    """Kernels for a counter pass: the pull kernel 90 times, then a plain copy of known size."""
+def profile_pass(G, out, torch, variant):
+    """Kernels for a counter pass: 3 runs of 30 supersteps of the chosen variant, then a plain copy
+    of known size."""
     for _ in range(3):
-        G.pagerank(0.85, STEPS_PR, out=out)
+        G.pagerank(0.85, STEPS_PR, out=out, variant=variant)
     torch.cuda.synchronize()
     a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
     b = torch.empty_like(a)
     b.copy_(a)
     torch.cuda.synchronize()
-    print("profile pass done: 90 pull launches, copy of 268435456 bytes", flush=True)
+    print("profile pass done: 90 supersteps, copy of 268435456 bytes", flush=True)
 
 
 def traffic_from_profiles():
@@ -82,20 +85,21 @@ def main():
     rp, col = wl.rmat_graph(scale=a.scale, edge_factor=16, seed=2)
     n, m = len(rp) - 1, len(col)
     G = md.Graph(rp, col)
+    var = {"pull": md.PR_PULL, "push": md.PR_PUSH, "binned": md.PR_BINNED}[a.variant]
     out = torch.empty(n, dtype=torch.float32, device="cuda")
     if a.profile:
-        profile_pass(G, out, torch)
+        profile_pass(G, out, torch, var)
         G.close()
         return
 
     # warmup (also builds the pull layout and the captured superstep graph)
     for _ in range(max(1, a.warmup)):
-        G.pagerank(0.85, STEPS_PR, out=out, use_graph=1)
+        G.pagerank(0.85, STEPS_PR, out=out, use_graph=1, variant=var)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dev_ms = 0.0
     for _ in range(a.steps):
-        _, st = G.pagerank(0.85, STEPS_PR, out=out, use_graph=1)
+        _, st = G.pagerank(0.85, STEPS_PR, out=out, use_graph=1, variant=var)
         dev_ms += st.device_ms
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -107,7 +111,7 @@ def main():
     # roofline: HIP events around every pull-kernel launch on the library's stream
     kms, launches, kbytes = 0.0, 0, 0
     for _ in range(3):
-        _, st = G.pagerank(0.85, STEPS_PR, out=out, timing=1)
+        _, st = G.pagerank(0.85, STEPS_PR, out=out, timing=1, variant=var)
         kms += st.kernel_ms
         launches += st.kernel_launches
         kbytes += st.kernel_bytes

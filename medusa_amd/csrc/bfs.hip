@@ -566,28 +566,57 @@ static uint32_t bfs_mode(const medusa_opts& o) {
     return 0;
 }
 
-// Seeding: every source at level 0 (one source by value, several from g->q_aux), then the
+// Work buffers of one BFS run (the handle's own, allocated on its first BFS).
+struct BfsBufs {
+    int32_t* level;
+    uint32_t *q_small, *q_large, *q_aux;
+    unsigned long long* bitmap;
+    BfsCtl* ctl;
+};
+static BfsBufs handle_bufs(medusa_graph* g) {
+    g->bfs_ctl.ensure(1);
+    static_assert(sizeof(BfsCtl) <= sizeof(BfsCtlStorage), "control block storage");
+    return {g->level.p, g->q_small.p, g->q_large.p, g->q_aux.p, g->bitmap.p, reinterpret_cast<BfsCtl*>(g->bfs_ctl.p)};
+}
+
+// Seeding: every source at level 0 (one source by value, several from B.q_aux), then the
 // compaction of level 0 builds the first frontier's bitmap, queues and counters exactly
 // as a superstep would.
-static void bfs_seed(medusa_graph* g, uint32_t nsrc, uint32_t single, uint32_t large) {
+static void bfs_seed(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uint32_t single, uint32_t large) {
     const uint32_t n = g->n, nwords = (n + 63) / 64;
     hipStream_t s = g->stream;
     FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
-    hipLaunchKernelGGL(k_bfs_reset, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->level.p,
+    hipLaunchKernelGGL(k_bfs_reset, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, B.level,
                        nsrc == 1 ? single : 0xFFFFFFFFu, ctr);
     MDS_LAUNCHED();
     if (nsrc > 1) {
-        hipLaunchKernelGGL(k_bfs_mark, dim3(grid_for(nsrc, 256, 1024)), dim3(256), 0, s, (const uint32_t*)g->q_aux.p,
-                           nsrc, g->level.p);
+        hipLaunchKernelGGL(k_bfs_mark, dim3(grid_for(nsrc, 256, 1024)), dim3(256), 0, s, (const uint32_t*)B.q_aux,
+                           nsrc, B.level);
         MDS_LAUNCHED();
     }
-    launch_compact(s, nullptr, n, 0, (const int32_t*)g->level.p, g->row_ptr.p, large, g->q_small.p, g->q_large.p,
-                   g->bitmap.p, nwords, ctr);
+    launch_compact(s, nullptr, n, 0, (const int32_t*)B.level, g->row_ptr.p, large, B.q_small, B.q_large, B.bitmap,
+                   nwords, ctr);
     MDS_LAUNCHED();
 }
 
-// Device-driven run: seed, first decision, then replays of a captured batch of supersteps.
-static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_t* out, const medusa_opts& o, medusa_stats& st) {
+// One device-driven superstep slot: expansion, compaction, decision.
+static void bfs_enqueue_slot(medusa_graph* g, const BfsBufs& B, uint32_t large, uint32_t grid, const float* policy) {
+    const uint32_t n = g->n, nwords = (n + 63) / 64;
+    hipStream_t s = g->stream;
+    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
+    hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)B.ctl, (const uint32_t*)B.q_small,
+                       (const uint32_t*)B.q_large, g->row_ptr.p, g->col.p, g->cptr.p, g->csrc.p,
+                       (const unsigned long long*)B.bitmap, B.level, n, ctr);
+    launch_compact(s, B.ctl, n, 0, (const int32_t*)B.level, g->row_ptr.p, large, B.q_small, B.q_large, B.bitmap,
+                   nwords, ctr);
+    hipLaunchKernelGGL(k_bfs_decide, dim3(1), dim3(256), 0, s, B.ctl, n, nwords, ctr, policy, grid);
+}
+
+// Device-driven run: seed, first decision, then batches of superstep slots until the control
+// block says done.  capture: the batches are captured once as hipGraphs and replayed; without
+// it the same slots are launched one by one.
+static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uint32_t single, int32_t* out,
+                           const medusa_opts& o, medusa_stats& st, bool capture = true) {
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const uint32_t nwords = (n + 63) / 64;
@@ -595,26 +624,17 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
     hipStream_t s = g->stream;
     const uint32_t mode = bfs_mode(o);
     const float* policy = mode == 3 ? policy_device_params(o.policy) : nullptr;
-    g->bfs_ctl.ensure(1);
-    BfsCtl* ctl = reinterpret_cast<BfsCtl*>(g->bfs_ctl.p);
-    static_assert(sizeof(BfsCtl) <= sizeof(BfsCtlStorage), "control block storage");
+    BfsCtl* ctl = B.ctl;
     FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
 
     // capture (once per graph, threshold, policy buffer and slot count) a batch of supersteps
     auto captured = [&](GraphCache& gc, uint32_t slots) -> hipGraphExec_t {
-        const uintptr_t key[4] = {(uintptr_t)g->level.p, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
-                                  (uintptr_t)g->bitmap.p ^ ((uintptr_t)slots << 48)};
+        const uintptr_t key[4] = {(uintptr_t)B.level, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
+                                  (uintptr_t)B.bitmap ^ ((uintptr_t)slots << 48)};
         if (gc.exec && std::equal(key, key + 4, gc.key)) return gc.exec;
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (uint32_t k = 0; k < slots; ++k) {
-            hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)ctl,
-                               (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
-                               g->cptr.p, g->csrc.p, (const unsigned long long*)g->bitmap.p, g->level.p, n, ctr);
-            launch_compact(s, ctl, n, 0, (const int32_t*)g->level.p, g->row_ptr.p, large, g->q_small.p,
-                           g->q_large.p, g->bitmap.p, nwords, ctr);
-            hipLaunchKernelGGL(k_bfs_decide, dim3(1), dim3(256), 0, s, ctl, n, nwords, ctr, policy, grid);
-        }
+        for (uint32_t k = 0; k < slots; ++k) bfs_enqueue_slot(g, B, large, grid, policy);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
@@ -624,10 +644,10 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
         return gc.exec;
     };
     const uint32_t head = head_slots(g->bfs_last_steps, kBfsBatch);
-    const hipGraphExec_t head_exec = captured(head == kBfsBatch ? g->bfs_graph : g->bfs_head, head);
+    const hipGraphExec_t head_exec = capture ? captured(head == kBfsBatch ? g->bfs_graph : g->bfs_head, head) : nullptr;
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    bfs_seed(g, nsrc, single, large);
+    bfs_seed(g, B, nsrc, single, large);
     hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, ctl, ctr, policy, n, (unsigned long long)m,
                        mode, grid);
     MDS_LAUNCHED();
@@ -635,14 +655,19 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
     for (uint32_t batch = 0;; ++batch) {
         // the tail batch is captured after the timed region has started only on a handle's
         // first runs; later runs find it cached
-        MDS_HIP(hipGraphLaunch(batch == 0 ? head_exec : captured(g->bfs_graph, kBfsBatch), s));
+        if (capture) {
+            MDS_HIP(hipGraphLaunch(batch == 0 ? head_exec : captured(g->bfs_graph, kBfsBatch), s));
+        } else {
+            for (uint32_t k = 0; k < kBfsBatch; ++k) bfs_enqueue_slot(g, B, large, grid, policy);
+            MDS_LAUNCHED();
+        }
         MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
         MDS_HIP(hipStreamSynchronize(s));
         std::memcpy(&h, g->h_ctr_raw.p, sizeof(BfsCtl));
         if (h.done) break;
         MDS_CHECK(batch < (n + kBfsBatch) / kBfsBatch + 2, MEDUSA_ERR_INTERNAL, "bfs: superstep loop did not end");
     }
-    g->bfs_last_steps = h.supersteps;
+    if (capture) g->bfs_last_steps = h.supersteps;
     MDS_HIP(hipEventRecord(g->ev_end, s));
     MDS_HIP(hipStreamSynchronize(s));
     float ms = 0;
@@ -655,7 +680,7 @@ static void bfs_run_device(medusa_graph* g, uint32_t nsrc, uint32_t single, int3
     st.reached = (uint32_t)h.reached;
     st.kernel_bytes = h.kernel_bytes;
     st.alg_bytes = h.alg_bytes + 8ull * n + nwords * 8ull + 8ull * nsrc;
-    copy_out(g, out, g->level.p, (size_t)n * 4);
+    if (out != B.level) copy_out(g, out, B.level, (size_t)n * 4);
 }
 
 // Host-driven run: the same expansion and compaction kernels, with the host deciding each
@@ -669,12 +694,12 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
     hipStream_t s = g->stream;
     const bool timing = o.timing != 0;
     FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
-    g->bfs_ctl.ensure(1);
-    BfsCtl* ctl = reinterpret_cast<BfsCtl*>(g->bfs_ctl.p);
+    const BfsBufs B = handle_bufs(g);
+    BfsCtl* ctl = B.ctl;
     g->trace.clear();
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    bfs_seed(g, nsrc, single, large);
+    bfs_seed(g, B, nsrc, single, large);
     FrontierCounters c = read_counters<FrontierCounters>(g);
 
     Frontier f{};
@@ -772,7 +797,7 @@ static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, 
         MDS_HIP(hipMemcpyAsync(g->q_aux.p, sources, (size_t)count * 4, hipMemcpyHostToDevice, g->stream));
     medusa_stats st{};
     if (o.timing || o.trace) bfs_run_host(g, count, sources[0], levels, o, st);
-    else bfs_run_device(g, count, sources[0], levels, o, st);
+    else bfs_run_device(g, handle_bufs(g), count, sources[0], levels, o, st);
     if (stats) *stats = st;
 }
 

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pr_bins.h"
 
 namespace medusa {
 
@@ -143,10 +144,18 @@ struct PrBlock {
 struct PrPart {
     uint32_t first_block, nblocks, n, pad;
 };
-constexpr uint32_t kPrLane = 0;  // one lane per vertex, in-degree 1..64, ELL slab
-constexpr uint32_t kPrWave = 1;  // one wave per vertex, in-degree > 64
-constexpr uint32_t kPrZero = 2;  // no in-edges
-constexpr uint32_t kPrLaneMax = 64;
+struct PrBins {
+    uint32_t nbins = 0, nhubs = 0, ntiles = 0, max_runs = 0;
+    uint64_t nmsg = 0;     // messages per superstep (in-edges of non-hub destinations)
+    uint64_t nentry = 0;   // message-array entries (runs padded for aligned access)
+    DBuf<PrItem> items;    // combine items, bin by bin; then one kPrWave item per hub
+    DBuf<PrBin> bins;
+    DBuf<PrMsgRun> runs;   // per bin: its runs and a sentinel
+    DBuf<PrTile> tiles;
+    DBuf<uint16_t> srcloc; // message order: source offset of every entry in its chunk
+    DBuf<uint16_t> slot;   // bin order: LDS slot of every entry
+    DBuf<float> msg;       // the message array
+};
 
 // In-degree-sorted pull layout for PageRank (built on first use).
 // Internal ids keep every packed graph's id range (a single graph is one range);
@@ -175,6 +184,9 @@ struct PrLayout {
     DBuf<uint32_t> outdeg;  // out-degree per internal id (n)
     DBuf<uint32_t> optr;    // out-edge offsets in internal order (n+1), push variant
     DBuf<uint32_t> oslot;   // in-edge slot of every out-edge (m), push variant
+    // host copies of cptr / crow / outdeg, from which the binned layout is built on first use
+    std::vector<uint32_t> h_cptr, h_crow, h_od;
+    std::unique_ptr<PrBins> bins;
     // work
     DBuf<float> rank, rank_prev, contrib0, contrib1, msg, base, unperm;
     DBuf<float> partials;   // (s, c) pairs of the dangling-sum blocks
@@ -298,6 +310,7 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_wg, uint32_t cap = 8192) {
     return (uint32_t)g;
 }
 void build_pr_layout(medusa_graph* g);
+void build_pr_bins(medusa_graph* g);
 void build_pull_layout(medusa_graph* g);
 
 constexpr size_t kCounterWords = 256;  // 2 KiB

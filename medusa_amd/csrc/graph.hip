@@ -249,7 +249,46 @@ void build_pr_layout(medusa_graph* g) {
     L->optr.upload(optr.data(), (size_t)n + 1, s);
     L->oslot.upload(oslot.data(), m, s);
     MDS_HIP(hipStreamSynchronize(s));
+    L->h_cptr = std::move(cptr);
+    L->h_crow = std::move(crow);
+    L->h_od = std::move(od);
     g->pr = std::move(L);
+}
+
+// Binned message exchange of the PageRank layout: planned on the host from the internal CSC,
+// every index checked against the buffers it addresses, then uploaded.
+void build_pr_bins(medusa_graph* g) {
+    build_pr_layout(g);
+    PrLayout& L = *g->pr;
+    if (L.bins) return;
+    PrBinsPlan P;
+    const std::string err = plan_pr_bins(g->n, L.h_cptr.data(), L.h_crow.data(), g->parts.data(),
+                                         (uint32_t)g->parts.size() - 1, P);
+    MDS_CHECK(err.empty(), MEDUSA_ERR_STATE, "%s", err.c_str());
+    const std::string bad = check_pr_bins(g->n, L.h_cptr.data(), P);
+    MDS_CHECK(bad.empty(), MEDUSA_ERR_INTERNAL, "binned PageRank plan: %s", bad.c_str());
+    auto B = std::make_unique<PrBins>();
+    hipStream_t s = g->stream;
+    B->nbins = (uint32_t)P.bins.size();
+    B->nhubs = P.nhubs;
+    B->ntiles = (uint32_t)P.tiles.size();
+    B->max_runs = P.max_runs;
+    B->nmsg = P.nmsg;
+    B->nentry = P.nentry;
+    B->items.upload(P.items.data(), P.items.size(), s);
+    B->bins.upload(P.bins.data(), P.bins.size(), s);
+    B->runs.upload(P.runs.data(), P.runs.size(), s);
+    B->tiles.upload(P.tiles.data(), P.tiles.size(), s);
+    if (P.nentry) {
+        B->srcloc.upload(P.srcloc.data(), P.nentry, s);
+        B->slot.upload(P.slot.data(), P.nentry, s);
+    } else {  // no message: every in-edge goes to a hub (or there are none)
+        B->srcloc.alloc(1);
+        B->slot.alloc(1);
+    }
+    B->msg.alloc(std::max<uint64_t>(P.nentry, 1));
+    MDS_HIP(hipStreamSynchronize(s));
+    L.bins = std::move(B);
 }
 
 }  // namespace medusa

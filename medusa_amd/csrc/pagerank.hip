@@ -137,6 +137,43 @@ __device__ __forceinline__ float lane_vertex_sum(uint32_t deg, const uint32_t* _
     return s0 - c0;
 }
 
+// The same canonical sum for a lane whose elements come from a functor (LDS slots of the
+// binned variant).
+template <class Load>
+__device__ __forceinline__ void lane_subtree_at(float& S, float& C, uint32_t r, uint32_t st, uint32_t deg, Load load) {
+    float s[16], c[16];
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+        s[j] = (r + st * j < deg) ? load(r + st * j) : 0.f;
+        c[j] = 0.f;
+    }
+    tree16(s, c);
+    S = s[0];
+    C = c[0];
+}
+template <class Load>
+__device__ __forceinline__ float lane_vertex_sum_at(uint32_t deg, Load load) {
+    // the 1, 2 or 4 interleaved subtrees one at a time (a loop, so only one subtree's leaves
+    // are live), then merged as the 64-slot tree merges them
+    const uint32_t st = deg <= 16 ? 1u : deg <= 32 ? 2u : 4u;
+    float ss[4], cc[4];
+#pragma unroll 1
+    for (uint32_t r = 0; r < st; ++r) {
+        float S, C;
+        lane_subtree_at(S, C, r, st, deg, load);
+        ss[r] = S;
+        cc[r] = C;
+    }
+    if (st == 2) {
+        kahan_combine(ss[0], cc[0], ss[1], cc[1]);
+    } else if (st == 4) {
+        kahan_combine(ss[0], cc[0], ss[2], cc[2]);
+        kahan_combine(ss[1], cc[1], ss[3], cc[3]);
+        kahan_combine(ss[0], cc[0], ss[1], cc[1]);
+    }
+    return ss[0] - cc[0];
+}
+
 // One superstep for every vertex.  kFromMsg: values come from the message
 // array in slot order (push variant) instead of contrib[src] (pull variant).
 template <bool kFromMsg>
@@ -181,16 +218,12 @@ __global__ __launch_bounds__(256) void k_pr_main(const PrRun* __restrict__ runs,
     finish(p, lane_vertex_sum<kFromMsg>(R.deg, ell + R.ell_base + i, R.ell_stride, vals, seg));
 }
 
-// Dangling mass, step 1: one wave per canonical block of a graph's dangling tail
-// writes the block's (s, c) pair.
-__global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ rank, const PrBlock* __restrict__ blocks,
-                                                   uint32_t nblk, float* __restrict__ partials) {
-    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const uint32_t b = blockIdx.x * 4 + wave;
-    if (b >= nblk) return;  // uniform across the wave
-    const PrBlock B = blocks[b];
+// Canonical (s, c) pair of one dangling block, computed by one wave (lane 0 holds it).
+__device__ __forceinline__ void dangle_block_sum(const float* __restrict__ rank, const PrBlock B, float& s, float& c) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     const float* x = rank + B.begin;
-    float s = 0.f, c = 0.f;
+    s = 0.f;
+    c = 0.f;
     for (uint32_t r0 = 0; r0 * kWave < B.len; r0 += 16) {
         float v[16];
 #pragma unroll
@@ -203,27 +236,41 @@ __global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ ran
             if (lane + (r0 + j) * kWave < B.len) kahan_add(s, c, v[j]);
     }
     group_tree(s, c, kWave);
+}
+
+// Dangling mass, step 1: one wave per canonical block of a graph's dangling tail
+// writes the block's (s, c) pair.
+__global__ __launch_bounds__(256) void k_pr_dangle(const float* __restrict__ rank, const PrBlock* __restrict__ blocks,
+                                                   uint32_t nblk, float* __restrict__ partials) {
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const uint32_t b = blockIdx.x * 4 + wave;
+    if (b >= nblk) return;  // uniform across the wave
+    float s, c;
+    dangle_block_sum(rank, blocks[b], s, c);
     if (lane == 0) {
         partials[2 * b] = s;
         partials[2 * b + 1] = c;
     }
 }
 
-// Dangling mass, step 2 (after the block kernel, so its partials are visible): one wave
-// per graph merges the graph's block pairs left to right and writes
-// base = fmaf(d, D / (float)n_g, (float)((1 - d) / n_g)).
-__global__ __launch_bounds__(256) void k_pr_base(const float* __restrict__ partials, const PrPart* __restrict__ parts,
-                                                 uint32_t nparts, float d, float* __restrict__ base_out) {
+// base = fmaf(d, D / (float)n_g, (float)((1 - d) / n_g)) for dangling mass D.
+__device__ __forceinline__ float pr_base_value(float D, float d, uint32_t n_g) {
+    const float tele = (float)((1.0 - (double)d) / (double)n_g);
+    return fmaf(d, D / (float)n_g, tele);
+}
+
+// base = fmaf(d, D / (float)n_g, (float)((1 - d) / n_g)) of a graph whose `nblocks` block pairs
+// start at partials[2 * first]: the pairs merge left to right.  Called by a whole wave; every
+// lane returns the value.
+__device__ __forceinline__ float pr_base_fold(const float* __restrict__ partials, uint32_t first, uint32_t nblocks,
+                                              uint32_t n_g, float d) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t w = blockIdx.x * 4 + threadIdx.x / kWave;
-    if (w >= nparts) return;  // uniform across the wave
-    const PrPart P = parts[w];
     float as = 0.f, ac = 0.f;
-    for (uint32_t c0 = 0; c0 < P.nblocks; c0 += kWave) {
+    for (uint32_t c0 = 0; c0 < nblocks; c0 += kWave) {
         const uint32_t i = c0 + lane;
-        const float ps = i < P.nblocks ? partials[2 * (P.first_block + i)] : 0.f;
-        const float pc = i < P.nblocks ? partials[2 * (P.first_block + i) + 1] : 0.f;
-        const uint32_t lim = min(P.nblocks - c0, (uint32_t)kWave);
+        const float ps = i < nblocks ? partials[2 * (first + i)] : 0.f;
+        const float pc = i < nblocks ? partials[2 * (first + i) + 1] : 0.f;
+        const uint32_t lim = min(nblocks - c0, (uint32_t)kWave);
         for (uint32_t j = 0; j < lim; ++j) {
             const float s2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ps), j));
             const float c2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pc), j));
@@ -235,9 +282,19 @@ __global__ __launch_bounds__(256) void k_pr_base(const float* __restrict__ parti
             }
         }
     }
-    const float D = as - ac;
-    const float tele = (float)((1.0 - (double)d) / (double)P.n);
-    if (lane == 0) base_out[w] = fmaf(d, D / (float)P.n, tele);
+    return pr_base_value(as - ac, d, n_g);
+}
+
+// Dangling mass, step 2 (after the block kernel, so its partials are visible): one wave
+// per graph merges the graph's block pairs and writes its base.
+__global__ __launch_bounds__(256) void k_pr_base(const float* __restrict__ partials, const PrPart* __restrict__ parts,
+                                                 uint32_t nparts, float d, float* __restrict__ base_out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t w = blockIdx.x * 4 + threadIdx.x / kWave;
+    if (w >= nparts) return;  // uniform across the wave
+    const PrPart P = parts[w];
+    const float base = pr_base_fold(partials, P.first_block, P.nblocks, P.n, d);
+    if (lane == 0) base_out[w] = base;
 }
 
 constexpr uint32_t kSmallBlocks = 64;  // dangling blocks up to which one workgroup does both steps
@@ -288,6 +345,223 @@ __global__ __launch_bounds__(256) void k_pr_dangle_base(const float* __restrict_
         const float D = as - ac;
         const float tele = (float)((1.0 - (double)d) / (double)P.n);
         base_out[g] = fmaf(d, D / (float)P.n, tele);
+    }
+}
+
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs (each with its own L2):
+// item b of a launch of `count` maps to XCD (b mod 8)'s contiguous share of the items, so
+// consecutive items run on one XCD.
+__device__ __forceinline__ uint32_t xcd_major(uint32_t b, uint32_t count) {
+    const uint32_t q = count / 8, r = count % 8, x = b % 8, j = b / 8;
+    return x * q + min(x, r) + j;
+}
+
+// Binned send (one launch per superstep).  Workgroups [0, ntiles) each copy their chunk's
+// contributions into LDS and write one tile of the message array (contiguous; runs are padded
+// to 4 entries, so every access is 16-byte aligned).  Tiles of one chunk run on one XCD, so the
+// chunk's contributions are read from HBM once per XCD.  Workgroups from ntiles on write the
+// (s, c) pairs of the dangling blocks of `rank` (4 blocks each), which the combine of the same
+// superstep merges into its base.
+__global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tiles, uint32_t ntiles,
+                                                 const uint16_t* __restrict__ srcloc, const float* __restrict__ contrib,
+                                                 float* __restrict__ msg, const float* __restrict__ rank,
+                                                 const PrBlock* __restrict__ blocks, uint32_t nblk,
+                                                 float* __restrict__ partials) {
+    __shared__ float s_c[kPrChunkSources];
+    using u16x4 = unsigned short __attribute__((ext_vector_type(4)));
+    using f32x4 = float __attribute__((ext_vector_type(4)));
+    if (blockIdx.x >= ntiles) {
+        const uint32_t lane = threadIdx.x & (kWave - 1);
+        const uint32_t b = (blockIdx.x - ntiles) * 4 + threadIdx.x / kWave;
+        if (b >= nblk) return;  // uniform across the wave
+        float bs, bc;
+        dangle_block_sum(rank, blocks[b], bs, bc);
+        if (lane == 0) {
+            partials[2 * b] = bs;
+            partials[2 * b + 1] = bc;
+        }
+        return;
+    }
+    const PrTile T = tiles[xcd_major(blockIdx.x, ntiles)];
+    {  // chunks start on multiples of kPrChunkSources, so the slice is 16-byte aligned
+        const f32x4* c4 = reinterpret_cast<const f32x4*>(contrib + T.src_begin);
+        f32x4* s4 = reinterpret_cast<f32x4*>(s_c);
+        const uint32_t n4 = T.src_count / 4;
+        for (uint32_t i = threadIdx.x; i < n4; i += 256) s4[i] = c4[i];
+        for (uint32_t i = 4 * n4 + threadIdx.x; i < T.src_count; i += 256) s_c[i] = contrib[T.src_begin + i];
+    }
+    __syncthreads();
+    const u16x4* __restrict__ sl = reinterpret_cast<const u16x4*>(srcloc + T.msg_begin);
+    f32x4* __restrict__ out = reinterpret_cast<f32x4*>(msg + T.msg_begin);
+    const uint32_t nq = (T.msg_end - T.msg_begin) / 4;
+    for (uint32_t q0 = 0; q0 < nq; q0 += 4 * 256) {
+        u16x4 a[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t q = q0 + u * 256 + threadIdx.x;
+            a[u] = q < nq ? sl[q] : u16x4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t q = q0 + u * 256 + threadIdx.x;
+            if (q < nq) out[q] = f32x4{s_c[a[u].x], s_c[a[u].y], s_c[a[u].z], s_c[a[u].w]};
+        }
+    }
+}
+
+constexpr uint32_t kCombineThreads = 512;
+constexpr uint32_t kCombineWaves = kCombineThreads / kWave;
+constexpr uint32_t kCombineUnroll = 4;  // groups of four entries per thread in flight
+
+// Binned combine (one launch per superstep).  Workgroup b < nbins:
+//   1. stages its run table (message start and first entry number of every run) in LDS and
+//      zeroes its LDS slots, while every wave loads the records of its items (items w, w+8, ...
+//      for wave w, one per lane) into registers;
+//   2. reads its entries four at a time: a binary search over the staged table gives a group
+//      its run, so the group's four messages (16 bytes) and four slots (8 bytes) load together;
+//      each message goes to its LDS slot (padding entries to a dummy slot);
+//   3. after the barrier, wave w sums its items from LDS in the canonical order.  A lane
+//      vertex sums as many slots as the largest in-degree of its group: the slots past its own
+//      in-degree hold +0.0, which merges as the exact identity the canonical tree uses for
+//      absent elements, so no per-vertex degree is read.
+// Workgroups from nbins on sum the hub destinations (one wave each) by gathering contrib
+// over the CSC, as the pull variant does.  kFold: a single graph with at most 64 dangling
+// blocks, whose base every wave merges here from the send's block pairs; otherwise
+// base[part] was written by k_pr_base.
+// Dynamic LDS after the slots: max_runs run starts and max_runs + 1 run numbers.
+template <bool kFold>
+__global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
+    const PrBin* __restrict__ bins, uint32_t nbins, const PrItem* __restrict__ items,
+    const PrItem* __restrict__ hubs, uint32_t nhubs, const PrMsgRun* __restrict__ runs, uint32_t max_runs,
+    const float* __restrict__ msg, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ cptr,
+    const uint32_t* __restrict__ crow, const float* __restrict__ contrib_in, const uint32_t* __restrict__ outdeg,
+    const float* __restrict__ base_p, const float* __restrict__ partials, uint32_t nblk, uint32_t n, float d,
+    float* __restrict__ rank_out, float* __restrict__ contrib_out) {
+    __shared__ float lds[kPrBinValues + 4];  // + the dummy slot of padding entries
+    extern __shared__ uint32_t s_runs[];
+    uint32_t* s_rm = s_runs;
+    uint32_t* s_rk = s_runs + max_runs;
+    using u16x4 = unsigned short __attribute__((ext_vector_type(4)));
+    using f32x4 = float __attribute__((ext_vector_type(4)));
+    const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    auto finish = [&](uint32_t p, float S, float base) {
+        const float r = fmaf(d, S, base);
+        rank_out[p] = r;
+        const uint32_t od = outdeg[p];
+        if (od) contrib_out[p] = r / (float)od;
+    };
+    if (blockIdx.x >= nbins) {
+        const uint32_t h = (blockIdx.x - nbins) * kCombineWaves + wave;
+        if (h >= nhubs) return;  // uniform across the wave
+        const PrItem I = hubs[h];
+        const float base = kFold ? pr_base_fold(partials, 0, nblk, n, d) : base_p[I.part];
+        const uint32_t b = cptr[I.v], len = cptr[I.v + 1] - b;
+        const float S = wave_segment_sum(len, [&](uint32_t k) { return contrib_in[crow[b + k]]; });
+        if (lane == 0) finish(I.v, S, base);
+        return;
+    }
+    const PrBin B = bins[blockIdx.x];
+    // 1. this wave's items, one record per lane (a bin has at most 64 items per wave)
+    const uint32_t my_item = B.item_begin + wave + kCombineWaves * lane;
+    uint32_t iv = 0, icount = 0, ikind = 0, ilds = 0, ipart = 0, ideg = 0;
+    if (my_item < B.item_end) {
+        const PrItem I = items[my_item];
+        iv = I.v;
+        icount = I.count;
+        ikind = I.kind;
+        ilds = I.lds;
+        ipart = I.part;
+        ideg = I.deg;
+    }
+    float ps = 0.f, pc = 0.f;  // kFold: this lane's dangling block pair
+    if (kFold && lane < nblk) {
+        ps = partials[2 * lane];
+        pc = partials[2 * lane + 1];
+    }
+    const uint32_t nr = B.run_end - B.run_begin;
+    for (uint32_t j = threadIdx.x; j <= nr; j += kCombineThreads) {
+        const PrMsgRun R = runs[B.run_begin + j];  // j = nr: the sentinel, k = the bin's length
+        s_rk[j] = R.k;
+        if (j < nr) s_rm[j] = R.msg;
+    }
+    {
+        f32x4* z = reinterpret_cast<f32x4*>(lds);
+        for (uint32_t i = threadIdx.x; i < kPrBinValues / 4; i += kCombineThreads) z[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    __syncthreads();
+    // 2. entries into their slots
+    const u16x4* __restrict__ s4 = reinterpret_cast<const u16x4*>(slot + B.slot_begin);
+    const f32x4* __restrict__ m4 = reinterpret_cast<const f32x4*>(msg);
+    const uint32_t nq = s_rk[nr] / 4;
+    for (uint32_t q0 = 0; q0 < nq; q0 += kCombineThreads * kCombineUnroll) {
+        uint32_t at[kCombineUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kCombineUnroll; ++u) {
+            const uint32_t k = 4 * (q0 + u * kCombineThreads + threadIdx.x);
+            uint32_t lo = 0, hi = nr;  // s_rk[lo] <= k < s_rk[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_rk[mid] <= k) lo = mid;
+                else hi = mid;
+            }
+            at[u] = (s_rm[lo] + (k - s_rk[lo])) / 4;
+        }
+        f32x4 x[kCombineUnroll];
+        u16x4 t[kCombineUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kCombineUnroll; ++u) {
+            const uint32_t q = q0 + u * kCombineThreads + threadIdx.x;
+            if (q < nq) {
+                x[u] = m4[at[u]];
+                t[u] = s4[q];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kCombineUnroll; ++u) {
+            if (q0 + u * kCombineThreads + threadIdx.x < nq) {
+                lds[t[u].x] = x[u].x;
+                lds[t[u].y] = x[u].y;
+                lds[t[u].z] = x[u].z;
+                lds[t[u].w] = x[u].w;
+            }
+        }
+    }
+    __syncthreads();
+    // 3. sums
+    float base_all = 0.f;
+    if (kFold) {
+        float as = 0.f, ac = 0.f;
+        for (uint32_t j = 0; j < nblk; ++j) {
+            const float s2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ps), j));
+            const float c2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pc), j));
+            if (j == 0) {
+                as = s2;
+                ac = c2;
+            } else {
+                kahan_combine(as, ac, s2, c2);
+            }
+        }
+        base_all = pr_base_value(as - ac, d, n);
+    }
+    const uint32_t mine = B.item_end > B.item_begin + wave
+                              ? (B.item_end - B.item_begin - wave + kCombineWaves - 1) / kCombineWaves
+                              : 0u;
+    for (uint32_t j = 0; j < mine; ++j) {
+        const uint32_t v = __builtin_amdgcn_readlane(iv, j), cnt = __builtin_amdgcn_readlane(icount, j);
+        const uint32_t kind = __builtin_amdgcn_readlane(ikind, j), at = __builtin_amdgcn_readlane(ilds, j);
+        const uint32_t deg = __builtin_amdgcn_readlane(ideg, j);
+        const float base = kFold ? base_all : base_p[__builtin_amdgcn_readlane(ipart, j)];
+        if (kind == kPrWave) {
+            const float S = wave_segment_sum(deg, [&](uint32_t k) { return lds[at + k]; });
+            if (lane == 0) finish(v, S, base);
+        } else if (lane < cnt) {
+            if (kind == kPrZero) {
+                finish(v + lane, 0.f, base);
+            } else {
+                const float* x = lds + at + lane;
+                finish(v + lane, lane_vertex_sum_at(deg, [&](uint32_t k) { return x[k * kWave]; }), base);
+            }
+        }
     }
 }
 
@@ -346,6 +620,10 @@ __global__ void k_zero_f64(double* p) { *p = 0.0; }
 
 __global__ void k_set_f32(float* p, float v) { *p = v; }
 
+// A single graph with at most this many dangling blocks merges its base inside the binned
+// combine (every wave merges the block pairs itself) instead of in a launch of its own.
+constexpr uint32_t kFoldBlocks = 64;
+
 static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, const medusa_opts& o,
                    medusa_stats& st) {
     build_pr_layout(g);
@@ -355,8 +633,8 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     const uint32_t nd = L.n_dangling;
     const uint32_t nblk = (uint32_t)L.blocks.size();
     const uint32_t nparts = (uint32_t)L.parts.size();
-    MDS_CHECK(o.variant == MEDUSA_PR_PULL || o.variant == MEDUSA_PR_PUSH, MEDUSA_ERR_ARG,
-              "pagerank: unknown variant %d", o.variant);
+    MDS_CHECK(o.variant == MEDUSA_PR_PULL || o.variant == MEDUSA_PR_PUSH || o.variant == MEDUSA_PR_BINNED,
+              MEDUSA_ERR_ARG, "pagerank: unknown variant %d", o.variant);
     // kernel variant of every superstep: the caller's, or the learned policy's per superstep
     // (it keeps the caller's unless it predicts a clear gain); the ranks are the same bits
     uint64_t wave_vertices = 0, lane_vertices = 0;
@@ -374,6 +652,10 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
         sig = (sig ^ var[t]) * 1099511628211ull;
     }
     const bool push = std::find(var.begin(), var.end(), (uint8_t)MEDUSA_PR_PUSH) != var.end();
+    const bool binned = std::find(var.begin(), var.end(), (uint8_t)MEDUSA_PR_BINNED) != var.end();
+    if (binned) build_pr_bins(g);
+    PrBins* Bn = L.bins.get();
+    const bool fold = nparts == 1 && nblk <= kFoldBlocks;
     L.rank.ensure(n);
     L.contrib0.ensure(n);
     L.contrib1.ensure(n);
@@ -410,7 +692,6 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     hipLaunchKernelGGL(k_pr_init, dim3(grid_for(n, 256)), dim3(256), 0, s, n, (const uint32_t*)L.d_part_begin.p,
                        nparts, L.outdeg.p, L.rank.p, L.contrib0.p);
     MDS_LAUNCHED();
-    enqueue_base(L.rank.p);
 
     const bool timing = o.timing != 0 || o.trace != 0;
     // per superstep: [3t] superstep start, [3t+1] main kernel start, [3t+2] main kernel end
@@ -421,11 +702,41 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
 
     auto enqueue_steps = [&](bool with_events) {
         for (uint32_t t = 0; t < steps; ++t) {
+            const float* r_in = rk[t & 1];
             float* r_out = rk[(t + 1) & 1];
             const float* c_in = ct[t & 1];
             float* c_out = ct[(t + 1) & 1];
             const bool step_push = var[t] == MEDUSA_PR_PUSH;
             if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t], s));
+            if (var[t] == MEDUSA_PR_BINNED) {
+                // send: messages of c_t and dangling block pairs of r_t; combine: r_{t+1}
+                const uint32_t dwg = (nblk + 3) / 4;
+                if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 1], s));
+                if (dwg + Bn->ntiles) {  // none when nothing dangles and every in-edge goes to a hub
+                    hipLaunchKernelGGL(k_pr_send, dim3(Bn->ntiles + dwg), dim3(256), 0, s,
+                                       (const PrTile*)Bn->tiles.p, Bn->ntiles, (const uint16_t*)Bn->srcloc.p, c_in,
+                                       Bn->msg.p, r_in, (const PrBlock*)L.d_blocks.p, nblk, L.partials.p);
+                    MDS_LAUNCHED();
+                }
+                if (!fold) {
+                    hipLaunchKernelGGL(k_pr_base, dim3(base_wgs), dim3(256), 0, s, (const float*)L.partials.p,
+                                       (const PrPart*)L.d_parts.p, nparts, d_f, L.base.p);
+                    MDS_LAUNCHED();
+                }
+                const uint32_t cwg = Bn->nbins + (Bn->nhubs + kCombineWaves - 1) / kCombineWaves;
+                const size_t run_lds = (2 * (size_t)Bn->max_runs + 1) * 4;
+                const PrItem* hubs = Bn->items.p + (Bn->items.n - Bn->nhubs);
+                hipLaunchKernelGGL(fold ? k_pr_combine<true> : k_pr_combine<false>, dim3(cwg),
+                                   dim3(kCombineThreads), run_lds, s, (const PrBin*)Bn->bins.p, Bn->nbins,
+                                   (const PrItem*)Bn->items.p, hubs, Bn->nhubs, (const PrMsgRun*)Bn->runs.p,
+                                   Bn->max_runs, (const float*)Bn->msg.p, (const uint16_t*)Bn->slot.p, L.cptr.p,
+                                   L.crow.p, c_in, L.outdeg.p, (const float*)L.base.p, (const float*)L.partials.p,
+                                   nblk, n, d_f, r_out, c_out);
+                MDS_LAUNCHED();
+                if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 2], s));
+                continue;
+            }
+            enqueue_base(r_in);
             if (step_push) {
                 hipLaunchKernelGGL(k_pr_scatter, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, n, L.optr.p,
                                    L.oslot.p, c_in, L.msg.p);
@@ -442,7 +753,6 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
                                    L.outdeg.p, (const float*)L.base.p, d_f, r_out, c_out);
             MDS_LAUNCHED();
             if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 2], s));
-            if (t + 1 < steps) enqueue_base(r_out);
         }
     };
 
@@ -502,10 +812,12 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     const uint64_t push_bytes = 4ull * m + 8ull * n + 8ull * n + 4ull * (n - nd);
     const uint64_t scatter_bytes = 4ull * (n + 1) + 8ull * m + 4ull * n;
     for (uint32_t t = 0; t < steps; ++t) {
+        // the binned variant computes the same superstep as the pull and is costed by the same model
         const bool p = var[t] == MEDUSA_PR_PUSH;
         st.kernel_bytes += p ? push_bytes : pull_bytes;
         st.alg_bytes += (p ? push_bytes + scatter_bytes : pull_bytes) + 4ull * nd;
-        if (p) ++st.push_steps; else ++st.pull_steps;
+        if (p) ++st.push_steps;
+        else if (var[t] == MEDUSA_PR_PULL) ++st.pull_steps;
         if (timing) {
             float k_ms = 0, s_ms = 0;
             MDS_HIP(hipEventElapsedTime(&k_ms, g->events[3 * t + 1], g->events[3 * t + 2]));

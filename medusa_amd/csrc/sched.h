@@ -140,6 +140,9 @@ __host__ __device__ inline bool bfs_policy_consulted(bool cur_pull, double n_f, 
     return num * kPolicyBand > den && num < den * kPolicyBand;
 }
 
+// Largest bucket width a policy may pick: one below the value that turns buckets off.
+constexpr double kMaxWidth = 4294967294.0;
+
 struct Sched {
     enum Algo { kBfs, kSssp, kMbfs };
     // Direction-optimising thresholds (Beamer, Asanovic, Patterson, SC'12): pull once the
@@ -190,7 +193,8 @@ struct Sched {
         float best = 0.f;
         uint32_t pick = dflt;
         for (int k = -2; k <= 8; ++k) {
-            const double d = std::max(1.0, std::ldexp((double)wmean, k));
+            // clamped below the "no buckets" value, so the cast cannot wrap (wmean may reach 2^32-1)
+            const double d = std::min(kMaxWidth, std::max(1.0, std::ldexp((double)wmean, k)));
             policy_row(MEDUSA_POLICY_DELTA, rec, d, x);
             const float c = mlp_eval_host(policy_params(o.policy), x);
             if (k == -2 || c < best) {

@@ -48,7 +48,11 @@ constexpr uint32_t kStage = 2048;      // LDS staging entries per queue per work
 constexpr uint32_t kStageSlices = 4;   // a vertex with more slices goes straight to HBM
 
 __host__ __device__ inline uint32_t slices_of(uint32_t deg) { return deg <= kSlice ? 1u : (deg + kSlice - 1) / kSlice; }
-__host__ inline size_t near_capacity(uint32_t n, uint64_t m) { return (size_t)n + m / kSlice + 64; }
+// Entries one slot may append to a near queue.  A vertex enters once per kSlice of its
+// out-edges and at most twice per slot: a split slot stamps the vertices it moves with the
+// slot's epoch and the vertices its local rounds lower with the next one, so a vertex moved
+// by the split and lowered again by a local round is appended under both.
+__host__ inline size_t near_capacity(uint32_t n, uint64_t m) { return 2 * ((size_t)n + m / kSlice) + 64; }
 
 __global__ void k_sssp_init(uint32_t n, uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp, uint32_t src,
                             uint32_t src_slices, unsigned long long* __restrict__ near_q) {
@@ -454,7 +458,7 @@ constexpr uint32_t kSsspGrid = 2048;
 struct SsspCtl {
     uint32_t done, epoch, thr, old_thr, delta, n_near, n_far, split_pending;
     uint32_t overflow, phase, supersteps, local_rounds, mode, wmean, dflt_delta, near_sel;
-    uint32_t far_sel, local_cfg, trace, pol_far, pol_width, pad;
+    uint32_t far_sel, local_cfg, trace, pol_far, pol_width, lost;  // lost: near entries beyond the queue
     unsigned long long n, m, relaxed, kernel_bytes;
 };
 
@@ -492,7 +496,10 @@ __device__ uint32_t sssp_pick_delta(SsspCtl& c, uint32_t n_far, const float* __r
     const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.phase, (double)n_far, (double)c.wmean, (double)c.thr, 0.0,
                                              (double)c.n, (double)c.m, 0.0, 0.0};
     // row 0: the default width; rows 1..11: k = -2..8
-    auto width = [&](int r) { return r == 0 ? (double)c.dflt_delta : fmax(1.0, ldexp((double)c.wmean, r - 3)); };
+    // candidates clamped below the "no buckets" value, so the cast cannot wrap
+    auto width = [&](int r) {
+        return r == 0 ? (double)c.dflt_delta : fmin(kMaxWidth, fmax(1.0, ldexp((double)c.wmean, r - 3)));
+    };
     if (threadIdx.x < 12) delta_row(rec, width(threadIdx.x), m.x + threadIdx.x * MEDUSA_POLICY_INPUTS);
     mlp_eval_rows(policy, m.P, m.x, 12, m.a1, m.a2, m.out);
     int best = 1;
@@ -577,7 +584,8 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
 
 // One workgroup after the work kernel (its counters are visible at the kernel boundary).
 __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
-                                                     uint32_t far_cap, const float* __restrict__ policy,
+                                                     uint32_t far_cap, uint32_t near_cap,
+                                                     const float* __restrict__ policy,
                                                      unsigned long long* __restrict__ marks) {
     __shared__ MlpShared s_mlp;
     SsspDevCounters k;
@@ -588,6 +596,13 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
     k.relax = load_now(&ctr->relax);
     SsspCtl c = load_now_block(ctl);
     if (c.done) return;
+    if (k.n_near > near_cap) {
+        // entries past the queue were dropped by the writers (near_capacity rules this out):
+        // stop, and let the host report it, rather than read past the queue
+        k.n_near = near_cap;
+        c.lost = 1;
+        c.done = 1;
+    }
     uint32_t n_far;
     // phase mark to write (trace on): index, then its fields; ~0u = none
     uint32_t mark = 0xFFFFFFFFu, mark_phase = 0, mark_far = 0, mark_thr = 0, mark_delta = 0;
@@ -820,7 +835,8 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
             hipLaunchKernelGGL(local_cfg ? k_sssp_work<true> : k_sssp_work<false>, dim3(kSsspGrid), dim3(256), 0, s,
                                (const SsspCtl*)ctl, near0, near1, (uint32_t)near_cap, far0, far1, (uint32_t)far_cap,
                                g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, ctr);
-            hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap, policy, marks);
+            hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap,
+                               (uint32_t)near_cap, policy, marks);
         }
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
@@ -845,6 +861,7 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
         MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(SsspCtl), hipMemcpyDeviceToHost, s));
         MDS_HIP(hipStreamSynchronize(s));
         std::memcpy(&h, g->h_ctr_raw.p, sizeof(SsspCtl));
+        MDS_CHECK(!h.lost, MEDUSA_ERR_INTERNAL, "sssp: near queue overflow");
         if (h.done) break;
         MDS_CHECK(batch < max_batches, MEDUSA_ERR_INTERNAL, "sssp: superstep loop did not end");
     }
@@ -942,6 +959,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
                 const SsspCounters sc = read_counters<SsspCounters>(g);
                 std::swap(far_a, far_b);
                 n_near = sc.n_near;
+                MDS_CHECK(n_near <= near_cap, MEDUSA_ERR_INTERNAL, "sssp: near queue overflow");
                 far_incomplete = sc.n_far > far_cap;
                 n_far = std::min<uint32_t>(sc.n_far, (uint32_t)far_cap);
                 near_deg = sc.near_deg;
@@ -980,6 +998,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         ++st.push_steps;
         std::swap(near_cur, near_nxt);
         n_near = c.n_near;
+        MDS_CHECK(n_near <= near_cap, MEDUSA_ERR_INTERNAL, "sssp: near queue overflow");
         near_deg = c.near_deg;
         if (c.n_far > far_cap) {  // the far queue is full: keep what fits, rescan at the next split
             far_incomplete = true;

@@ -60,7 +60,7 @@ def test_packed_pagerank_per_graph_bits(oracle):
         begin = G.batch()
         assert len(begin) == len(graphs) + 1
         for steps in (1, 20):
-            for v in (md.PR_PULL, md.PR_PUSH):
+            for v in (md.PR_PULL, md.PR_PUSH, md.PR_BINNED):
                 for use_graph in (0, 1):
                     r, st = G.pagerank(0.85, steps, variant=v, use_graph=use_graph)
                     for k, (grp, gcol) in enumerate(graphs):

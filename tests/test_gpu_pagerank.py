@@ -28,7 +28,7 @@ def _bits(a):
     return np.asarray(a, np.float32).view(np.uint32)
 
 
-def _check(G, rp, col, oracle, steps=30, variants=(md.PR_PULL, md.PR_PUSH)):
+def _check(G, rp, col, oracle, steps=30, variants=(md.PR_PULL, md.PR_PUSH, md.PR_BINNED)):
     want32 = oracle.pagerank_f32(rp, col, 0.85, steps)
     want64 = oracle.pagerank_f64(rp, col, 0.85, steps)
     for v in variants:
@@ -92,6 +92,29 @@ def test_pagerank_edge_cases(oracle):
             G.pagerank(1.5, 3)
 
 
+def test_pagerank_binned_shapes(oracle):
+    """Shapes at the edges of the binned variant's plan: no edges at all (nothing to send),
+    every in-edge into one hub of in-degree above 4096 with nothing dangling (no message and no
+    dangling block, so the send launch is skipped), a hub plus ordinary vertices, and a graph
+    whose bins cut runs of equal in-degree."""
+    n = 5000
+    shapes = [csr_from_edges(6, np.zeros((0, 2), np.int64))]
+    shapes.append(csr_from_edges(n, np.stack([np.arange(n), np.zeros(n, np.int64)], 1)))
+    rng = np.random.default_rng(5)
+    hub = np.stack([rng.integers(0, n, 6000), np.full(6000, 7)], 1)
+    rest = np.stack([rng.integers(0, n, 20000), rng.integers(0, n, 20000)], 1)
+    shapes.append(csr_from_edges(n, np.concatenate([hub, rest])))
+    big = np.stack([rng.integers(0, 40000, 400000), rng.integers(0, 40000, 400000) % 2000], 1)
+    shapes.append(csr_from_edges(40000, big))
+    for rp, col, _ in shapes:
+        with md.Graph(rp, col) as G:
+            for steps in (1, 6):
+                want = oracle.pagerank_f32(rp, col, 0.85, steps)
+                for use_graph in (0, 1):
+                    r, _ = G.pagerank(0.85, steps, variant=md.PR_BINNED, use_graph=use_graph)
+                    assert np.array_equal(_bits(r), _bits(want)), (len(rp) - 1, steps, use_graph)
+
+
 def test_pagerank_damping_and_repeat(oracle):
     rp, col, _ = random_graph(8000, 50000, 4)
     with md.Graph(rp, col) as G:
@@ -112,7 +135,7 @@ def test_pagerank_er_config(oracle):
 def test_pagerank_rmat20_config(oracle):
     rp, col = wl.rmat_graph()
     with md.Graph(rp, col) as G:
-        r, r64 = _check(G, rp, col, oracle, steps=30, variants=(md.PR_PULL,))
+        r, r64 = _check(G, rp, col, oracle, steps=30, variants=(md.PR_PULL, md.PR_BINNED))
         assert abs(r.astype(np.float64).sum() - 1.0) < 1e-5
         # FP64 residual between the last two iterates is reported and small
         _, st = G.pagerank(0.85, 30)

@@ -6,7 +6,7 @@ import pytest
 
 import medusa_amd as md
 from medusa_amd import workloads as wl
-from conftest import random_graph
+from conftest import csr_from_edges, random_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -149,6 +149,25 @@ def test_learned_delta_sssp(oracle):
         assert np.array_equal(d_host, want)
         print(f"[sssp 700x700] plain: {st_plain.supersteps} supersteps {st_plain.device_ms:.2f} ms; "
               f"learned: {st_learn.supersteps} supersteps {st_learn.device_ms:.2f} ms", flush=True)
+
+
+def test_delta_policy_huge_mean_weight(oracle):
+    """Weights near 2^28 put the policy's candidate widths (mean weight x 2^k, k up to 8) past
+    2^32; they are clamped below the no-buckets value instead of wrapping to 0.  Untrained
+    policies pick arbitrary candidates; distances stay Dijkstra's on both drivers."""
+    rng = np.random.default_rng(28)
+    n = 3000
+    star = np.stack([np.zeros(n - 1, np.int64), np.arange(1, n)], 1)
+    rest = rng.integers(0, n, (12000, 2))
+    rp, col, _ = csr_from_edges(n, np.concatenate([star, rest]))
+    w = rng.integers(1 << 27, 1 << 28, len(col)).astype(np.uint32)
+    want = oracle.sssp_dijkstra(rp, col, w, 0)
+    with md.Graph(rp, col, w) as G:
+        for seed in range(3):
+            pol = md.Policy(md.POLICY_DELTA, seed=seed)
+            for timing in (0, 1):
+                d, _ = G.sssp(0, policy=pol, timing=timing)
+                assert np.array_equal(d, want), (seed, timing)
 
 
 def test_sssp_device_trace_marks_every_phase(oracle):

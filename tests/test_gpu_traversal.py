@@ -309,6 +309,22 @@ def test_sssp_sparse_power_law_local_rounds(oracle):
             assert st.reached == int((want != U32MAX).sum())
 
 
+def test_sssp_wide_bucket_split_moves_most_vertices(oracle):
+    """A sparse graph of more than 2^20 vertices with local rounds on and a bucket so wide that
+    the first split moves almost every vertex below the threshold: the split appends the
+    vertices it moves and its local rounds append the ones they lower again, so one slot may
+    append a vertex twice.  Distances stay Dijkstra's for several wide widths."""
+    rp, col = wl.chung_lu_graph(n=1_200_000, m=2_500_000, seed=29, max_degree=5_000)
+    w = wl.weighted(rp, col, seed=29)
+    src = wl.max_outdeg_vertex(rp)
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        for delta in (20_000, 200_000, 1 << 30):
+            d, st = G.sssp(src, delta=delta, local_rounds=64)
+            assert np.array_equal(d, want), delta
+            assert st.reached == int((want != U32MAX).sum())
+
+
 def test_sssp_hub_source_slices(oracle):
     """A source whose out-degree spans many near-queue slices, plus hubs inside the graph."""
     rng = np.random.default_rng(12)
