@@ -147,6 +147,18 @@ int medusa_bfs(medusa_graph* g, uint32_t source, int32_t* levels, const medusa_o
 int medusa_bfs_multi(medusa_graph* g, const uint32_t* sources, uint32_t count, int32_t* levels,
                      const medusa_opts* opts, medusa_stats* stats);
 
+/* BFS from one source with no device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  levels[n] must be device memory of the graph's
+ * GPU; the run writes it in place (the same levels as medusa_bfs, heuristic direction).  The
+ * frontier queues, bitmap and control block live in one statically sized device array per
+ * GPU, held until the process ends, so n may not exceed MEDUSA_NOALLOC_MAX_VERTICES
+ * (MEDUSA_ERR_ARG otherwise).  Every call on a GPU shares that array: calls that overlap on one
+ * GPU (from several threads or handles) race and must be serialised by the caller.  The graph
+ * handle itself is allocated by medusa_graph_create beforehand. */
+#define MEDUSA_NOALLOC_MAX_VERTICES (1u << 21)
+int medusa_bfs_noalloc(medusa_graph* g, uint32_t source, int32_t* levels, medusa_stats* stats);
+
 /* SSSP: replaces SSSP/UserOperators.h:200-264 (InitMessageBuffer(FLT_MAX), ssspEdgeProcAA
  * scatter, combineAllDevice = CUDPP segmented MIN scan, UpdateVertex) and its loop
  * SSSP/main.cu:369-382.  uint32 weights and distances; MEDUSA_DIST_UNREACHED unreached.

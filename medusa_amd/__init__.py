@@ -33,7 +33,7 @@ EXPORTED = [
     "medusa_gen_lattice", "medusa_gen_chung_lu", "medusa_gen_weights", "medusa_csr_from_arcs",
     "medusa_policy_create", "medusa_policy_destroy", "medusa_policy_params", "medusa_policy_train",
     "medusa_policy_predict", "medusa_policy_gradients", "medusa_policy_features", "medusa_graph_set_batch",
-    "medusa_graph_batch", "medusa_bfs_multi", "medusa_autotune",
+    "medusa_graph_batch", "medusa_bfs_multi", "medusa_autotune", "medusa_bfs_noalloc",
 ]
 TUNE_BFS, TUNE_MBFS = 1, 2
 POLICY_INPUTS = 16
@@ -150,6 +150,7 @@ def lib():
             "medusa_graph_batch": ([_vp, ctypes.POINTER(_u32), _vp, _u32], ctypes.c_int),
             "medusa_bfs_multi": ([_vp, _vp, _u32, _vp, ctypes.POINTER(Opts), ctypes.POINTER(Stats)], ctypes.c_int),
             "medusa_autotune": ([_vp, _u32, _vp, _u32, _u32, ctypes.POINTER(Tuning)], ctypes.c_int),
+            "medusa_bfs_noalloc": ([_vp, _u32, _vp, ctypes.POINTER(Stats)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -348,6 +349,12 @@ class Graph:
         o = _opts(**opts)
         _check(lib().medusa_bfs(self.handle, source, _ptr(out), ctypes.byref(o), ctypes.byref(st)))
         return out, st
+
+    def bfs_noalloc(self, source, out):
+        """BFS into `out` (a device tensor of n int32) with no device allocation; returns Stats."""
+        st = Stats()
+        _check(lib().medusa_bfs_noalloc(self.handle, source, _ptr(out), ctypes.byref(st)))
+        return st
 
     def bfs_multi(self, sources, out=None, **opts):
         """BFS from several sources at once (all at level 0); returns (levels, Stats)."""

@@ -566,7 +566,8 @@ static uint32_t bfs_mode(const medusa_opts& o) {
     return 0;
 }
 
-// Work buffers of one BFS run (the handle's own, allocated on its first BFS).
+// Work buffers of one BFS run: the handle's own (allocated on its first BFS), or, for
+// medusa_bfs_noalloc, the caller's level array and the library's static scratch.
 struct BfsBufs {
     int32_t* level;
     uint32_t *q_small, *q_large, *q_aux;
@@ -614,7 +615,7 @@ static void bfs_enqueue_slot(medusa_graph* g, const BfsBufs& B, uint32_t large, 
 
 // Device-driven run: seed, first decision, then batches of superstep slots until the control
 // block says done.  capture: the batches are captured once as hipGraphs and replayed; without
-// it the same slots are launched one by one.
+// it (medusa_bfs_noalloc, which may not allocate) the same slots are launched one by one.
 static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uint32_t single, int32_t* out,
                            const medusa_opts& o, medusa_stats& st, bool capture = true) {
     const uint32_t n = g->n;
@@ -801,6 +802,63 @@ static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, 
     if (stats) *stats = st;
 }
 
+// ---- BFS with no device allocation ------------------------------------------------
+// Work buffers of medusa_bfs_noalloc: two frontier queues of kNoAllocMaxVertices entries,
+// the frontier bitmap and a control block, in one array per GPU that lives as long as the
+// process.  Host code must not pass `scratch` itself to a kernel (in host code the name is
+// a stand-in, not a device address): the device address is looked up once per device with
+// hipGetSymbolAddress and kept in s_scratch.
+constexpr uint32_t kNoAllocMaxVertices = MEDUSA_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocQueueBytes = (size_t)kNoAllocMaxVertices * 4;
+constexpr size_t kNoAllocBitmapBytes = (size_t)kNoAllocMaxVertices / 8;
+constexpr size_t kNoAllocBytes = 2 * kNoAllocQueueBytes + kNoAllocBitmapBytes + sizeof(CtlStorage);
+static_assert(kNoAllocMaxVertices % 2048 == 0, "queue and bitmap regions stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex s_scratch_mu;
+static unsigned char* s_scratch[kMaxDevices] = {};
+
+static unsigned char* scratch_on(int device) {
+    MDS_CHECK(device >= 0 && device < kMaxDevices, MEDUSA_ERR_ARG, "bfs_noalloc: device %d out of range", device);
+    const std::lock_guard<std::mutex> lock(s_scratch_mu);
+    if (!s_scratch[device]) {
+        void* p = nullptr;
+        MDS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the calling thread has `device` current
+        MDS_CHECK(p != nullptr, MEDUSA_ERR_HIP, "bfs_noalloc: no address for the scratch array");
+        s_scratch[device] = static_cast<unsigned char*>(p);
+    }
+    return s_scratch[device];
+}
+
+static void bfs_noalloc(medusa_graph* g, uint32_t source, int32_t* levels, medusa_stats* stats) {
+    const auto lock = enter(g);  // makes the graph's device current
+    const uint32_t n = g->n;
+    MDS_CHECK(levels, MEDUSA_ERR_ARG, "bfs_noalloc: null output");
+    MDS_CHECK(source < n, MEDUSA_ERR_ARG, "bfs_noalloc: source %u out of range (n=%u)", source, n);
+    MDS_CHECK(n <= kNoAllocMaxVertices, MEDUSA_ERR_ARG,
+              "bfs_noalloc: %u vertices exceed the scratch limit of %u (MEDUSA_NOALLOC_MAX_VERTICES)", n,
+              kNoAllocMaxVertices);
+    hipPointerAttribute_t attr{};
+    const hipError_t e = hipPointerGetAttributes(&attr, levels);
+    (void)hipGetLastError();
+    MDS_CHECK(e == hipSuccess && attr.type == hipMemoryTypeDevice && attr.device == g->device, MEDUSA_ERR_ARG,
+              "bfs_noalloc: levels must be device memory of the graph's GPU");
+    unsigned char* base = scratch_on(g->device);
+    BfsBufs B{};
+    B.level = levels;
+    B.q_small = reinterpret_cast<uint32_t*>(base);
+    B.q_large = reinterpret_cast<uint32_t*>(base + kNoAllocQueueBytes);
+    B.bitmap = reinterpret_cast<unsigned long long*>(base + 2 * kNoAllocQueueBytes);
+    B.ctl = reinterpret_cast<BfsCtl*>(base + 2 * kNoAllocQueueBytes + kNoAllocBitmapBytes);
+    B.q_aux = nullptr;  // one source: not used
+    static_assert(sizeof(BfsCtl) <= sizeof(CtlStorage), "control block storage");
+    medusa_opts o{};  // heuristic direction, default or tuned launch shape
+    medusa_stats st{};
+    bfs_run_device(g, B, 1, source, levels, o, st, /*capture=*/false);
+    if (stats) *stats = st;
+}
+
 }  // namespace medusa
 
 using namespace medusa;
@@ -813,4 +871,8 @@ extern "C" int medusa_bfs(medusa_graph* g, uint32_t source, int32_t* levels, con
 extern "C" int medusa_bfs_multi(medusa_graph* g, const uint32_t* sources, uint32_t count, int32_t* levels,
                                 const medusa_opts* opts, medusa_stats* stats) {
     return guard([&] { bfs_entry(g, sources, count, levels, opts, stats); });
+}
+
+extern "C" int medusa_bfs_noalloc(medusa_graph* g, uint32_t source, int32_t* levels, medusa_stats* stats) {
+    return guard([&] { bfs_noalloc(g, source, levels, stats); });
 }
