@@ -151,8 +151,31 @@ __device__ __forceinline__ void lane_subtree_at(float& S, float& C, uint32_t r, 
     S = s[0];
     C = c[0];
 }
+// The canonical 64-slot tree over a lane's first L slots (L a power of two <= 16, deg <= L):
+// slot j absorbs slot j+off for off = L/2, ..., 1.  Every slot from deg on is (0,0), so the
+// 64-slot tree's levels above L/2 only merge (0,0) pairs, an exact identity, and this is the
+// whole sum.
+template <uint32_t L, class Load>
+__device__ __forceinline__ float lane_tree_small(uint32_t deg, Load load) {
+    float s[L], c[L];
+#pragma unroll
+    for (uint32_t j = 0; j < L; ++j) {
+        s[j] = j < deg ? load(j) : 0.f;
+        c[j] = 0.f;
+    }
+#pragma unroll
+    for (uint32_t off = L / 2; off >= 1; off >>= 1)
+#pragma unroll
+        for (uint32_t j = 0; j < off; ++j) kahan_combine(s[j], c[j], s[j + off], c[j + off]);
+    return s[0] - c[0];
+}
+
 template <class Load>
 __device__ __forceinline__ float lane_vertex_sum_at(uint32_t deg, Load load) {
+    if (deg <= 1) return lane_tree_small<1>(deg, load);
+    if (deg <= 2) return lane_tree_small<2>(deg, load);
+    if (deg <= 4) return lane_tree_small<4>(deg, load);
+    if (deg <= 8) return lane_tree_small<8>(deg, load);
     // the 1, 2 or 4 interleaved subtrees one at a time (a loop, so only one subtree's leaves
     // are live), then merged as the 64-slot tree merges them
     const uint32_t st = deg <= 16 ? 1u : deg <= 32 ? 2u : 4u;
@@ -356,20 +379,21 @@ __device__ __forceinline__ uint32_t xcd_major(uint32_t b, uint32_t count) {
     return x * q + min(x, r) + j;
 }
 
-// Binned send (one launch per superstep).  Workgroups [0, ntiles) each copy their chunk's
-// contributions into LDS and write one tile of the message array (contiguous; runs are padded
-// to 4 entries, so every access is 16-byte aligned).  Tiles of one chunk run on one XCD, so the
-// chunk's contributions are read from HBM once per XCD.  Workgroups from ntiles on write the
-// (s, c) pairs of the dangling blocks of `rank` (4 blocks each), which the combine of the same
-// superstep merges into its base.
+// Binned send (one launch per superstep).  Workgroups [0, ntiles) each compute their chunk's
+// contributions c = r / outdeg into LDS and write one tile of the message array (contiguous;
+// runs are padded to 4 entries, so every access is 16-byte aligned).  Tiles of one chunk run on
+// one XCD, so the chunk's ranks are read from HBM once per XCD.
+// Workgroups from ntiles on write the (s, c) pairs of the dangling blocks of `rank` (4 blocks
+// each), which the combine of the same superstep merges into its base.
 __global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tiles, uint32_t ntiles,
-                                                 const uint16_t* __restrict__ srcloc, const float* __restrict__ contrib,
-                                                 float* __restrict__ msg, const float* __restrict__ rank,
-                                                 const PrBlock* __restrict__ blocks, uint32_t nblk,
-                                                 float* __restrict__ partials) {
+                                                 const uint16_t* __restrict__ srcloc, const float* __restrict__ rank,
+                                                 const uint32_t* __restrict__ outdeg, float* __restrict__ msg,
+                                                 const PrBlock* __restrict__ blocks,
+                                                 uint32_t nblk, float* __restrict__ partials) {
     __shared__ float s_c[kPrChunkSources];
     using u16x4 = unsigned short __attribute__((ext_vector_type(4)));
     using f32x4 = float __attribute__((ext_vector_type(4)));
+    using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
     if (blockIdx.x >= ntiles) {
         const uint32_t lane = threadIdx.x & (kWave - 1);
         const uint32_t b = (blockIdx.x - ntiles) * 4 + threadIdx.x / kWave;
@@ -383,29 +407,43 @@ __global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tile
         return;
     }
     const PrTile T = tiles[xcd_major(blockIdx.x, ntiles)];
-    {  // chunks start on multiples of kPrChunkSources, so the slice is 16-byte aligned
-        const f32x4* c4 = reinterpret_cast<const f32x4*>(contrib + T.src_begin);
-        f32x4* s4 = reinterpret_cast<f32x4*>(s_c);
-        const uint32_t n4 = T.src_count / 4;
-        for (uint32_t i = threadIdx.x; i < n4; i += 256) s4[i] = c4[i];
-        for (uint32_t i = 4 * n4 + threadIdx.x; i < T.src_count; i += 256) s_c[i] = contrib[T.src_begin + i];
-    }
-    __syncthreads();
     const u16x4* __restrict__ sl = reinterpret_cast<const u16x4*>(srcloc + T.msg_begin);
     f32x4* __restrict__ out = reinterpret_cast<f32x4*>(msg + T.msg_begin);
     const uint32_t nq = (T.msg_end - T.msg_begin) / 4;
-    for (uint32_t q0 = 0; q0 < nq; q0 += 4 * 256) {
-        u16x4 a[4];
+    auto fetch = [&](uint32_t q0, u16x4(&a)[4]) {
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
             const uint32_t q = q0 + u * 256 + threadIdx.x;
             a[u] = q < nq ? sl[q] : u16x4{0, 0, 0, 0};
         }
+    };
+    u16x4 a[4];
+    fetch(0, a);  // in flight while the chunk's contributions are computed
+    {  // chunks start on multiples of kPrChunkSources, so the slice is 16-byte aligned
+        auto contrib = [](float r, uint32_t od) { return od ? r / (float)od : 0.f; };
+        const f32x4* r4 = reinterpret_cast<const f32x4*>(rank + T.src_begin);
+        const u32x4* o4 = reinterpret_cast<const u32x4*>(outdeg + T.src_begin);
+        f32x4* s4 = reinterpret_cast<f32x4*>(s_c);
+        const uint32_t n4 = T.src_count / 4;
+        for (uint32_t i = threadIdx.x; i < n4; i += 256) {
+            const f32x4 r = r4[i];
+            const u32x4 o = o4[i];
+            s4[i] = f32x4{contrib(r.x, o.x), contrib(r.y, o.y), contrib(r.z, o.z), contrib(r.w, o.w)};
+        }
+        for (uint32_t i = 4 * n4 + threadIdx.x; i < T.src_count; i += 256)
+            s_c[i] = contrib(rank[T.src_begin + i], outdeg[T.src_begin + i]);
+    }
+    __syncthreads();
+    for (uint32_t q0 = 0; q0 < nq; q0 += 4 * 256) {
+        u16x4 nxt[4];
+        fetch(q0 + 4 * 256, nxt);  // the next round's offsets fly while this round writes
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
             const uint32_t q = q0 + u * 256 + threadIdx.x;
             if (q < nq) out[q] = f32x4{s_c[a[u].x], s_c[a[u].y], s_c[a[u].z], s_c[a[u].w]};
         }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) a[u] = nxt[u];
     }
 }
 
@@ -424,17 +462,18 @@ constexpr uint32_t kCombineUnroll = 4;  // groups of four entries per thread in 
 //      vertex sums as many slots as the largest in-degree of its group: the slots past its own
 //      in-degree hold +0.0, which merges as the exact identity the canonical tree uses for
 //      absent elements, so no per-vertex degree is read.
-// Workgroups from nbins on sum the hub destinations (one wave each) by gathering contrib
-// over the CSC, as the pull variant does.  kFold: a single graph with at most 64 dangling
-// blocks, whose base every wave merges here from the send's block pairs; otherwise
-// base[part] was written by k_pr_base.
+// Workgroups from nbins on sum the hub destinations (one wave each) by gathering r / outdeg of
+// their sources over the CSC, with the same division the send does.  kFold: a single graph with at most 64
+// dangling blocks, whose base every wave merges here from the send's block pairs; otherwise
+// base[part] was written by k_pr_base.  kContrib: also write r / outdeg (the next superstep
+// runs another variant, which reads it).
 // Dynamic LDS after the slots: max_runs run starts and max_runs + 1 run numbers.
-template <bool kFold>
+template <bool kFold, bool kContrib>
 __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
     const PrBin* __restrict__ bins, uint32_t nbins, const PrItem* __restrict__ items,
     const PrItem* __restrict__ hubs, uint32_t nhubs, const PrMsgRun* __restrict__ runs, uint32_t max_runs,
     const float* __restrict__ msg, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ cptr,
-    const uint32_t* __restrict__ crow, const float* __restrict__ contrib_in, const uint32_t* __restrict__ outdeg,
+    const uint32_t* __restrict__ crow, const float* __restrict__ rank_in, const uint32_t* __restrict__ outdeg,
     const float* __restrict__ base_p, const float* __restrict__ partials, uint32_t nblk, uint32_t n, float d,
     float* __restrict__ rank_out, float* __restrict__ contrib_out) {
     __shared__ float lds[kPrBinValues + 4];  // + the dummy slot of padding entries
@@ -447,8 +486,10 @@ __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
     auto finish = [&](uint32_t p, float S, float base) {
         const float r = fmaf(d, S, base);
         rank_out[p] = r;
-        const uint32_t od = outdeg[p];
-        if (od) contrib_out[p] = r / (float)od;
+        if (kContrib) {
+            const uint32_t od = outdeg[p];
+            if (od) contrib_out[p] = r / (float)od;
+        }
     };
     if (blockIdx.x >= nbins) {
         const uint32_t h = (blockIdx.x - nbins) * kCombineWaves + wave;
@@ -456,7 +497,10 @@ __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
         const PrItem I = hubs[h];
         const float base = kFold ? pr_base_fold(partials, 0, nblk, n, d) : base_p[I.part];
         const uint32_t b = cptr[I.v], len = cptr[I.v + 1] - b;
-        const float S = wave_segment_sum(len, [&](uint32_t k) { return contrib_in[crow[b + k]]; });
+        const float S = wave_segment_sum(len, [&](uint32_t k) {
+            const uint32_t u = crow[b + k];
+            return rank_in[u] / (float)outdeg[u];  // u has an out-edge
+        });
         if (lane == 0) finish(I.v, S, base);
         return;
     }
@@ -484,47 +528,50 @@ __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
         s_rk[j] = R.k;
         if (j < nr) s_rm[j] = R.msg;
     }
-    {
-        f32x4* z = reinterpret_cast<f32x4*>(lds);
-        for (uint32_t i = threadIdx.x; i < kPrBinValues / 4; i += kCombineThreads) z[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-    __syncthreads();
-    // 2. entries into their slots
+    __syncthreads();  // the run table is staged
+    // 2. entries into their slots: the first round's loads are issued before the slots are
+    // zeroed, so they fly during the zeroing and its barrier
     const u16x4* __restrict__ s4 = reinterpret_cast<const u16x4*>(slot + B.slot_begin);
     const f32x4* __restrict__ m4 = reinterpret_cast<const f32x4*>(msg);
     const uint32_t nq = s_rk[nr] / 4;
-    for (uint32_t q0 = 0; q0 < nq; q0 += kCombineThreads * kCombineUnroll) {
-        uint32_t at[kCombineUnroll];
+    f32x4 x[kCombineUnroll];
+    u16x4 t[kCombineUnroll];
+    auto fetch = [&](uint32_t q0) {
 #pragma unroll
         for (uint32_t u = 0; u < kCombineUnroll; ++u) {
-            const uint32_t k = 4 * (q0 + u * kCombineThreads + threadIdx.x);
+            const uint32_t q = q0 + u * kCombineThreads + threadIdx.x;
+            if (q >= nq) continue;
+            const uint32_t k = 4 * q;
             uint32_t lo = 0, hi = nr;  // s_rk[lo] <= k < s_rk[hi]
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (s_rk[mid] <= k) lo = mid;
                 else hi = mid;
             }
-            at[u] = (s_rm[lo] + (k - s_rk[lo])) / 4;
+            x[u] = m4[(s_rm[lo] + (k - s_rk[lo])) / 4];
+            t[u] = s4[q];
         }
-        f32x4 x[kCombineUnroll];
-        u16x4 t[kCombineUnroll];
+    };
+    auto drop = [&](uint32_t q0) {
 #pragma unroll
         for (uint32_t u = 0; u < kCombineUnroll; ++u) {
-            const uint32_t q = q0 + u * kCombineThreads + threadIdx.x;
-            if (q < nq) {
-                x[u] = m4[at[u]];
-                t[u] = s4[q];
-            }
+            if (q0 + u * kCombineThreads + threadIdx.x >= nq) continue;
+            lds[t[u].x] = x[u].x;
+            lds[t[u].y] = x[u].y;
+            lds[t[u].z] = x[u].z;
+            lds[t[u].w] = x[u].w;
         }
-#pragma unroll
-        for (uint32_t u = 0; u < kCombineUnroll; ++u) {
-            if (q0 + u * kCombineThreads + threadIdx.x < nq) {
-                lds[t[u].x] = x[u].x;
-                lds[t[u].y] = x[u].y;
-                lds[t[u].z] = x[u].z;
-                lds[t[u].w] = x[u].w;
-            }
-        }
+    };
+    fetch(0);
+    {
+        f32x4* z = reinterpret_cast<f32x4*>(lds);
+        for (uint32_t i = threadIdx.x; i < kPrBinValues / 4; i += kCombineThreads) z[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    __syncthreads();
+    drop(0);
+    for (uint32_t q0 = kCombineThreads * kCombineUnroll; q0 < nq; q0 += kCombineThreads * kCombineUnroll) {
+        fetch(q0);
+        drop(q0);
     }
     __syncthreads();
     // 3. sums
@@ -709,13 +756,15 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
             const bool step_push = var[t] == MEDUSA_PR_PUSH;
             if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t], s));
             if (var[t] == MEDUSA_PR_BINNED) {
-                // send: messages of c_t and dangling block pairs of r_t; combine: r_{t+1}
+                // send: the contributions of r_t as messages, and the dangling block pairs of r_t;
+                // combine: r_{t+1} (and c_{t+1} when the next superstep runs another variant)
                 const uint32_t dwg = (nblk + 3) / 4;
                 if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 1], s));
                 if (dwg + Bn->ntiles) {  // none when nothing dangles and every in-edge goes to a hub
                     hipLaunchKernelGGL(k_pr_send, dim3(Bn->ntiles + dwg), dim3(256), 0, s,
-                                       (const PrTile*)Bn->tiles.p, Bn->ntiles, (const uint16_t*)Bn->srcloc.p, c_in,
-                                       Bn->msg.p, r_in, (const PrBlock*)L.d_blocks.p, nblk, L.partials.p);
+                                       (const PrTile*)Bn->tiles.p, Bn->ntiles, (const uint16_t*)Bn->srcloc.p, r_in,
+                                       (const uint32_t*)L.outdeg.p, Bn->msg.p, (const PrBlock*)L.d_blocks.p, nblk,
+                                       L.partials.p);
                     MDS_LAUNCHED();
                 }
                 if (!fold) {
@@ -723,15 +772,17 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
                                        (const PrPart*)L.d_parts.p, nparts, d_f, L.base.p);
                     MDS_LAUNCHED();
                 }
+                const bool keep = t + 1 < steps && var[t + 1] != MEDUSA_PR_BINNED;
                 const uint32_t cwg = Bn->nbins + (Bn->nhubs + kCombineWaves - 1) / kCombineWaves;
                 const size_t run_lds = (2 * (size_t)Bn->max_runs + 1) * 4;
                 const PrItem* hubs = Bn->items.p + (Bn->items.n - Bn->nhubs);
-                hipLaunchKernelGGL(fold ? k_pr_combine<true> : k_pr_combine<false>, dim3(cwg),
-                                   dim3(kCombineThreads), run_lds, s, (const PrBin*)Bn->bins.p, Bn->nbins,
-                                   (const PrItem*)Bn->items.p, hubs, Bn->nhubs, (const PrMsgRun*)Bn->runs.p,
-                                   Bn->max_runs, (const float*)Bn->msg.p, (const uint16_t*)Bn->slot.p, L.cptr.p,
-                                   L.crow.p, c_in, L.outdeg.p, (const float*)L.base.p, (const float*)L.partials.p,
-                                   nblk, n, d_f, r_out, c_out);
+                auto kern = fold ? (keep ? k_pr_combine<true, true> : k_pr_combine<true, false>)
+                                 : (keep ? k_pr_combine<false, true> : k_pr_combine<false, false>);
+                hipLaunchKernelGGL(kern, dim3(cwg), dim3(kCombineThreads), run_lds, s, (const PrBin*)Bn->bins.p,
+                                   Bn->nbins, (const PrItem*)Bn->items.p, hubs, Bn->nhubs,
+                                   (const PrMsgRun*)Bn->runs.p, Bn->max_runs, (const float*)Bn->msg.p,
+                                   (const uint16_t*)Bn->slot.p, L.cptr.p, L.crow.p, r_in, L.outdeg.p,
+                                   (const float*)L.base.p, (const float*)L.partials.p, nblk, n, d_f, r_out, c_out);
                 MDS_LAUNCHED();
                 if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 2], s));
                 continue;
