@@ -1,18 +1,21 @@
 """Benchmark: PageRank on an RMAT scale-20 graph (1,048,576 vertices, 16,777,216 arcs),
 30 supersteps, FP32 with compensated combine, on one MI355X.
 
-A step is one full PageRank run (30 supersteps) over the HBM-resident graph; the
-graph is built and uploaded before the timed region.  Prints one JSON line with
-GTEPS (30*m per run / time), the roofline of the dominant kernel (the pull
-superstep) measured with HIP events on the library's stream, and the reference
-CPU path (message-array push + per-vertex segment sum, OpenMP) timed on the
-host cores.
+A step is one full PageRank run (30 supersteps) over the HBM-resident graph; the graph is
+built and uploaded before the timed region.  Prints one JSON line with GTEPS (30*m per run /
+time), the roofline of the superstep (its kernels' algorithmic bytes over their time, HIP
+events on the library's stream), the parity of the ranks against the oracle, and the CPU path
+timed on the host cores.  The same line carries the other RMAT-20 targets measured on the same
+graph (BFS, SSSP, 64-source MBFS: GTEPS, roofline, learned or batched against the plain engine,
+identical results); `--no-extras` leaves them out, `--workload` measures one other
+configuration instead.
 """
 import argparse
 import json
 import os
 import pathlib
 import re
+import statistics
 import sys
 import time
 
@@ -29,13 +32,15 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-extras", action="store_true", help="headline only (no BFS/SSSP/MBFS keys)")
     p.add_argument("--scale", type=int, default=20)
     p.add_argument("--workload", default="pagerank_rmat20",
                    choices=["pagerank_rmat20", "bfs_er", "bfs_rmat20", "sssp_lattice", "mbfs_rmat20",
                             "chung_lu", "chung_lu_batch", "sssp_rmat20"],
                    help="pagerank_rmat20 is the headline line; the others measure the remaining configurations")
-    p.add_argument("--variant", default="pull", choices=["pull", "push", "binned"],
-                   help="PageRank superstep kernel variant (all give bit-identical ranks)")
+    p.add_argument("--variant", default="default", choices=["default", "pull", "push", "binned"],
+                   help="PageRank superstep kernel variant (all give bit-identical ranks); default = the "
+                        "engine's choice (binned)")
     p.add_argument("--profile", action="store_true",
                    help="short run for hardware counters: 3 PageRank runs (non-graph launches) and one "
                         "256 MiB device copy as the byte-count calibration; prints no JSON")
@@ -55,13 +60,150 @@ def profile_pass(G, out, torch, variant):
     print("profile pass done: 90 supersteps, copy of 268435456 bytes", flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the pull kernel from the committed rocprofv3 PMC summary."""
-    f = ROOT / "profiles" / "pagerank_pull_rmat20_traffic.md"
+def traffic_from_profiles(name):
+    """HBM bytes per superstep from a committed rocprofv3 PMC summary, or None."""
+    f = ROOT / "profiles" / name
     if not f.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
     return float(m.group(1)) if m else None
+
+
+PR_KERNELS = {
+    "default": ("k_pr_send + k_pr_combine (binned superstep)", "pagerank_binned_rmat20_traffic.md"),
+    "binned": ("k_pr_send + k_pr_combine (binned superstep)", "pagerank_binned_rmat20_traffic.md"),
+    "pull": ("k_pr_main<false> (pull superstep)", "pagerank_pull_rmat20_traffic.md"),
+    "push": ("k_pr_scatter + k_pr_main<true> (push superstep)", None),
+}
+
+
+def _pr_time(G, out, var, steps, warmup, torch):
+    for _ in range(max(1, warmup)):
+        G.pagerank(0.85, STEPS_PR, out=out, use_graph=1, variant=var)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for _ in range(steps):
+        _, st = G.pagerank(0.85, STEPS_PR, out=out, use_graph=1, variant=var)
+        dev_ms += st.device_ms
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, dev_ms, st
+
+
+def _pr_roofline(G, out, var, name):
+    """HIP events around every superstep's kernels (send to combine, or the pull kernel) on the
+    library's stream; bytes = the algorithmic model 4(n+1) + 8m + 16n per superstep."""
+    kms, launches, kbytes = 0.0, 0, 0
+    for _ in range(3):
+        _, st = G.pagerank(0.85, STEPS_PR, out=out, timing=1, variant=var)
+        kms += st.kernel_ms
+        launches += st.kernel_launches
+        kbytes += st.kernel_bytes
+    avg_s = kms / launches / 1e3
+    per = kbytes / launches
+    achieved = per / avg_s / 1e9
+    kernel, traffic = PR_KERNELS[name]
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(traffic) if traffic else None, "kernel": kernel,
+            "kernel_avg_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": int(per),
+            "launch": "one superstep (every kernel of it, with the gaps between them)"}
+
+
+def _pr_cpu(rp, col, m, np):
+    """The build's formula on the CPU (the oracle's FP32 restatement, OpenMP, all host cores):
+    median of 5 whole 30-superstep runs, minus the median of its set-up alone (CSC build), and
+    the reference formula (message-array push + segment sums) beside it."""
+    import oracle
+
+    threads = _cpu_threads()
+    setup = statistics.median(_timed_call(lambda: oracle.pagerank_f32(rp, col, 0.85, 0, threads)) for _ in range(3))
+    runs = [_timed_call(lambda: oracle.pagerank_f32(rp, col, 0.85, STEPS_PR, threads)) for _ in range(5)]
+    t_f32 = max(1e-9, statistics.median(runs) - setup)
+    ref_setup = statistics.median(_timed_call(lambda: oracle.pagerank_ref(rp, col, 0, threads)) for _ in range(3))
+    ref = [_timed_call(lambda: oracle.pagerank_ref(rp, col, STEPS_PR, threads)) for _ in range(5)]
+    t_ref = max(1e-9, statistics.median(ref) - ref_setup)
+    return {"value": round(STEPS_PR * m / t_f32 / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(),
+            "sample": f"the build's formula (r0 = 1/n, dangling mass redistributed, FP32 compensated sums in "
+                      f"the canonical order; oracle restatement, OpenMP) on the same RMAT-20 graph: median of "
+                      f"5 whole 30-superstep runs ({statistics.median(runs):.2f} s) minus the median set-up "
+                      f"({setup:.2f} s)",
+            "reference_formula_gteps": round(STEPS_PR * m / t_ref / 1e9, 4),
+            "reference_formula_sample": "reference CPU path restated (init 1.0, 0.15 + 0.85 sum, message-array "
+                                        "push + per-vertex segment sum, OpenMP): median of 5 whole 30-superstep "
+                                        "runs minus set-up"}
+
+
+def _timed_call(fn):
+    c0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - c0
+
+
+def headline(a, md, wl, torch, np):
+    rp, col = wl.rmat_graph(scale=a.scale, edge_factor=16, seed=2)
+    n, m = len(rp) - 1, len(col)
+    G = md.Graph(rp, col)
+    var = {"default": md.PR_DEFAULT, "pull": md.PR_PULL, "push": md.PR_PUSH, "binned": md.PR_BINNED}[a.variant]
+    out = torch.empty(n, dtype=torch.float32, device="cuda")
+    if a.profile:
+        profile_pass(G, out, torch, var)
+        G.close()
+        return None, None, None
+    wall, dev_ms, st = _pr_time(G, out, var, a.steps, a.warmup, torch)
+    ms_per_step = wall / a.steps * 1e3
+    gteps = a.steps * STEPS_PR * m / wall / 1e9
+    ranks = out.cpu().numpy().copy()
+    roofline = _pr_roofline(G, out, var, a.variant)
+    extra = {}
+    if a.variant == "default":  # the pull variant beside it, same bits
+        wp, _, _ = _pr_time(G, out, md.PR_PULL, max(1, a.steps // 2), 1, torch)
+        extra["gteps_pull_variant"] = round(max(1, a.steps // 2) * STEPS_PR * m / wp / 1e9, 3)
+        extra["pull_variant_bits_identical"] = bool(np.array_equal(out.cpu().numpy().view(np.uint32),
+                                                                   ranks.view(np.uint32)))
+        extra["roofline_pull_variant"] = _pr_roofline(G, out, md.PR_PULL, "pull")
+    cpu, parity = None, None
+    if not a.no_cpu:
+        import oracle
+
+        cpu = _pr_cpu(rp, col, m, np)
+        want32 = oracle.pagerank_f32(rp, col, 0.85, STEPS_PR)
+        want64 = oracle.pagerank_f64(rp, col, 0.85, STEPS_PR)
+        err = np.abs(ranks.astype(np.float64) - want64)
+        parity = {"bits_equal_fp32_oracle": bool(np.array_equal(ranks.view(np.uint32), want32.view(np.uint32))),
+                  "l1_vs_fp64": float(err.sum()), "max_abs_vs_fp64": float(err.max()), "l1_tolerance": 1e-6}
+    line = {
+        "metric": "GTEPS",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pagerank_rmat{a.scale}",
+            "vertices": n,
+            "arcs": m,
+            "supersteps": STEPS_PR,
+            "damping": 0.85,
+            "variant": a.variant,
+            "graph": "RMAT a/b/c/d=0.45/0.15/0.15/0.25, +-10% per-level noise, seed 2",
+        },
+        "device_ms_per_step": round(dev_ms / a.steps, 4),
+        "hbm_gbs_alg": round(st.alg_bytes / (dev_ms / a.steps / 1e3) / 1e9, 1) if dev_ms else None,
+        "rank_sum": round(float(ranks.astype(np.float64).sum()), 7),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    line.update(extra)
+    return line, G, (rp, col)
 
 
 def main():
@@ -82,105 +224,30 @@ def main():
     if a.workload != "pagerank_rmat20":
         run_workload(a, md, wl, torch, np)
         return
-    rp, col = wl.rmat_graph(scale=a.scale, edge_factor=16, seed=2)
-    n, m = len(rp) - 1, len(col)
-    G = md.Graph(rp, col)
-    var = {"pull": md.PR_PULL, "push": md.PR_PUSH, "binned": md.PR_BINNED}[a.variant]
-    out = torch.empty(n, dtype=torch.float32, device="cuda")
-    if a.profile:
-        profile_pass(G, out, torch, var)
-        G.close()
+    line, G, graph = headline(a, md, wl, torch, np)
+    if line is None:
         return
-
-    # warmup (also builds the pull layout and the captured superstep graph)
-    for _ in range(max(1, a.warmup)):
-        G.pagerank(0.85, STEPS_PR, out=out, use_graph=1, variant=var)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    dev_ms = 0.0
-    for _ in range(a.steps):
-        _, st = G.pagerank(0.85, STEPS_PR, out=out, use_graph=1, variant=var)
-        dev_ms += st.device_ms
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    ms_per_step = wall / a.steps * 1e3
-    gteps = a.steps * STEPS_PR * m / wall / 1e9
-    rank_sum = float(out.double().sum().item())
-
-    # roofline: HIP events around every pull-kernel launch on the library's stream
-    kms, launches, kbytes = 0.0, 0, 0
-    for _ in range(3):
-        _, st = G.pagerank(0.85, STEPS_PR, out=out, timing=1, variant=var)
-        kms += st.kernel_ms
-        launches += st.kernel_launches
-        kbytes += st.kernel_bytes
-    avg_launch_s = kms / launches / 1e3
-    bytes_per_launch = kbytes / launches
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles(),
-        "kernel": "k_pr_main<false> (pull superstep)",
-        "kernel_avg_us": round(avg_launch_s * 1e6, 2),
-        "alg_bytes_per_launch": int(bytes_per_launch),
-    }
-
-    cpu = None
-    if not a.no_cpu:
-        import oracle
-
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        steps_cpu = 30  # whole runs of the measured workload, repeated for at least 10 s
-        oracle.pagerank_ref(rp, col, 1, threads)  # first touch
-        reps, tc = 0, 0.0
-        while reps < 64 and tc < 10.0:
-            c0 = time.perf_counter()
-            oracle.pagerank_ref(rp, col, steps_cpu, threads)
-            tc += time.perf_counter() - c0
-            reps += 1
-        cpu = {
-            "value": round(reps * steps_cpu * m / tc / 1e9, 4),
-            "unit": "GTEPS",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"reference CPU path restated (message-array push + per-vertex segment sum, OpenMP), "
-                      f"same RMAT-20 graph, {reps} full 30-superstep runs ({tc:.1f} s)",
+    if not a.no_extras and a.scale == 20:
+        rp, col = graph
+        G.close()  # the other engines get their own handle of the same graph
+        steps = max(3, a.steps // 2)
+        line["bfs_rmat20"] = bfs_block(md, wl, torch, np, rp, col, wl.max_outdeg_vertex(rp), steps, 2, oracle_check=True)
+        line["sssp_rmat20"] = sssp_block(md, wl, torch, np, rp, col, wl.weighted(rp, col, seed=6),
+                                         wl.max_outdeg_vertex(rp), "sssp_rmat20", steps, 2, oracle_check=True)
+        line["mbfs_rmat20"] = mbfs_block(md, wl, torch, np, rp, col, wl.mbfs_sources(rp, 64, seed=4), steps, 2)
+        line["learned_batched_vs_plain"] = {
+            "bfs_learned_over_plain": line["bfs_rmat20"]["learned_over_plain"],
+            "sssp_learned_over_plain": line["sssp_rmat20"]["learned_over_plain"],
+            "mbfs_batched_over_plain": line["mbfs_rmat20"]["batched_over_plain"],
+            "mbfs_learned_over_heuristic": line["mbfs_rmat20"]["learned_over_heuristic"],
+            "results_identical": bool(line["bfs_rmat20"]["levels_identical_to_plain"]
+                                      and line["sssp_rmat20"]["distances_identical_to_plain"]
+                                      and line["mbfs_rmat20"]["levels_identical_learned_vs_heuristic"]
+                                      and line["mbfs_rmat20"]["levels_identical_to_64_bfs"]),
         }
-
-    line = {
-        "metric": "GTEPS",
-        "value": round(gteps, 3),
-        "unit": "GTEPS",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"pagerank_rmat{a.scale}",
-            "vertices": n,
-            "arcs": m,
-            "supersteps": STEPS_PR,
-            "damping": 0.85,
-            "graph": "RMAT a/b/c/d=0.45/0.15/0.15/0.25, +-10% per-level noise, seed 2",
-        },
-        "device_ms_per_step": round(dev_ms / a.steps, 4),
-        "hbm_gbs_alg": round(st.alg_bytes / (dev_ms / a.steps / 1e3) / 1e9, 1) if dev_ms else None,
-        "rank_sum": round(rank_sum, 7),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    else:
+        G.close()
     print(json.dumps(line), flush=True)
-    G.close()
 
 
 def _cpu_threads():
@@ -292,6 +359,103 @@ def _train_delta_policy(md, wl, np, family):
     pol = md.Policy(md.POLICY_DELTA, seed=3)
     pol.train(np.concatenate(Xs), np.concatenate(ys), iters=3000, lr=3e-3)
     return pol
+
+
+def bfs_block(md, wl, torch, np, rp, col, src, steps, warmup, oracle_check=False):
+    """BFS from `src` on a graph: plain engine (heuristic direction) and the learned direction
+    policy, same levels; roofline of the expansion supersteps (host-driven timing runs)."""
+    n = len(rp) - 1
+    G = md.Graph(rp, col)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    wall, res = _timed(lambda: G.bfs(src, out=out)[1], steps, warmup, torch)
+    m_r = res[-1].edges_traversed
+    plain = out.cpu().numpy().copy()
+    pol = _train_direction_policy(md, wl, np)
+    wl_, rl_ = _timed(lambda: G.bfs(src, out=out, policy=pol)[1], steps, 1, torch)
+    d = {"gteps": round(steps * m_r / wall / 1e9, 3), "ms": round(wall / steps * 1e3, 4), "source": src,
+         "supersteps": res[-1].supersteps, "push_pull_steps": [res[-1].push_steps, res[-1].pull_steps],
+         "edges_reached": m_r,
+         "roofline": _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)],
+                               "k_bfs_expand_dev (push or pull superstep)"),
+         "gteps_learned": round(steps * m_r / wl_ / 1e9, 3), "learned_over_plain": round(wall / wl_, 3),
+         "levels_identical_to_plain": bool(np.array_equal(plain, out.cpu().numpy()))}
+    if oracle_check:
+        import oracle
+
+        d["levels_equal_oracle"] = bool(np.array_equal(plain, oracle.bfs_queue(rp, col, src)))
+    pol.close()
+    G.close()
+    return d
+
+
+def sssp_block(md, wl, torch, np, rp, col, wt, src, family, steps, warmup, oracle_check=False):
+    """SSSP from `src`: plain engine (default width) and the learned width policy, same distances;
+    roofline of the relaxation supersteps (host-driven timing runs of the plain engine)."""
+    n = len(rp) - 1
+    G = md.Graph(rp, col, wt)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    wp, rp_ = _timed(lambda: G.sssp(src, out=out)[1], steps, warmup, torch)
+    m_r = rp_[-1].edges_traversed
+    plain = out.cpu().numpy().copy()
+    pol = _train_delta_policy(md, wl, np, family)
+    wl_, rl_ = _timed(lambda: G.sssp(src, out=out, policy=pol)[1], steps, 1, torch)
+    d = {"gteps": round(steps * m_r / wp / 1e9, 4), "ms": round(wp / steps * 1e3, 4), "source": src,
+         "supersteps": rp_[-1].supersteps, "edges_reached": m_r,
+         "roofline": _roofline([G.sssp(src, out=out, timing=1)[1] for _ in range(2)],
+                               "k_sssp_relax (relaxation superstep)"),
+         "gteps_learned": round(steps * m_r / wl_ / 1e9, 4), "supersteps_learned": rl_[-1].supersteps,
+         "learned_over_plain": round(wp / wl_, 3),
+         "distances_identical_to_plain": bool(np.array_equal(plain, out.cpu().numpy()))}
+    if oracle_check:
+        import oracle
+
+        d["distances_equal_dijkstra"] = bool(np.array_equal(plain.view(np.uint32),
+                                                            oracle.sssp_dijkstra(rp, col, wt, src)))
+    pol.close()
+    G.close()
+    return d
+
+
+def mbfs_block(md, wl, torch, np, rp, col, srcs, steps, warmup):
+    """64-source BFS as 64-bit masks (learned direction and heuristic) against 64 plain BFS runs
+    of the single-source engine; levels identical in every form."""
+    n = len(rp) - 1
+    G = md.Graph(rp, col)
+    out = torch.empty(len(srcs) * n, dtype=torch.uint8, device="cuda")
+    wh, rh = _timed(lambda: G.mbfs(srcs, out=out)[1], steps, warmup, torch)
+    m_r = rh[-1].edges_traversed
+    heur = out.cpu().numpy().copy()
+    pol = _train_mbfs_policy(md, wl, np)
+    wl_, rl_ = _timed(lambda: G.mbfs(srcs, out=out, policy=pol)[1], steps, 1, torch)
+    same = bool(np.array_equal(heur, out.cpu().numpy()))
+    lv = torch.empty(n, dtype=torch.int32, device="cuda")
+    G.bfs(int(srcs[0]), out=lv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_edges = 0
+    for s_ in srcs:
+        plain_edges += G.bfs(int(s_), out=lv)[1].edges_traversed
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    rows_equal = True
+    heur = heur.reshape(len(srcs), n)
+    for i, s_ in enumerate(srcs):
+        G.bfs(int(s_), out=lv)
+        x = lv.cpu().numpy()
+        want = np.where(x < 0, 255, np.minimum(x, 255)).astype(np.uint8)
+        rows_equal &= bool(np.array_equal(heur[i], want))
+    d = {"gteps": round(steps * m_r / wl_ / 1e9, 3), "ms": round(wl_ / steps * 1e3, 4), "sources": len(srcs),
+         "supersteps": rl_[-1].supersteps, "edges_reached_all_sources": m_r,
+         "roofline": _roofline([G.mbfs(srcs, out=out, policy=pol, timing=1)[1] for _ in range(3)],
+                               "k_mbfs_push / k_mbfs_pull (mask superstep)"),
+         "gteps_heuristic": round(steps * m_r / wh / 1e9, 3), "learned_over_heuristic": round(wh / wl_, 3),
+         "levels_identical_learned_vs_heuristic": same,
+         "plain_64_bfs_gteps": round(plain_edges / tp / 1e9, 3),
+         "batched_over_plain": round((steps * m_r / wl_) / (plain_edges / tp), 3),
+         "levels_identical_to_64_bfs": rows_equal}
+    pol.close()
+    G.close()
+    return d
 
 
 def run_workload(a, md, wl, torch, np):

@@ -52,10 +52,12 @@ extern "C" {
 #define MEDUSA_DIR_PULL 2  /* bottom-up over CSC every superstep */
 
 /* PageRank kernel variant (opts.variant); all variants give bit-identical ranks. */
-#define MEDUSA_PR_PULL 0   /* gather contributions over the in-edge (CSC) layout */
+#define MEDUSA_PR_DEFAULT 0 /* the engine's choice: MEDUSA_PR_BINNED, or MEDUSA_PR_PULL on a graph whose
+                               binned layout cannot be planned */
 #define MEDUSA_PR_PUSH 1   /* scatter into per-in-edge message slots, then combine (reference style) */
 #define MEDUSA_PR_BINNED 2 /* message array blocked by source chunk and destination bin: streaming
                               writes of one message per out-edge, then a combine from LDS per bin */
+#define MEDUSA_PR_PULL 3   /* gather contributions over the in-edge (CSC) layout */
 
 typedef struct medusa_graph medusa_graph;   /* opaque; owns all device memory of one graph */
 typedef struct medusa_policy medusa_policy; /* opaque; learned per-superstep scheduler */

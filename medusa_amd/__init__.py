@@ -13,14 +13,14 @@ import numpy as np
 __all__ = [
     "lib", "MedusaError", "Opts", "Stats", "Graph", "csr_from_arcs", "gen_erdos_renyi", "gen_rmat",
     "gen_lattice", "gen_chung_lu", "gen_weights", "device_count", "DIR_AUTO", "DIR_PUSH", "DIR_PULL",
-    "PR_PULL", "PR_PUSH", "PR_BINNED", "LEVEL_UNREACHED", "DIST_UNREACHED", "MBFS_UNREACHED", "TRACE_FIELDS",
+    "PR_DEFAULT", "PR_PULL", "PR_PUSH", "PR_BINNED", "LEVEL_UNREACHED", "DIST_UNREACHED", "MBFS_UNREACHED", "TRACE_FIELDS",
 ]
 
 _HERE = pathlib.Path(__file__).resolve().parent
 _LIBPATH = _HERE / "libmedusa_amd.so"
 
 DIR_AUTO, DIR_PUSH, DIR_PULL = 0, 1, 2
-PR_PULL, PR_PUSH, PR_BINNED = 0, 1, 2
+PR_DEFAULT, PR_PUSH, PR_BINNED, PR_PULL = 0, 1, 2, 3
 LEVEL_UNREACHED = -1
 DIST_UNREACHED = 0xFFFFFFFF
 MBFS_UNREACHED = 0xFF

@@ -187,6 +187,7 @@ struct PrLayout {
     // host copies of cptr / crow / outdeg, from which the binned layout is built on first use
     std::vector<uint32_t> h_cptr, h_crow, h_od;
     std::unique_ptr<PrBins> bins;
+    bool bins_refused = false;  // the binned plan failed for this layout
     // work
     DBuf<float> rank, rank_prev, contrib0, contrib1, msg, base, unperm;
     DBuf<float> partials;   // (s, c) pairs of the dangling-sum blocks
@@ -310,7 +311,9 @@ inline uint32_t grid_for(uint64_t items, uint32_t per_wg, uint32_t cap = 8192) {
     return (uint32_t)g;
 }
 void build_pr_layout(medusa_graph* g);
-void build_pr_bins(medusa_graph* g);
+// Plans and uploads the binned PageRank layout (once per layout).  Returns false when the graph
+// cannot be planned; with `required` that is a MEDUSA_ERR_STATE error instead.
+bool build_pr_bins(medusa_graph* g, bool required);
 void build_pull_layout(medusa_graph* g);
 
 constexpr size_t kCounterWords = 256;  // 2 KiB

@@ -257,14 +257,19 @@ void build_pr_layout(medusa_graph* g) {
 
 // Binned message exchange of the PageRank layout: planned on the host from the internal CSC,
 // every index checked against the buffers it addresses, then uploaded.
-void build_pr_bins(medusa_graph* g) {
+bool build_pr_bins(medusa_graph* g, bool required) {
     build_pr_layout(g);
     PrLayout& L = *g->pr;
-    if (L.bins) return;
+    if (L.bins) return true;
+    if (L.bins_refused && !required) return false;
     PrBinsPlan P;
     const std::string err = plan_pr_bins(g->n, L.h_cptr.data(), L.h_crow.data(), g->parts.data(),
                                          (uint32_t)g->parts.size() - 1, P);
-    MDS_CHECK(err.empty(), MEDUSA_ERR_STATE, "%s", err.c_str());
+    if (!err.empty()) {
+        L.bins_refused = true;
+        MDS_CHECK(!required, MEDUSA_ERR_STATE, "%s", err.c_str());
+        return false;
+    }
     const std::string bad = check_pr_bins(g->n, L.h_cptr.data(), P);
     MDS_CHECK(bad.empty(), MEDUSA_ERR_INTERNAL, "binned PageRank plan: %s", bad.c_str());
     auto B = std::make_unique<PrBins>();
@@ -289,6 +294,7 @@ void build_pr_bins(medusa_graph* g) {
     B->msg.alloc(std::max<uint64_t>(P.nentry, 1));
     MDS_HIP(hipStreamSynchronize(s));
     L.bins = std::move(B);
+    return true;
 }
 
 }  // namespace medusa

@@ -379,13 +379,16 @@ __device__ __forceinline__ uint32_t xcd_major(uint32_t b, uint32_t count) {
     return x * q + min(x, r) + j;
 }
 
+constexpr uint32_t kSendThreads = 512;
+constexpr uint32_t kSendWaves = kSendThreads / kWave;
+
 // Binned send (one launch per superstep).  Workgroups [0, ntiles) each compute their chunk's
 // contributions c = r / outdeg into LDS and write one tile of the message array (contiguous;
 // runs are padded to 4 entries, so every access is 16-byte aligned).  Tiles of one chunk run on
 // one XCD, so the chunk's ranks are read from HBM once per XCD.
-// Workgroups from ntiles on write the (s, c) pairs of the dangling blocks of `rank` (4 blocks
-// each), which the combine of the same superstep merges into its base.
-__global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tiles, uint32_t ntiles,
+// Workgroups from ntiles on write the (s, c) pairs of the dangling blocks of `rank` (one block
+// per wave), which the combine of the same superstep merges into its base.
+__global__ __launch_bounds__(kSendThreads) void k_pr_send(const PrTile* __restrict__ tiles, uint32_t ntiles,
                                                  const uint16_t* __restrict__ srcloc, const float* __restrict__ rank,
                                                  const uint32_t* __restrict__ outdeg, float* __restrict__ msg,
                                                  const PrBlock* __restrict__ blocks,
@@ -396,7 +399,7 @@ __global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tile
     using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
     if (blockIdx.x >= ntiles) {
         const uint32_t lane = threadIdx.x & (kWave - 1);
-        const uint32_t b = (blockIdx.x - ntiles) * 4 + threadIdx.x / kWave;
+        const uint32_t b = (blockIdx.x - ntiles) * kSendWaves + threadIdx.x / kWave;
         if (b >= nblk) return;  // uniform across the wave
         float bs, bc;
         dangle_block_sum(rank, blocks[b], bs, bc);
@@ -413,7 +416,7 @@ __global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tile
     auto fetch = [&](uint32_t q0, u16x4(&a)[4]) {
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
-            const uint32_t q = q0 + u * 256 + threadIdx.x;
+            const uint32_t q = q0 + u * kSendThreads + threadIdx.x;
             a[u] = q < nq ? sl[q] : u16x4{0, 0, 0, 0};
         }
     };
@@ -425,21 +428,21 @@ __global__ __launch_bounds__(256) void k_pr_send(const PrTile* __restrict__ tile
         const u32x4* o4 = reinterpret_cast<const u32x4*>(outdeg + T.src_begin);
         f32x4* s4 = reinterpret_cast<f32x4*>(s_c);
         const uint32_t n4 = T.src_count / 4;
-        for (uint32_t i = threadIdx.x; i < n4; i += 256) {
+        for (uint32_t i = threadIdx.x; i < n4; i += kSendThreads) {
             const f32x4 r = r4[i];
             const u32x4 o = o4[i];
             s4[i] = f32x4{contrib(r.x, o.x), contrib(r.y, o.y), contrib(r.z, o.z), contrib(r.w, o.w)};
         }
-        for (uint32_t i = 4 * n4 + threadIdx.x; i < T.src_count; i += 256)
+        for (uint32_t i = 4 * n4 + threadIdx.x; i < T.src_count; i += kSendThreads)
             s_c[i] = contrib(rank[T.src_begin + i], outdeg[T.src_begin + i]);
     }
     __syncthreads();
-    for (uint32_t q0 = 0; q0 < nq; q0 += 4 * 256) {
+    for (uint32_t q0 = 0; q0 < nq; q0 += 4 * kSendThreads) {
         u16x4 nxt[4];
-        fetch(q0 + 4 * 256, nxt);  // the next round's offsets fly while this round writes
+        fetch(q0 + 4 * kSendThreads, nxt);  // the next round's offsets fly while this round writes
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
-            const uint32_t q = q0 + u * 256 + threadIdx.x;
+            const uint32_t q = q0 + u * kSendThreads + threadIdx.x;
             if (q < nq) out[q] = f32x4{s_c[a[u].x], s_c[a[u].y], s_c[a[u].z], s_c[a[u].w]};
         }
 #pragma unroll
@@ -680,8 +683,13 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
     const uint32_t nd = L.n_dangling;
     const uint32_t nblk = (uint32_t)L.blocks.size();
     const uint32_t nparts = (uint32_t)L.parts.size();
-    MDS_CHECK(o.variant == MEDUSA_PR_PULL || o.variant == MEDUSA_PR_PUSH || o.variant == MEDUSA_PR_BINNED,
+    MDS_CHECK(o.variant == MEDUSA_PR_DEFAULT || o.variant == MEDUSA_PR_PULL || o.variant == MEDUSA_PR_PUSH ||
+                  o.variant == MEDUSA_PR_BINNED,
               MEDUSA_ERR_ARG, "pagerank: unknown variant %d", o.variant);
+    // the engine's default: the binned exchange wherever its layout can be planned
+    const int dflt = o.variant != MEDUSA_PR_DEFAULT       ? o.variant
+                     : build_pr_bins(g, /*required=*/false) ? MEDUSA_PR_BINNED
+                                                            : MEDUSA_PR_PULL;
     // kernel variant of every superstep: the caller's, or the learned policy's per superstep
     // (it keeps the caller's unless it predicts a clear gain); the ranks are the same bits
     uint64_t wave_vertices = 0, lane_vertices = 0;
@@ -690,17 +698,18 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
         if (r.kind == kPrLane) lane_vertices += r.count;
     }
     Sched sched(o, Sched::kBfs);
+    const bool binned_ok = dflt == MEDUSA_PR_BINNED || (o.policy && build_pr_bins(g, /*required=*/false));
     std::vector<uint8_t> var(steps);
     uint64_t sig = 1469598103934665603ull;  // FNV-1a over the sequence, keys the captured graph
     for (uint32_t t = 0; t < steps; ++t) {
         const double rec[MEDUSA_TRACE_FIELDS] = {(double)t, (double)nd, (double)wave_vertices, (double)lane_vertices,
                                                  0.0, (double)n, (double)m, 0.0, 0.0};
-        var[t] = (uint8_t)sched.pr_variant(rec, o.variant);
+        var[t] = (uint8_t)sched.pr_variant(rec, dflt, binned_ok);
         sig = (sig ^ var[t]) * 1099511628211ull;
     }
     const bool push = std::find(var.begin(), var.end(), (uint8_t)MEDUSA_PR_PUSH) != var.end();
     const bool binned = std::find(var.begin(), var.end(), (uint8_t)MEDUSA_PR_BINNED) != var.end();
-    if (binned) build_pr_bins(g);
+    if (binned) build_pr_bins(g, /*required=*/true);
     PrBins* Bn = L.bins.get();
     const bool fold = nparts == 1 && nblk <= kFoldBlocks;
     L.rank.ensure(n);
@@ -758,10 +767,10 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
             if (var[t] == MEDUSA_PR_BINNED) {
                 // send: the contributions of r_t as messages, and the dangling block pairs of r_t;
                 // combine: r_{t+1} (and c_{t+1} when the next superstep runs another variant)
-                const uint32_t dwg = (nblk + 3) / 4;
+                const uint32_t dwg = (nblk + kSendWaves - 1) / kSendWaves;
                 if (with_events) MDS_HIP(hipEventRecord(g->events[3 * t + 1], s));
                 if (dwg + Bn->ntiles) {  // none when nothing dangles and every in-edge goes to a hub
-                    hipLaunchKernelGGL(k_pr_send, dim3(Bn->ntiles + dwg), dim3(256), 0, s,
+                    hipLaunchKernelGGL(k_pr_send, dim3(Bn->ntiles + dwg), dim3(kSendThreads), 0, s,
                                        (const PrTile*)Bn->tiles.p, Bn->ntiles, (const uint16_t*)Bn->srcloc.p, r_in,
                                        (const uint32_t*)L.outdeg.p, Bn->msg.p, (const PrBlock*)L.d_blocks.p, nblk,
                                        L.partials.p);

@@ -71,23 +71,28 @@ __host__ __device__ inline void delta_row(const double* r, double action, float*
 
 // Kernel-variant rows (PageRank record = step, dangling, wave-run vertices, lane-run vertices,
 // 0, n, m, ...): size, mean degree, the shares of dangling and high in-degree vertices, step,
-// and the candidate variant (0 pull, 1 push) with its interactions.
+// and the candidate variant (push, binned or pull) as indicators with their interactions.
 __host__ __device__ inline void variant_row(const double* r, double action, float* x) {
     for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
     const double n = r[5] > 1.0 ? r[5] : 1.0, m = r[6] > 1.0 ? r[6] : 1.0;
-    const float a = action != 0.0 ? 1.f : 0.f;
+    const float push = action == (double)MEDUSA_PR_PUSH ? 1.f : 0.f;
+    const float bin = action == (double)MEDUSA_PR_BINNED ? 1.f : 0.f;
     x[0] = lg1(n) / 32.f;
     x[1] = lg1(m / n) / 8.f;
     x[2] = (float)(r[1] / n);
     x[3] = (float)(r[2] / n);
     x[4] = (float)(r[3] / n);
     x[5] = (float)(r[0] < 64.0 ? r[0] : 64.0) / 64.f;
-    x[6] = a;
-    x[7] = a * x[0];
-    x[8] = a * x[1];
-    x[9] = a * x[2];
-    x[10] = a * x[3];
-    x[11] = 1.f - a;
+    x[6] = push;
+    x[7] = push * x[0];
+    x[8] = push * x[1];
+    x[9] = push * x[2];
+    x[10] = push * x[3];
+    x[11] = 1.f - push - bin;
+    x[12] = bin;
+    x[13] = bin * x[0];
+    x[14] = bin * x[1];
+    x[15] = bin * x[2];
 }
 
 #if defined(__HIPCC__)
@@ -205,17 +210,25 @@ struct Sched {
         return best + kWidthMargin < c_dflt ? pick : dflt;
     }
     // PageRank kernel variant for one superstep (record as in variant_row): the policy's
-    // cheaper variant if it predicts a clear gain over `dflt`, else `dflt`.
-    int pr_variant(const double* rec, int dflt) const {
+    // cheapest variant (the binned one only where its layout exists) if it predicts a clear gain
+    // over `dflt`, else `dflt`.
+    int pr_variant(const double* rec, int dflt, bool binned_ok) const {
         if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_VARIANT) return dflt;
         float x[MEDUSA_POLICY_INPUTS];
-        policy_row(MEDUSA_POLICY_VARIANT, rec, 0.0, x);
-        const float c_pull = mlp_eval_host(policy_params(o.policy), x);
-        policy_row(MEDUSA_POLICY_VARIANT, rec, 1.0, x);
-        const float c_push = mlp_eval_host(policy_params(o.policy), x);
-        const float c_dflt = dflt == MEDUSA_PR_PUSH ? c_push : c_pull;
-        const float c_other = dflt == MEDUSA_PR_PUSH ? c_pull : c_push;
-        return c_other + kPolicyMargin < c_dflt ? (dflt == MEDUSA_PR_PUSH ? MEDUSA_PR_PULL : MEDUSA_PR_PUSH) : dflt;
+        const int cand[3] = {MEDUSA_PR_PULL, MEDUSA_PR_PUSH, MEDUSA_PR_BINNED};
+        float cost[3];
+        for (int i = 0; i < 3; ++i) {
+            policy_row(MEDUSA_POLICY_VARIANT, rec, (double)cand[i], x);
+            cost[i] = mlp_eval_host(policy_params(o.policy), x);
+        }
+        int best = -1;
+        float c_dflt = 0.f;
+        for (int i = 0; i < 3; ++i) {
+            if (cand[i] == MEDUSA_PR_BINNED && !binned_ok) continue;
+            if (cand[i] == dflt) c_dflt = cost[i];
+            if (best < 0 || cost[i] < cost[best]) best = i;
+        }
+        return cost[best] + kPolicyMargin < c_dflt ? cand[best] : dflt;
     }
     static void to_record(const Frontier& f, double action, double kms, double* r) {
         r[0] = f.step;

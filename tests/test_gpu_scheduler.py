@@ -46,16 +46,20 @@ def test_training_reduces_loss(oracle):
     assert np.array_equal(pol.params, pol2.params)
 
 
-def _direction_samples(G, src):
-    """Per-superstep (record, push µs, pull µs) from a push-only and a pull-only traced run;
-    BFS visits the same frontier sequence in either direction."""
-    _, _ = G.bfs(src, direction=md.DIR_PUSH, timing=1, trace=1)
-    tp = G.trace()
-    _, _ = G.bfs(src, direction=md.DIR_PULL, timing=1, trace=1)
-    tl = G.trace()
-    assert len(tp) == len(tl)
-    assert np.array_equal(tp[:, :7], tl[:, :7])
-    return tp, tp[:, 8] * 1e3, tl[:, 8] * 1e3
+def _direction_samples(G, src, runs=1):
+    """Per-superstep (record, push µs, pull µs) from push-only and pull-only traced runs (the
+    median of `runs` runs each); BFS visits the same frontier sequence in either direction."""
+    tp, tl = [], []
+    for _ in range(runs):
+        G.bfs(src, direction=md.DIR_PUSH, timing=1, trace=1)
+        tp.append(G.trace())
+        G.bfs(src, direction=md.DIR_PULL, timing=1, trace=1)
+        tl.append(G.trace())
+    assert len(tp[0]) == len(tl[0])
+    assert np.array_equal(tp[0][:, :7], tl[0][:, :7])
+    push_us = np.median(np.stack([t[:, 8] for t in tp]), 0) * 1e3
+    pull_us = np.median(np.stack([t[:, 8] for t in tl]), 0) * 1e3
+    return tp[0], push_us, pull_us
 
 
 def _rows(records, action):
@@ -96,7 +100,7 @@ def test_learned_direction_bfs(oracle, direction_policy, which):
     want = oracle.bfs_queue(rp, col, src)
     pol = direction_policy
     with md.Graph(rp, col) as G:
-        rec, push_us, pull_us = _direction_samples(G, src)
+        rec, push_us, pull_us = _direction_samples(G, src, runs=3)
         # the heuristic's and the policy's choices on the same supersteps (host-driven, traced)
         lv_h, _ = G.bfs(src, trace=1)
         act_h = G.trace()[:, 7].astype(int)
@@ -114,7 +118,7 @@ def test_learned_direction_bfs(oracle, direction_policy, which):
         c_best = cost.min(1).sum()
         print(f"[{which}] modelled superstep time: heuristic {c_heur:.1f} us, learned {c_learn:.1f} us, "
               f"best {c_best:.1f} us; learned/heuristic speedup {c_heur / c_learn:.2f}x", flush=True)
-        assert c_learn <= 1.10 * c_heur
+        assert c_learn <= c_heur
 
 
 def _delta_samples(G, src, deltas):
@@ -211,8 +215,11 @@ def test_device_and_host_drivers_agree(oracle):
 
 
 def _variant_samples(G, steps):
+    """Per-superstep feature rows, log-cost targets and costs (µs) of each PageRank variant,
+    from one traced run per variant (every variant runs the same supersteps)."""
     rows, ys, cost = [], [], {}
-    for v in (md.PR_PULL, md.PR_PUSH):
+    for v in (md.PR_PULL, md.PR_PUSH, md.PR_BINNED):
+        G.pagerank(0.85, steps, variant=v)  # layouts built outside the traced run
         G.pagerank(0.85, steps, variant=v, trace=1)
         tr = G.trace()
         assert len(tr) == steps and np.all(tr[:, 7] == v)
@@ -223,39 +230,62 @@ def _variant_samples(G, steps):
     return rows, ys, cost
 
 
+def _variant_training_graphs():
+    g = [wl.rmat_graph(scale=sc, edge_factor=16, seed=60 + sc) for sc in (11, 12, 13, 14, 15, 17, 18, 19)]
+    g += [wl.rmat_graph(scale=17, edge_factor=4, seed=61), wl.rmat_graph(scale=15, edge_factor=64, seed=62),
+          wl.er_graph(2 ** 14, 2 ** 18, seed=63), wl.er_graph(2 ** 17, 2 ** 21, seed=64),
+          wl.chung_lu_graph(100_000, 210_000, seed=65), wl.chung_lu_graph(400_000, 840_000, seed=66)]
+    return g
+
+
+def _device_ms(G, **kw):
+    """Median of 7 device times of a 30-superstep run replayed as one captured graph."""
+    G.pagerank(0.85, 30, use_graph=1, **kw)
+    return float(np.median([G.pagerank(0.85, 30, use_graph=1, **kw)[1].device_ms for _ in range(7)]))
+
+
 def test_learned_pagerank_variant(oracle):
-    """Kernel variant per superstep (pull or push): the ranks are the same bits whatever the
-    policy picks; a policy trained on the real traces keeps the cheaper variant, and one
-    trained on swapped costs really switches to push."""
+    """Kernel variant per superstep, learned from the plain engine's per-superstep traces of other
+    graphs.  The plain engine runs one fixed variant (the binned exchange) everywhere; on small
+    and dense graphs its two launches cost more than the pull kernel, and on large sparse ones
+    less.  On held-out graphs the learned choice keeps the ranks' bits and never costs more than
+    the fixed variant (per-superstep costs measured on that graph), and on RMAT-16 it runs a
+    30-superstep PageRank at least 1.1x faster, timed on the GPU.  A policy trained on costs
+    that make push cheapest really switches to push."""
     Xs, ys = [], []
-    for sc in (12, 13, 14):
-        rp, col = wl.rmat_graph(scale=sc, edge_factor=16, seed=60 + sc)
+    for rp, col in _variant_training_graphs():
         with md.Graph(rp, col) as G:
-            x, y, _ = _variant_samples(G, 8)
+            x, y, _ = _variant_samples(G, 6)
             Xs += x
             ys += y
     X = np.array(Xs, np.float32)
     y = np.array(ys, np.float32)
     pol = md.Policy(md.POLICY_VARIANT, seed=4)
-    pol.train(X, y, iters=2000, lr=3e-3)
+    pol.train(X, y, iters=3000, lr=3e-3)
     swapped = md.Policy(md.POLICY_VARIANT, seed=4)
     is_push = X[:, 6] > 0.5
     swapped.train(X, np.where(is_push, y - 4.0, y + 4.0).astype(np.float32), iters=2000, lr=3e-3)
-    rp, col = wl.rmat_graph(scale=16, edge_factor=16, seed=77)
-    want32 = oracle.pagerank_f32(rp, col, 0.85, 12)
-    with md.Graph(rp, col) as G:
-        _, _, cost = _variant_samples(G, 12)
-        for p in (pol, swapped):
-            for use_graph in (0, 1):
-                r, st = G.pagerank(0.85, 12, policy=p, use_graph=use_graph)
-                assert np.array_equal(r.view(np.uint32), want32.view(np.uint32))
-        G.pagerank(0.85, 12, policy=pol, trace=1)
-        acts = G.trace()[:, 7].astype(int)
-        G.pagerank(0.85, 12, policy=swapped, trace=1)
-        acts_sw = G.trace()[:, 7].astype(int)
-        assert np.all(acts_sw == md.PR_PUSH)
-        c_plain = cost[md.PR_PULL].sum()
-        c_learn = sum(cost[a][i] for i, a in enumerate(acts))
-        print(f"[pagerank variant] modelled: pull-only {c_plain:.1f} us, learned {c_learn:.1f} us "
-              f"(push steps {int((acts == 1).sum())}/12)", flush=True)
-        assert c_learn <= 1.10 * c_plain
+    held_out = {"rmat16": wl.rmat_graph(scale=16, edge_factor=16, seed=77),
+                "chung_lu_300k": wl.chung_lu_graph(300_000, 630_000, seed=78),
+                "rmat18": wl.rmat_graph(scale=18, edge_factor=16, seed=79)}
+    for name, (rp, col) in held_out.items():
+        want32 = oracle.pagerank_f32(rp, col, 0.85, 12)
+        with md.Graph(rp, col) as G:
+            _, _, cost = _variant_samples(G, 12)
+            for p in (pol, swapped):
+                for use_graph in (0, 1):
+                    r, st = G.pagerank(0.85, 12, policy=p, use_graph=use_graph)
+                    assert np.array_equal(r.view(np.uint32), want32.view(np.uint32)), (name, use_graph)
+            G.pagerank(0.85, 12, policy=pol, trace=1)
+            acts = G.trace()[:, 7].astype(int)
+            G.pagerank(0.85, 12, policy=swapped, trace=1)
+            assert np.all(G.trace()[:, 7].astype(int) == md.PR_PUSH)
+            c_plain = cost[md.PR_BINNED].sum()
+            c_learn = sum(cost[a][i] for i, a in enumerate(acts))
+            t_plain, t_learn = _device_ms(G), _device_ms(G, policy=pol)
+            print(f"[pagerank variant {name}] modelled: fixed {c_plain:.1f} us, learned {c_learn:.1f} us; "
+                  f"30-superstep run: fixed {t_plain:.3f} ms, learned {t_learn:.3f} ms "
+                  f"({t_plain / t_learn:.2f}x); variants {np.bincount(acts, minlength=4)[1:].tolist()}", flush=True)
+            assert c_learn <= c_plain, name
+            if name == "rmat16":
+                assert t_plain / t_learn >= 1.1, (t_plain, t_learn)
