@@ -10,29 +10,34 @@
 //
 // Any number of sources may be given (medusa_bfs_multi): all start at level 0, which on a
 // packed batch of disjoint graphs with one source per graph gives every graph's own BFS
-// levels in one run.  Here a superstep only touches the frontier:
-//   push:  frontier queue -> wave-cooperative expansion (lanes share the out-edges
-//          of 64 frontier vertices through an in-wave prefix scan), hubs above
-//          `large` out-degree get split over whole workgroups;
-//   pull:  every unvisited vertex scans its in-edges for a parent in the
-//          frontier bitmap and stops at the first hit;
+// levels in one run.  Here a superstep is ONE kernel that only touches the frontier and
+// builds the next one as it goes:
+//   frontier: a bitmap (one bit per vertex, three buffers in rotation) plus a queue of its
+//          hubs (out-degree above `large`);
+//   push:  waves take bitmap words and share the out-edges of a word's frontier vertices
+//          through an in-wave prefix scan; hubs are split over whole workgroups; a target is
+//          claimed by the lane whose 64-bit atomic OR first sets its bit in the next bitmap,
+//          so each joining vertex is counted exactly once;
+//   pull:  every unvisited vertex scans its in-edges for a parent in the frontier bitmap and
+//          stops at the first hit; each wave owns 64 consecutive vertices and writes their
+//          word of the next bitmap whole (ballot), no atomics;
+//   counters: joined vertices, their out-degrees, hub degrees and probed in-edges, summed
+//          per workgroup into two packed 64-bit atomics; joined hubs are appended to the next
+//          hub queue (one atomic per wave that has any);
 //   direction: Beamer's rule with thresholds measured for this GPU (Sched), or the
-//          learned policy near the rule's switching point;
-//   compaction: one pass builds the next frontier bitmap (ballot per 64 vertices)
-//          and the small/large queues (ballot + in-workgroup prefix scan, one
-//          atomic per workgroup per queue) and counts the frontier's out-edges.
-// The claim `if (level[v] < 0) level[v] = step+1` is a plain store: every
-// writer stores the same value, so the race is benign (as in the reference).
+//          learned policy near the rule's switching point.
+// In the reference every writer stores the same level, so its claim is a plain store; here the
+// first setter of the bit also writes the level, which gives the same levels.
 //
-// Two drivers run the same kernels:
-//   device-driven (default): a one-workgroup kernel after each compaction decides the
-//     next superstep (direction by the heuristic or the learned policy, queue sizes, the
-//     termination vote) into a control block in HBM; batches of [expand, compact, decide]
-//     slots are captured once as hipGraphs and replayed until the control block says
-//     done, so the host waits once per batch, not per superstep (the first batch has as
-//     many slots as the previous run on the handle had supersteps, then kBfsBatch);
+// Two drivers run the same kernel:
+//   device-driven (default): the last workgroup of each superstep (by a ticket taken after
+//     its counters) decides the next one (direction, hub queue split, the termination vote)
+//     into a control block in HBM; batches of superstep slots are captured once as hipGraphs
+//     and replayed until the control block says done, so the host waits once per batch, not
+//     per superstep (the first batch has as many slots as the previous run on the handle had
+//     supersteps, then kBfsBatch);
 //   host-driven (timing or trace requested): the host reads the counters after every
-//     superstep, times the expansion with hipEvents and logs the trace record.
+//     superstep, times it with hipEvents and logs the trace record.
 
 #include <algorithm>
 
@@ -44,7 +49,7 @@ namespace medusa {
 constexpr uint32_t kLargeDefault = 2048;
 constexpr uint32_t kLargeSplit = 16;  // workgroups per hub in the large-vertex push
 constexpr uint32_t kBfsBatch = 8;     // supersteps per captured graph
-constexpr uint32_t kExpandGrid = 2048;
+constexpr uint32_t kExpandGrid = 1024;
 
 // Control block of the device-driven loop.
 struct BfsCtl {
@@ -55,38 +60,12 @@ struct BfsCtl {
     uint32_t supersteps, push_steps, pull_steps;
     uint32_t mode;      // 0 heuristic, 1 push, 2 pull, 3 learned policy
     uint32_t hub_wgs;   // push: workgroups of the expansion given to the hub queue
-    uint32_t pad[2];
+    uint32_t pending;   // a push with plain claims has run and its compaction has not
+    uint32_t small;     // push with exact (atomic) claims: the frontier's out-edges fit one queue segment
+    uint32_t slots;     // kernel launches the loop has used (a push with plain claims takes two)
+    uint32_t pad;
     unsigned long long n, m, n_f, m_f, n_u, m_u, reached, reached_edges, kernel_bytes, alg_bytes;
 };
-
-// Level reset (with `single` at level 0 when it is < n) and zeroed frontier counters.
-__global__ void k_bfs_reset(uint32_t n, int32_t* __restrict__ level, uint32_t single,
-                            FrontierCounters* __restrict__ ctr) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-        level[v] = v == single ? 0 : -1;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctr->n_small = 0;
-        ctr->n_large = 0;
-        ctr->n_new = 0;
-        ctr->deg_sum = 0;
-        ctr->probes = 0;
-        ctr->deg_large = 0;
-    }
-}
-
-__global__ void k_bfs_mark(const uint32_t* __restrict__ src, uint32_t count, int32_t* __restrict__ level) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        level[src[i]] = 0;
-}
-
-__global__ void k_zero_counters(FrontierCounters* c) {
-    c->n_small = 0;
-    c->n_large = 0;
-    c->n_new = 0;
-    c->deg_sum = 0;
-    c->probes = 0;
-    c->deg_large = 0;
-}
 
 // Workgroups of a push expansion given to the hub queue: in proportion to its share of
 // the frontier's out-edges, at least one for each non-empty queue.
@@ -101,68 +80,420 @@ __host__ __device__ inline uint32_t bfs_hub_wgs(uint32_t grid, uint32_t n_small,
 }
 
 // Host-driven superstep: the host's decision written into the control block.
-__global__ void k_bfs_set_ctl(BfsCtl* ctl, int32_t step, uint32_t pull, uint32_t n_small, uint32_t n_large,
-                              uint32_t hub_wgs) {
+__global__ void k_bfs_set_ctl(BfsCtl* ctl, int32_t step, uint32_t pull, uint32_t small, uint32_t n_small,
+                              uint32_t n_large, uint32_t hub_wgs) {
     ctl->step = step;
     ctl->pull = pull;
+    ctl->small = small;
     ctl->done = 0;
     ctl->n_small = n_small;
     ctl->n_large = n_large;
     ctl->hub_wgs = hub_wgs;
+    ctl->pending = 0;
 }
 
-// ---- superstep bodies (shared by both drivers) -----------------------------------
+// Counters of one superstep.  Atomics on one address serialise (about 10 ns each on this
+// GPU), so they are spread over kBfsSlots addresses, each on its own 128-byte line:
+//   seg[p][s]: the frontier of parity p (superstep t's frontier has parity t % 2) is a queue
+//     in kBfsSlots segments by vertex range (segment s holds vertices [s * span, (s + 1) *
+//     span)); its counter packs the vertices appended (low half, also the append offset) and
+//     the sum of their out-degrees (high half);
+//   work[s]: workgroup b adds the out-degree sum of the hubs it found (low half) and the
+//     in-edges it probed (high half) into slot b % kBfsSlots;
+//   sub / top: two levels of tickets that find the last workgroup of a superstep.
+// Every half's total is bounded by n or by m (both below 2^32), so no carry crosses into a
+// high half.
+constexpr uint32_t kBfsSlots = 32;
+constexpr uint32_t kBfsRound = 1024;  // vertices per workgroup round of the compaction; segment spans are multiples
+struct BfsLine {
+    alignas(128) unsigned long long v;
+};
+struct BfsTicket {
+    alignas(128) uint32_t v;
+};
+struct BfsCounters {
+    BfsLine seg[2][kBfsSlots];
+    BfsLine work[kBfsSlots];
+    BfsTicket sub[kBfsSlots];      // workgroups of slot s that have finished the superstep
+    MDS_COUNTER(uint32_t, top);    // slots whose workgroups have all finished
+    MDS_COUNTER(uint32_t, n_hub);  // joined hubs, appended to the next hub queue
+};
 
-// Wave-cooperative push over a queue of vertices with 1 <= outdeg <= large.  Of the
-// `nwaves` waves taking part (`wave0` is this wave's index among them), wave w holds queue
-// entries w, w + nwaves, w + 2*nwaves, ... one per lane, so a short queue of high-degree
-// vertices spreads over many waves instead of sharing one.
-__device__ __forceinline__ void push_small(const uint32_t* __restrict__ q, uint32_t qn,
-                                           const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                                           int32_t* __restrict__ level, int32_t next, uint32_t wave0, uint32_t nwaves,
-                                           uint32_t (*s_beg)[kWave], uint32_t (*s_off)[kWave]) {
+// Buffers and shape of a BFS run, passed by value to every superstep kernel.  Superstep t
+// reads the frontier as bitmap bm[t % 3] (pull), queue q[t % 2] (push) and hub queue
+// hub[t % 2], builds the next frontier's in bm[(t + 1) % 3], q[(t + 1) % 2] and
+// hub[(t + 1) % 2], and clears bm[(t + 2) % 3] for superstep t + 1 (an exact-claim push sets
+// bits in its next bitmap).
+struct BfsArgs {
+    const uint32_t *row_ptr, *col, *cptr, *csrc;
+    int32_t* level;
+    uint32_t* q[2];    // segmented queues, kBfsSlots * span entries each
+    uint32_t* hub[2];  // hub queues, n entries each
+    unsigned long long* bm[3];
+    BfsCounters* ctr;
+    BfsCtl* ctl;
+    const float* policy;  // learned direction policy (device parameters) or null
+    uint32_t n, nwords, large, grid, span;
+    unsigned long long exact_edges;  // frontier out-edges up to which a push claims exactly
+};
+
+// Segment span for n vertices: kBfsSlots segments cover [0, n) in whole compaction rounds.
+__host__ __device__ constexpr uint32_t bfs_span(uint32_t n) {
+    const uint64_t per = ((uint64_t)n + kBfsSlots - 1) / kBfsSlots;
+    const uint64_t span = (per + kBfsRound - 1) / kBfsRound * kBfsRound;
+    return (uint32_t)(span ? span : kBfsRound);
+}
+
+// A push claims exactly (compare-and-swap, no compaction) when the frontier has at most
+// `exact_edges` out-edges: 1, or 2 when it also has at most 64 vertices (every wave then reads
+// all of them and takes an equal share of their out-edges).  Its winners all go to segment 0 of
+// the next queue, which may then run on over the other segments' space (kBfsSlots * span >= n
+// entries, and no other segment receives any).
+__host__ __device__ inline uint32_t bfs_small(bool pull, unsigned long long n_f, unsigned long long m_f,
+                                              unsigned long long exact_edges) {
+    return pull || m_f > exact_edges ? 0u : (n_f <= (unsigned long long)kWave ? 2u : 1u);
+}
+
+__device__ __forceinline__ unsigned long long bfs_pack(uint32_t lo, uint32_t hi) {
+    return (unsigned long long)lo | ((unsigned long long)hi << 32);
+}
+
+// LDS of a workgroup's queue appends.
+struct BfsAppendLds {
+    uint32_t cnt[4];
+    unsigned long long deg[4];
+    uint32_t base;
+};
+
+constexpr uint32_t kRoundChunks = kBfsRound / 256;  // 64-vertex chunks per wave per workgroup round
+
+// One workgroup round's joins (every thread calls it; the round covers kBfsRound consecutive
+// vertices from `blk`, wave w holding chunks w * kRoundChunks + k, and all of them lie in one
+// queue segment): jm[k] is this wave's join mask of its chunk k.  The joined vertices are
+// appended to the next queue's segment with one atomic, whose packed increment also counts
+// them and their out-degrees; joined hubs (out-degree above `large`) also go to the next hub
+// queue (one atomic per wave that has any; hubs are few) and their degrees into `hacc`.
+__device__ __forceinline__ void bfs_append_round(const BfsArgs& a, const unsigned long long* jm, uint64_t blk,
+                                                 uint32_t* __restrict__ q_next, BfsLine* __restrict__ seg_next,
+                                                 uint32_t* __restrict__ hub_next, unsigned long long& hacc,
+                                                 BfsAppendLds& L) {
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const uint32_t sg = (uint32_t)(blk / a.span);
+    const uint32_t v0 = (uint32_t)blk + wv * kRoundChunks * kWave + lane;  // this lane's vertex in chunk 0
+    uint32_t deg[kRoundChunks], dsum = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRoundChunks; ++k) {
+        const uint32_t v = v0 + k * kWave;
+        deg[k] = (jm[k] >> lane) & 1ull ? a.row_ptr[v + 1] - a.row_ptr[v] : 0u;
+    }
+    unsigned long long hm[kRoundChunks], hany = 0;
+    uint32_t hcnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRoundChunks; ++k) {
+        dsum += deg[k];
+        cnt += (uint32_t)__popcll(jm[k]);
+        hm[k] = __ballot(deg[k] > a.large);  // deg is 0 unless the lane's vertex joined
+        hany |= hm[k];
+        hcnt += (uint32_t)__popcll(hm[k]);
+    }
+    if (hany) {
+        uint32_t hb = 0;
+        if (lane == 0) hb = atomicAdd(&a.ctr->n_hub, hcnt);
+        hb = __shfl(hb, 0, kWave);
+#pragma unroll
+        for (uint32_t k = 0; k < kRoundChunks; ++k) {
+            if ((hm[k] >> lane) & 1ull) {
+                hub_next[hb + lanes_below(hm[k])] = v0 + k * kWave;
+                hacc += deg[k];
+            }
+            hb += (uint32_t)__popcll(hm[k]);
+        }
+    }
+    const unsigned long long dtot = wave_sum_u64(dsum);
+    if (lane == 0) {
+        L.cnt[wv] = cnt;
+        L.deg[wv] = dtot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t c = L.cnt[0] + L.cnt[1] + L.cnt[2] + L.cnt[3];
+        const unsigned long long d = L.deg[0] + L.deg[1] + L.deg[2] + L.deg[3];
+        L.base = c ? (uint32_t)atomicAdd(&seg_next[sg].v, bfs_pack(c, (uint32_t)d)) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = L.base;
+    for (uint32_t w = 0; w < wv; ++w) off += L.cnt[w];
+#pragma unroll
+    for (uint32_t k = 0; k < kRoundChunks; ++k) {
+        if ((jm[k] >> lane) & 1ull) q_next[(size_t)sg * a.span + off + lanes_below(jm[k])] = v0 + k * kWave;
+        off += (uint32_t)__popcll(jm[k]);
+    }
+    __syncthreads();  // L is reused by the next round
+}
+
+// End of a counting kernel: the workgroup's hub degrees and probes go into its slot (one
+// atomic), then, with kDecide, it draws a ticket; returns true (to every thread) in the last
+// workgroup to finish.  Tickets are drawn only after the counter atomics have returned (the
+// asm makes the increment depend on their results), and every append of the block returned
+// before a barrier above, so when the last ticket is drawn every workgroup's counts have been
+// performed at the device's coherence point.  Device-scope atomics and atomic loads are
+// coherent across the XCDs' L2 caches without a fence (a device-scope release fence would
+// write the L2 back).  Two levels of tickets (per slot, then across slots) keep the atomics on
+// any one address to about grid / kBfsSlots.
+template <bool kDecide>
+__device__ __forceinline__ bool bfs_finish(BfsCounters* __restrict__ ctr, unsigned long long wacc) {
+    __shared__ unsigned long long s_sum[4];
+    __shared__ uint32_t s_last;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    wacc = wave_sum_u64(wacc);
+    if (lane == 0) s_sum[wv] = wacc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long tw = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+        const uint32_t s = blockIdx.x % kBfsSlots;
+        unsigned long long ret = 0;
+        if (tw) ret = atomicAdd(&ctr->work[s].v, tw);
+        if (kDecide) {
+            uint32_t inc = 1;
+            asm volatile("" : "+v"(inc) : "v"(ret));
+            const uint32_t members = (gridDim.x - s + kBfsSlots - 1) / kBfsSlots;  // workgroups of slot s
+            const uint32_t nslots = gridDim.x < kBfsSlots ? gridDim.x : kBfsSlots;
+            bool last = false;
+            if (atomicAdd(&ctr->sub[s].v, inc) == members - 1) {
+                uint32_t inc2 = 1;
+                asm volatile("" : "+v"(inc2) : "v"(members));
+                last = atomicAdd(&ctr->top, inc2) == nslots - 1;
+            }
+            s_last = last ? 1u : 0u;
+        }
+    }
+    if constexpr (!kDecide) return false;
+    __syncthreads();
+    return s_last != 0;
+}
+
+// ---- superstep bodies ---------------------------------------------------------------
+
+// Prefix of the frontier queue's segment counts in LDS (one wave scans them): entries
+// [pre[s], pre[s + 1]) of the frontier live in segment s.
+__device__ __forceinline__ void bfs_queue_prefix(const BfsLine* __restrict__ seg, uint32_t* pre) {
+    if (threadIdx.x < kWave) {
+        const uint32_t lane = threadIdx.x;
+        const uint32_t c = lane < kBfsSlots ? (uint32_t)seg[lane].v : 0u;
+        const uint32_t incl = wave_incl_scan(c);
+        if (lane < kBfsSlots) pre[lane + 1] = incl;
+        if (lane == 0) pre[0] = 0;
+    }
+    __syncthreads();
+}
+
+// Queue entry i of the frontier.
+__device__ __forceinline__ uint32_t bfs_queue_at(const BfsArgs& a, const uint32_t* __restrict__ q, const uint32_t* pre,
+                                                 uint32_t i) {
+    uint32_t s = 0;  // last segment with pre[s] <= i
+#pragma unroll
+    for (uint32_t step = kBfsSlots / 2; step; step >>= 1)
+        if (pre[s + step] <= i) s += step;
+    return q[(size_t)s * a.span + (i - pre[s])];
+}
+
+// Output of an exact-claim push: its winners go to segment 0 of the next queue (which may run on
+// over the space of the others), the next bitmap and the hub queue.
+struct BfsExactOut {
+    unsigned long long* nxt;
+    uint32_t* q_next;
+    BfsLine* seg_next;
+    uint32_t* hub_next;
+};
+
+// A push claim of v for level `next`.  Plain: `if (level[v] < 0) level[v] = next`, a plain
+// store; every writer stores the same value, so the race is benign (as in the reference) and
+// the compaction after the push finds the vertices that joined.  Exact: a compare-and-swap
+// from -1 picks one winner per vertex, which is recorded at once (bfs_record_round).
+template <bool kExact>
+__device__ __forceinline__ bool bfs_claim(int32_t* __restrict__ level, uint32_t v, int32_t next) {
+    if (level[v] >= 0) return false;
+    if constexpr (!kExact) {
+        level[v] = next;
+        return false;
+    } else {
+        return atomicCAS(&level[v], -1, next) == -1;
+    }
+}
+
+constexpr uint32_t kPushUnroll = 4;  // out-edges per lane per round of the push, loads issued together
+
+// Winners of a round of an exact-claim push (the calling wave is converged; won[j] / v[j] for
+// the round's kPushUnroll edges of this lane): one packed atomic per wave appends them to
+// segment 0 of the next queue and counts them with their out-degrees; each sets its bit in the
+// next bitmap; hubs also go to the next hub queue.
+__device__ __forceinline__ void bfs_record_round(const BfsArgs& a, const BfsExactOut& o, const bool* won,
+                                                 const uint32_t* v, unsigned long long& hacc) {
+    unsigned long long wm[kPushUnroll], any = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPushUnroll; ++j) {
+        wm[j] = __ballot(won[j]);
+        any |= wm[j];
+    }
+    if (!any) return;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    uint32_t deg[kPushUnroll], dsum = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPushUnroll; ++j) deg[j] = won[j] ? a.row_ptr[v[j] + 1] - a.row_ptr[v[j]] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < kPushUnroll; ++j) {
+        if (won[j]) atomicOr(&o.nxt[v[j] >> 6], 1ull << (v[j] & 63));
+        dsum += deg[j];
+        cnt += (uint32_t)__popcll(wm[j]);
+    }
+    const unsigned long long dtot = wave_sum_u64(dsum);
+    uint32_t base = 0;
+    if (lane == 0) base = (uint32_t)atomicAdd(&o.seg_next[0].v, bfs_pack(cnt, (uint32_t)dtot));
+    base = __shfl(base, 0, kWave);
+    unsigned long long hm[kPushUnroll], hany = 0;
+    uint32_t hcnt = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPushUnroll; ++j) {
+        if (won[j]) o.q_next[base + lanes_below(wm[j])] = v[j];
+        base += (uint32_t)__popcll(wm[j]);
+        hm[j] = __ballot(won[j] && deg[j] > a.large);
+        hany |= hm[j];
+        hcnt += (uint32_t)__popcll(hm[j]);
+    }
+    if (hany) {
+        uint32_t hb = 0;
+        if (lane == 0) hb = atomicAdd(&a.ctr->n_hub, hcnt);
+        hb = __shfl(hb, 0, kWave);
+#pragma unroll
+        for (uint32_t j = 0; j < kPushUnroll; ++j) {
+            if ((hm[j] >> lane) & 1ull) {
+                o.hub_next[hb + lanes_below(hm[j])] = v[j];
+                hacc += deg[j];
+            }
+            hb += (uint32_t)__popcll(hm[j]);
+        }
+    }
+}
+
+// Wave-cooperative push over the frontier queue (hubs and vertices without out-edges are
+// skipped: out-degree taken as 0).  Of the `nwaves` waves taking part (`wave0` is this wave's
+// index among them), wave w holds queue entries w, w + nwaves, w + 2*nwaves, ... one per lane,
+// so a run of high-degree vertices spreads over many waves instead of sharing one; the lanes
+// then share the entries' out-edges through an in-wave prefix scan, kPushUnroll edges per lane
+// per round.
+template <bool kExact>
+__device__ __forceinline__ void push_queue(const BfsArgs& a, const uint32_t* __restrict__ q, const uint32_t* pre,
+                                           int32_t next, uint32_t wave0, uint32_t nwaves, uint32_t (*s_beg)[kWave],
+                                           uint32_t (*s_off)[kWave], const BfsExactOut& o, unsigned long long& hacc) {
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const uint32_t qn = pre[kBfsSlots];
     for (uint64_t r = 0; wave0 + r * kWave * nwaves < qn; ++r) {  // uniform across the wave
         const uint64_t i = wave0 + (r * kWave + lane) * nwaves;
         uint32_t beg = 0, deg = 0;
         if (i < qn) {
-            const uint32_t u = q[i];
-            beg = row_ptr[u];
-            deg = row_ptr[u + 1] - beg;
+            const uint32_t u = bfs_queue_at(a, q, pre, (uint32_t)i);
+            beg = a.row_ptr[u];
+            deg = a.row_ptr[u + 1] - beg;
+            if (deg > a.large) deg = 0;  // a hub: expanded from the hub queue
         }
         const uint32_t incl = wave_incl_scan(deg);
         const uint32_t total = __shfl(incl, kWave - 1, kWave);
         s_beg[wv][lane] = beg;
         s_off[wv][lane] = incl - deg;
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t e = lane; e < total; e += kWave) {
-            // owner = last lane whose exclusive offset is <= e (it has deg > 0)
-            uint32_t lo = 0;
+        for (uint32_t e0 = 0; e0 < total; e0 += kWave * kPushUnroll) {  // uniform across the wave
+            uint32_t v[kPushUnroll];
 #pragma unroll
-            for (uint32_t step = kWave / 2; step; step >>= 1)
-                if (s_off[wv][lo + step] <= e) lo += step;
-            const uint32_t v = col[s_beg[wv][lo] + (e - s_off[wv][lo])];
-            if (level[v] < 0) level[v] = next;
+            for (uint32_t j = 0; j < kPushUnroll; ++j) {
+                const uint32_t e = e0 + j * kWave + lane;
+                v[j] = 0xFFFFFFFFu;
+                if (e < total) {
+                    // owner = last lane whose exclusive offset is <= e (it has deg > 0)
+                    uint32_t lo = 0;
+#pragma unroll
+                    for (uint32_t step = kWave / 2; step; step >>= 1)
+                        if (s_off[wv][lo + step] <= e) lo += step;
+                    v[j] = a.col[s_beg[wv][lo] + (e - s_off[wv][lo])];
+                }
+            }
+            bool won[kPushUnroll];
+#pragma unroll
+            for (uint32_t j = 0; j < kPushUnroll; ++j)
+                won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<kExact>(a.level, v[j], next);
+            if constexpr (kExact) bfs_record_round(a, o, won, v, hacc);
         }
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-// Hubs: each vertex of the queue is split into `split` contiguous edge ranges, one per
+// Exact-claim push of a frontier of at most 64 vertices (hubs included): every wave reads the
+// whole frontier and its out-degree prefix, and wave w of the grid takes out-edges
+// [w * 64 * kPushUnroll, (w + 1) * 64 * kPushUnroll) of their concatenation, so a frontier of
+// one high-degree vertex still spreads over many waves.
+__device__ __forceinline__ void push_tiny(const BfsArgs& a, const uint32_t* __restrict__ q, const uint32_t* pre,
+                                          int32_t next, uint32_t (*s_beg)[kWave], uint32_t (*s_off)[kWave],
+                                          const BfsExactOut& o, unsigned long long& hacc) {
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const uint32_t qn = pre[kBfsSlots];
+    const uint64_t e_begin = ((uint64_t)blockIdx.x * (blockDim.x / kWave) + wv) * kWave * kPushUnroll;
+    uint32_t beg = 0, deg = 0;
+    if (lane < qn) {
+        const uint32_t u = bfs_queue_at(a, q, pre, lane);
+        beg = a.row_ptr[u];
+        deg = a.row_ptr[u + 1] - beg;
+    }
+    const uint32_t incl = wave_incl_scan(deg);
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    if (e_begin >= total) return;  // uniform across the wave
+    s_beg[wv][lane] = beg;
+    s_off[wv][lane] = incl - deg;
+    __builtin_amdgcn_wave_barrier();
+    uint32_t v[kPushUnroll];
+#pragma unroll
+    for (uint32_t j = 0; j < kPushUnroll; ++j) {
+        const uint64_t e = e_begin + j * kWave + lane;
+        v[j] = 0xFFFFFFFFu;
+        if (e < total) {
+            uint32_t lo = 0;
+#pragma unroll
+            for (uint32_t step = kWave / 2; step; step >>= 1)
+                if (s_off[wv][lo + step] <= e) lo += step;
+            v[j] = a.col[s_beg[wv][lo] + ((uint32_t)e - s_off[wv][lo])];
+        }
+    }
+    bool won[kPushUnroll];
+#pragma unroll
+    for (uint32_t j = 0; j < kPushUnroll; ++j) won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<true>(a.level, v[j], next);
+    bfs_record_round(a, o, won, v, hacc);
+}
+
+// Hubs: each vertex of the hub queue is split into `split` contiguous edge ranges, one per
 // workgroup item; the `nwg` workgroups taking part stride over the items.
-__device__ __forceinline__ void push_large(const uint32_t* __restrict__ q, uint32_t qn,
-                                           const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                                           int32_t* __restrict__ level, int32_t next, uint32_t wg0, uint32_t nwg) {
+template <bool kExact>
+__device__ __forceinline__ void push_hubs(const BfsArgs& a, const uint32_t* __restrict__ q, uint32_t qn, int32_t next,
+                                          uint32_t wg0, uint32_t nwg, const BfsExactOut& o,
+                                          unsigned long long& hacc) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t split = nwg / qn;
     split = split < kLargeSplit ? kLargeSplit : (split > 4096u ? 4096u : split);
     for (uint64_t w = wg0; w < (uint64_t)qn * split; w += nwg) {
         const uint32_t u = q[w / split], part = (uint32_t)(w % split);
-        const uint32_t b = row_ptr[u], deg = row_ptr[u + 1] - b;
+        const uint32_t b = a.row_ptr[u], deg = a.row_ptr[u + 1] - b;
         const uint32_t per = (deg + split - 1) / split;
         const uint32_t lo = b + min(deg, part * per), hi = b + min(deg, (part + 1) * per);
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += blockDim.x) {
-            const uint32_t v = col[k];
-            if (level[v] < 0) level[v] = next;
+        for (uint32_t k0 = lo + (threadIdx.x - lane) * kPushUnroll; k0 < hi;
+             k0 += blockDim.x * kPushUnroll) {  // uniform per wave
+            uint32_t v[kPushUnroll];
+            bool won[kPushUnroll];
+#pragma unroll
+            for (uint32_t j = 0; j < kPushUnroll; ++j) {
+                const uint32_t k = k0 + j * kWave + lane;
+                v[j] = k < hi ? a.col[k] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kPushUnroll; ++j)
+                won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<kExact>(a.level, v[j], next);
+            if constexpr (kExact) bfs_record_round(a, o, won, v, hacc);
         }
     }
 }
@@ -170,280 +501,152 @@ __device__ __forceinline__ void push_large(const uint32_t* __restrict__ q, uint3
 constexpr uint32_t kPullLane = 32;  // in-degree up to which one lane scans a vertex alone
 constexpr uint32_t kPullGroup = 4;  // in-edges a lane probes per round
 
-// Bottom-up: an unvisited vertex joins if any in-neighbour is in the frontier bitmap.  A
-// lane scans a vertex of in-degree <= kPullLane alone; above that the whole wave scans it
-// 64 in-edges at a time and stops at the first hit (ballot), so one high in-degree vertex
-// does not hold its wave for hundreds of sequential probes.  `probes` counts in-edges
-// examined up to and including the first hit in either case.
-__device__ __forceinline__ void pull_all(uint32_t n, const uint32_t* __restrict__ cptr,
-                                         const uint32_t* __restrict__ csrc,
-                                         const unsigned long long* __restrict__ bitmap, int32_t* __restrict__ level,
-                                         int32_t next, FrontierCounters* __restrict__ ctr) {
+// Bottom-up over 64 consecutive vertices from `base` (one per lane; the wave is converged): an
+// unvisited vertex joins if any in-neighbour is in the frontier bitmap.  A lane scans a vertex
+// of in-degree <= kPullLane alone; above that the whole wave scans it 64 in-edges at a time and
+// stops at the first hit (ballot), so one high in-degree vertex does not hold its wave for
+// hundreds of sequential probes.  Returns the join mask, which is also written as the chunk's
+// word of the next bitmap.  `probes` counts in-edges examined up to and including the first
+// hit in either case.
+__device__ __forceinline__ unsigned long long pull_chunk(const BfsArgs& a, const unsigned long long* __restrict__ cur,
+                                                         unsigned long long* __restrict__ nxt, uint64_t base,
+                                                         int32_t next, unsigned long long& probes) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    unsigned long long probes = 0;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x - lane); base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t v = base + lane;  // uniform loop: every lane of the wave iterates together
-        uint32_t b = 0, e = 0;
-        bool todo = false;
-        if (v < n && level[v] < 0) {
-            b = cptr[v];
-            e = cptr[v + 1];
-            todo = b < e;
-        }
-        const bool big = todo && e - b > kPullLane;
-        if (todo && !big) {
-            // kPullGroup in-edges per round: their source and bitmap loads are issued together,
-            // so a round costs two memory round trips instead of two per in-edge; probes still
-            // count the in-edges up to and including the first hit
-            for (uint32_t k0 = b; k0 < e; k0 += kPullGroup) {
-                uint32_t u[kPullGroup];
-                unsigned long long word[kPullGroup];
+    const uint32_t n = a.n;
+    if (base >= n) return 0;  // uniform across the wave
+    const uint32_t v = (uint32_t)base + lane;
+    uint32_t b = 0, e = 0;
+    bool todo = false, joined = false;
+    if (v < n && a.level[v] < 0) {
+        b = a.cptr[v];
+        e = a.cptr[v + 1];
+        todo = b < e;
+    }
+    const bool big = todo && e - b > kPullLane;
+    if (todo && !big) {
+        // kPullGroup in-edges per round: their source and bitmap loads are issued together,
+        // so a round costs two memory round trips instead of two per in-edge
+        for (uint32_t k0 = b; k0 < e; k0 += kPullGroup) {
+            uint32_t u[kPullGroup];
+            unsigned long long word[kPullGroup];
 #pragma unroll
-                for (uint32_t j = 0; j < kPullGroup; ++j) u[j] = k0 + j < e ? csrc[k0 + j] : 0u;
+            for (uint32_t j = 0; j < kPullGroup; ++j) u[j] = k0 + j < e ? a.csrc[k0 + j] : 0u;
 #pragma unroll
-                for (uint32_t j = 0; j < kPullGroup; ++j) word[j] = k0 + j < e ? bitmap[u[j] >> 6] : 0ull;
-                uint32_t hit = kPullGroup;
+            for (uint32_t j = 0; j < kPullGroup; ++j) word[j] = k0 + j < e ? cur[u[j] >> 6] : 0ull;
+            uint32_t hit = kPullGroup;
 #pragma unroll
-                for (uint32_t j = kPullGroup; j-- > 0;)
-                    if ((word[j] >> (u[j] & 63)) & 1ull) hit = j;
-                if (hit < kPullGroup) {
-                    probes += hit + 1;
-                    level[v] = next;
-                    break;
-                }
-                probes += min(e - k0, kPullGroup);
+            for (uint32_t j = kPullGroup; j-- > 0;)
+                if ((word[j] >> (u[j] & 63)) & 1ull) hit = j;
+            if (hit < kPullGroup) {
+                probes += hit + 1;
+                a.level[v] = next;
+                joined = true;
+                break;
             }
-        }
-        for (unsigned long long bm = __ballot(big); bm; bm &= bm - 1) {
-            const uint32_t owner = __builtin_ctzll(bm);
-            const uint32_t vb = __shfl(b, owner, kWave), ve = __shfl(e, owner, kWave);
-            for (uint32_t k0 = vb; k0 < ve; k0 += kWave) {
-                const uint32_t k = k0 + lane;
-                bool hit = false;
-                if (k < ve) {
-                    const uint32_t u = csrc[k];
-                    hit = (bitmap[u >> 6] >> (u & 63)) & 1ull;
-                }
-                const unsigned long long hm = __ballot(hit);
-                if (hm) {
-                    if (lane == 0) {
-                        probes += __builtin_ctzll(hm) + 1;
-                        level[base + owner] = next;
-                    }
-                    break;
-                }
-                if (lane == 0) probes += min(ve - k0, (uint32_t)kWave);
-            }
+            probes += min(e - k0, kPullGroup);
         }
     }
-    // one atomic per workgroup (every wave of the block reaches this point)
-    __shared__ unsigned long long s_probe[4];
-    probes = wave_sum_u64(probes);
-    if (lane == 0) s_probe[threadIdx.x / kWave] = probes;
-    __syncthreads();
+    unsigned long long bigjoin = 0;
+    for (unsigned long long bm = __ballot(big); bm; bm &= bm - 1) {
+        const uint32_t owner = __builtin_ctzll(bm);
+        const uint32_t vb = __shfl(b, owner, kWave), ve = __shfl(e, owner, kWave);
+        for (uint32_t k0 = vb; k0 < ve; k0 += kWave) {
+            const uint32_t k = k0 + lane;
+            bool hit = false;
+            if (k < ve) {
+                const uint32_t u = a.csrc[k];
+                hit = (cur[u >> 6] >> (u & 63)) & 1ull;
+            }
+            const unsigned long long hm = __ballot(hit);
+            if (hm) {
+                if (lane == 0) {
+                    probes += __builtin_ctzll(hm) + 1;
+                    a.level[(uint32_t)base + owner] = next;
+                }
+                bigjoin |= 1ull << owner;
+                break;
+            }
+            if (lane == 0) probes += min(ve - k0, (uint32_t)kWave);
+        }
+    }
+    const unsigned long long jm = __ballot(joined) | bigjoin;
+    if (lane == 0) nxt[base / kWave] = jm;
+    return jm;
+}
+
+// Pull superstep: workgroup rounds of kBfsRound consecutive vertices, wave w taking chunks
+// w * kRoundChunks + k; each wave writes its chunks' words of the next bitmap whole and the
+// round's joins are appended to the next queue, so the pull needs no compaction.
+__device__ __forceinline__ void pull_all(const BfsArgs& a, const unsigned long long* __restrict__ cur,
+                                         unsigned long long* __restrict__ nxt, uint32_t* __restrict__ q_next,
+                                         BfsLine* __restrict__ seg_next, uint32_t* __restrict__ hub_next, int32_t next,
+                                         unsigned long long& hacc, unsigned long long& probes, BfsAppendLds& L) {
+    const uint32_t wv = threadIdx.x / kWave;
+    for (uint64_t blk = (uint64_t)blockIdx.x * kBfsRound; blk < a.n; blk += (uint64_t)gridDim.x * kBfsRound) {
+        unsigned long long jm[kRoundChunks];
+#pragma unroll 1
+        for (uint32_t k = 0; k < kRoundChunks; ++k)
+            jm[k] = pull_chunk(a, cur, nxt, blk + (wv * kRoundChunks + k) * kWave, next, probes);
+        bfs_append_round(a, jm, blk, q_next, seg_next, hub_next, hacc, L);
+    }
+}
+
+// Sums of the next frontier's segment counters and of the work slots (each wave reduces them
+// itself, so every thread gets them without a barrier).
+__device__ __forceinline__ void bfs_sum_counters(BfsCounters* __restrict__ ctr, uint32_t par_next,
+                                                 unsigned long long& joined, unsigned long long& work) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const unsigned long long j = lane < kBfsSlots ? load_now(&ctr->seg[par_next][lane].v) : 0ull;
+    const unsigned long long w = lane < kBfsSlots ? load_now(&ctr->work[lane].v) : 0ull;
+    joined = wave_sum_u64(j);
+    work = wave_sum_u64(w);
+}
+
+// Counters cleared for the next superstep (by one workgroup, after every thread has read
+// them): the segment counters of parity `par_free` (a frontier already expanded), the work
+// slots, the tickets and the hub count.
+__device__ __forceinline__ void bfs_clear_counters(BfsCounters* __restrict__ ctr, uint32_t par_free) {
+    if (threadIdx.x < kBfsSlots) {
+        ctr->seg[par_free][threadIdx.x].v = 0;
+        ctr->work[threadIdx.x].v = 0;
+        ctr->sub[threadIdx.x].v = 0;
+    }
     if (threadIdx.x == 0) {
-        const unsigned long long t = s_probe[0] + s_probe[1] + s_probe[2] + s_probe[3];
-        if (t) atomicAdd(&ctr->probes, t);
+        ctr->top = 0;
+        ctr->n_hub = 0;
     }
 }
 
-// A compaction tile is 4 waves x K chunks of 64 vertices; every tile adds to the frontier
-// counters with one atomic each, and atomics on one address serialise, so large graphs use
-// large tiles while small graphs use small ones to keep the GPU busy (see compact_k).
-template <uint32_t K>
-constexpr uint32_t compact_tile() { return 4 * K * kWave; }
-
-// Next frontier = {v : level[v] == want}: bitmap word per 64 vertices, queues by
-// out-degree class, and frontier counters.
-template <uint32_t kCompactK>
-__device__ __forceinline__ void compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
-                                        const uint32_t* __restrict__ row_ptr, uint32_t large,
-                                        uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
-                                        unsigned long long* __restrict__ bitmap, uint32_t nwords,
-                                        FrontierCounters* __restrict__ ctr) {
-    __shared__ uint32_t s_cnt[4][3];
-    __shared__ unsigned long long s_deg[4], s_dlg[4];
-    __shared__ uint32_t s_base[2];
-    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    for (uint32_t tile = blockIdx.x; (uint64_t)tile * compact_tile<kCompactK>() < n; tile += gridDim.x) {
-        unsigned long long msm[kCompactK], mlg[kCompactK];
-        uint32_t csm = 0, clg = 0, cnew = 0;
-        unsigned long long dsum = 0, dlg = 0;
-        // all loads of the tile first (independent, so they overlap), then the ballots
-        int32_t lv[kCompactK];
-        uint32_t deg_of[kCompactK];
-#pragma unroll
-        for (uint32_t k = 0; k < kCompactK; ++k) {
-            const uint32_t v = (tile * (4 * kCompactK) + k * 4 + wv) * kWave + lane;
-            lv[k] = v < n ? level[v] : -1;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kCompactK; ++k) {
-            const uint32_t v = (tile * (4 * kCompactK) + k * 4 + wv) * kWave + lane;
-            deg_of[k] = lv[k] == want ? row_ptr[v + 1] - row_ptr[v] : 0u;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kCompactK; ++k) {
-            const uint32_t chunk = tile * (4 * kCompactK) + k * 4 + wv;
-            const bool pred = lv[k] == want;
-            const unsigned long long mask = __ballot(pred);
-            if (lane == 0 && chunk < nwords) bitmap[chunk] = mask;
-            const uint32_t deg = deg_of[k];
-            msm[k] = __ballot(pred && deg > 0 && deg <= large);
-            mlg[k] = __ballot(pred && deg > large);
-            cnew += __popcll(mask);
-            csm += __popcll(msm[k]);
-            clg += __popcll(mlg[k]);
-            dsum += deg;
-            dlg += deg > large ? deg : 0u;
-        }
-        dsum = wave_sum_u64(dsum);
-        dlg = wave_sum_u64(dlg);
-        if (lane == 0) {
-            s_cnt[wv][0] = csm;
-            s_cnt[wv][1] = clg;
-            s_cnt[wv][2] = cnew;
-            s_deg[wv] = dsum;
-            s_dlg[wv] = dlg;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tsm = 0, tlg = 0, tnew = 0;
-            unsigned long long tdeg = 0, tdlg = 0;
-            for (int w = 0; w < 4; ++w) {
-                tsm += s_cnt[w][0];
-                tlg += s_cnt[w][1];
-                tnew += s_cnt[w][2];
-                tdeg += s_deg[w];
-                tdlg += s_dlg[w];
-            }
-            s_base[0] = tsm ? atomicAdd(&ctr->n_small, tsm) : 0u;
-            s_base[1] = tlg ? atomicAdd(&ctr->n_large, tlg) : 0u;
-            if (tnew) atomicAdd(&ctr->n_new, tnew);
-            if (tdeg) atomicAdd(&ctr->deg_sum, tdeg);
-            if (tdlg) atomicAdd(&ctr->deg_large, tdlg);
-        }
-        __syncthreads();
-        uint32_t osm = s_base[0], olg = s_base[1];
-        for (uint32_t w = 0; w < wv; ++w) {
-            osm += s_cnt[w][0];
-            olg += s_cnt[w][1];
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kCompactK; ++k) {
-            const uint32_t v = (tile * (4 * kCompactK) + k * 4 + wv) * kWave + lane;
-            if ((msm[k] >> lane) & 1ull) q_small[osm + lanes_below(msm[k])] = v;
-            if ((mlg[k] >> lane) & 1ull) q_large[olg + lanes_below(mlg[k])] = v;
-            osm += __popcll(msm[k]);
-            olg += __popcll(mlg[k]);
-        }
-        __syncthreads();
-    }
-}
-
-// ---- kernels (both drivers) ---------------------------------------------------------
-
-template <uint32_t K>
-__global__ __launch_bounds__(256) void k_bfs_compact(uint32_t n, const int32_t* __restrict__ level, int32_t want,
-                                                     const uint32_t* __restrict__ row_ptr, uint32_t large,
-                                                     uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
-                                                     unsigned long long* __restrict__ bitmap, uint32_t nwords,
-                                                     FrontierCounters* __restrict__ ctr) {
-    compact<K>(n, level, want, row_ptr, large, q_small, q_large, bitmap, nwords, ctr);
-}
-
-// Expansion of the current superstep as the control block describes it: pull over every
-// vertex, or push where the first hub_wgs workgroups take the hub queue and the rest the
-// wave-cooperative queue.
-__global__ __launch_bounds__(256) void k_bfs_expand_dev(const BfsCtl* __restrict__ ctl,
-                                                        const uint32_t* __restrict__ q_small,
-                                                        const uint32_t* __restrict__ q_large,
-                                                        const uint32_t* __restrict__ row_ptr,
-                                                        const uint32_t* __restrict__ col,
-                                                        const uint32_t* __restrict__ cptr,
-                                                        const uint32_t* __restrict__ csrc,
-                                                        const unsigned long long* __restrict__ bitmap,
-                                                        int32_t* __restrict__ level, uint32_t n,
-                                                        FrontierCounters* __restrict__ ctr) {
-    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
-    if (ctl->done) return;
-    const int32_t next = ctl->step + 1;
-    if (ctl->pull) {
-        pull_all(n, cptr, csrc, bitmap, level, next, ctr);
-        return;
-    }
-    const uint32_t hub = ctl->hub_wgs;
-    if (blockIdx.x < hub) {
-        push_large(q_large, ctl->n_large, row_ptr, col, level, next, blockIdx.x, hub);
-        return;
-    }
-    const uint32_t wg = blockIdx.x - hub, nwg = gridDim.x - hub;
-    push_small(q_small, ctl->n_small, row_ptr, col, level, next, wg * 4 + threadIdx.x / kWave, nwg * 4, s_beg, s_off);
-}
-
-// Compaction of the current superstep.
-template <uint32_t K>
-__global__ __launch_bounds__(256) void k_bfs_compact_dev(const BfsCtl* __restrict__ ctl, uint32_t n,
-                                                         const int32_t* __restrict__ level,
-                                                         const uint32_t* __restrict__ row_ptr, uint32_t large,
-                                                         uint32_t* __restrict__ q_small, uint32_t* __restrict__ q_large,
-                                                         unsigned long long* __restrict__ bitmap, uint32_t nwords,
-                                                         FrontierCounters* __restrict__ ctr) {
-    if (ctl->done) return;
-    compact<K>(n, level, ctl->step + 1, row_ptr, large, q_small, q_large, bitmap, nwords, ctr);
-}
-
-// Chunks per wave of a compaction tile for a graph of n vertices: small tiles below 2^18
-// vertices (enough workgroups to fill the GPU), large ones above 1.5 M (fewer counter
-// atomics).  Measured on Erdos-Renyi 65 K, RMAT 1 M and Chung-Lu 2.4 M vertex graphs.
-static uint32_t compact_k(uint32_t n) { return n <= (1u << 18) ? 2u : (n <= 3u * (1u << 19) ? 8u : 16u); }
-
-// Launches the compaction of level `want` (host-driven and seeding: ctl == nullptr) or of
-// the control block's superstep (device-driven).
-template <uint32_t K>
-static void launch_compact_k(hipStream_t s, const BfsCtl* ctl, uint32_t n, int32_t want, const int32_t* level,
-                             const uint32_t* row_ptr, uint32_t large, uint32_t* q_small, uint32_t* q_large,
-                             unsigned long long* bitmap, uint32_t nwords, FrontierCounters* ctr) {
-    const dim3 grid(grid_for(n, compact_tile<K>(), 4096));
-    if (ctl)
-        hipLaunchKernelGGL(k_bfs_compact_dev<K>, grid, dim3(256), 0, s, ctl, n, level, row_ptr, large, q_small,
-                           q_large, bitmap, nwords, ctr);
-    else
-        hipLaunchKernelGGL(k_bfs_compact<K>, grid, dim3(256), 0, s, n, level, want, row_ptr, large, q_small, q_large,
-                           bitmap, nwords, ctr);
-}
-static void launch_compact(hipStream_t s, const BfsCtl* ctl, uint32_t n, int32_t want, const int32_t* level,
-                           const uint32_t* row_ptr, uint32_t large, uint32_t* q_small, uint32_t* q_large,
-                           unsigned long long* bitmap, uint32_t nwords, FrontierCounters* ctr) {
-    switch (compact_k(n)) {
-        case 2: launch_compact_k<2>(s, ctl, n, want, level, row_ptr, large, q_small, q_large, bitmap, nwords, ctr); break;
-        case 8: launch_compact_k<8>(s, ctl, n, want, level, row_ptr, large, q_small, q_large, bitmap, nwords, ctr); break;
-        default:
-            launch_compact_k<16>(s, ctl, n, want, level, row_ptr, large, q_small, q_large, bitmap, nwords, ctr);
-    }
-}
-
-// One workgroup after the compaction: its counters are visible at the kernel boundary,
-// so the decision needs no fence.  Decides the next superstep into the control block.
-__global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, uint32_t n, uint32_t nwords,
-                                                    FrontierCounters* __restrict__ ctr,
-                                                    const float* __restrict__ policy, uint32_t grid) {
+// The next superstep from this one's counters (one workgroup, after every workgroup of the
+// superstep has added its counters): direction by the heuristic or the learned policy, hub
+// queue split and the termination vote, written into the control block; the counters are
+// cleared for the next superstep.
+__device__ __forceinline__ void bfs_decide(const BfsArgs& a) {
     __shared__ MlpShared s_mlp;
-    const uint32_t n_new = load_now(&ctr->n_new);
-    const uint32_t n_small = load_now(&ctr->n_small);
-    const uint32_t n_large = load_now(&ctr->n_large);
-    const unsigned long long deg_sum = load_now(&ctr->deg_sum);
-    const unsigned long long probes = load_now(&ctr->probes);
-    const unsigned long long deg_large = load_now(&ctr->deg_large);
-    BfsCtl c = load_now_block(ctl);
-    if (c.done) return;
-    // byte model of the superstep just finished (same as the host-driven driver): push =
-    // 12 per frontier vertex + 8 per out-edge + 8 per claim; pull = 2 bitmap passes (n/4) +
-    // 8 per unvisited vertex + 8 per probed in-edge + 4 per claim
-    const unsigned long long kb = c.pull ? n / 4 + 8ull * c.n_u + 8ull * probes + 4ull * n_new
-                                         : 12ull * c.n_f + 8ull * c.m_f + 8ull * n_new;
+    BfsCounters* ctr = a.ctr;
+    BfsCtl c = load_now_block(a.ctl);
+    const uint32_t par = (uint32_t)c.step & 1u;
+    unsigned long long joined, work;
+    bfs_sum_counters(ctr, par ^ 1u, joined, work);
+    const uint32_t n_hub = load_now(&ctr->n_hub);
+    const uint32_t n = a.n, n_new = (uint32_t)joined;
+    const unsigned long long deg_sum = joined >> 32, deg_large = (uint32_t)work, probes = work >> 32;
+    // bytes of the superstep just finished (every expansion clears one bitmap: 8 per word):
+    // push = queue entry and out-offsets per frontier vertex (12) + target id and level probe
+    // per out-edge (8); exact claims then add compare-and-swap, bit OR, out-offsets and queue
+    // entry per join (24), plain claims the store per join (4) and the compaction's level pass
+    // (4n), next bitmap (8 per word), out-offsets and queue entry per join (12); pull = level of
+    // every vertex (4n) + in-offsets of unvisited vertices (8) + source id and bitmap word per
+    // probed in-edge (8) + next bitmap (8 per word) + level, out-offsets and queue entry per
+    // join (16)
+    const unsigned long long kb =
+        c.pull    ? 4ull * n + 8ull * c.n_u + 8ull * probes + 16ull * a.nwords + 16ull * n_new
+        : c.small ? 12ull * c.n_f + 8ull * c.m_f + 8ull * a.nwords + 24ull * n_new
+                  : 12ull * c.n_f + 8ull * c.m_f + 4ull * n + 16ull * a.nwords + 16ull * n_new;
     c.kernel_bytes += kb;
-    c.alg_bytes += kb + 4ull * n + nwords * 8ull + 12ull * n_new;
+    c.alg_bytes += kb;
+    c.slots += c.pending ? 2u : 1u;
+    c.pending = 0;
     if (n_new == 0) {
         c.done = 1;
     } else {
@@ -454,9 +657,9 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
         c.m_u -= deg_sum;
         c.reached += n_new;
         c.reached_edges += deg_sum;
-        c.n_small = n_small;
-        c.n_large = n_large;
-        c.hub_wgs = bfs_hub_wgs(grid, n_small, n_large, deg_large, deg_sum);
+        c.n_small = n_new - n_hub;
+        c.n_large = n_hub;
+        c.hub_wgs = bfs_hub_wgs(a.grid, c.n_small, n_hub, deg_large, deg_sum);
     }
     bool pull = c.pull != 0;
     if (!c.done) {
@@ -464,12 +667,12 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
             pull = false;
         } else if (c.mode == 2) {
             pull = true;
-        } else if (c.mode == 3 && policy &&
+        } else if (c.mode == 3 && a.policy &&
                    bfs_policy_consulted(pull, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
                                         Sched::kAlphaBfs, Sched::kBeta)) {
             const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
                                                      (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
-            policy_direction_costs(rec, policy, s_mlp);
+            policy_direction_costs(rec, a.policy, s_mlp);
             // the heuristic's choice stands unless the policy predicts a clear gain
             const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs
                                      : !((double)c.n_f < (double)c.n / Sched::kBeta);
@@ -482,6 +685,7 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
                 pull = !((double)c.n_f < (double)c.n / Sched::kBeta);
         }
         c.pull = pull ? 1u : 0u;
+        c.small = bfs_small(pull, c.n_f, c.m_f, a.exact_edges);
         c.supersteps += 1;
         if (pull)
             c.pull_steps += 1;
@@ -489,64 +693,220 @@ __global__ __launch_bounds__(256) void k_bfs_decide(BfsCtl* __restrict__ ctl, ui
             c.push_steps += 1;
     }
     __syncthreads();  // every wave has read the control block and counters
-    if (threadIdx.x == 0) {
-        *ctl = c;
-        ctr->n_small = 0;
-        ctr->n_large = 0;
-        ctr->n_new = 0;
-        ctr->deg_sum = 0;
-        ctr->probes = 0;
-        ctr->deg_large = 0;
+    bfs_clear_counters(ctr, par);
+    if (threadIdx.x == 0) *a.ctl = c;
+}
+
+// ---- kernels (both drivers) ---------------------------------------------------------
+
+// Level reset, counters and bitmap 1 cleared, and the first frontier when there is one source:
+// with `single` < n that vertex is placed at level 0, in bitmap 0 and in queue 0 here;
+// otherwise k_bfs_mark places the sources.
+__global__ void k_bfs_seed(BfsArgs a, uint32_t single) {
+    const uint32_t n = a.n;
+    const bool one = single < n;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        a.level[v] = v == single ? 0 : -1;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwords; w += stride) {
+        a.bm[0][w] = one && w == single / kWave ? 1ull << (single & 63) : 0ull;
+        a.bm[1][w] = 0;  // superstep 0's next bitmap (superstep 0 clears bitmap 2 itself)
+    }
+    if (blockIdx.x == 0) {
+        bfs_clear_counters(a.ctr, 0);
+        bfs_clear_counters(a.ctr, 1);
+        __syncthreads();
+        if (threadIdx.x == 0 && one) {
+            const uint32_t deg = a.row_ptr[single + 1] - a.row_ptr[single];
+            const bool hub = deg > a.large;
+            const uint32_t sg = single / a.span;
+            a.ctr->seg[0][sg].v = bfs_pack(1u, deg);
+            a.q[0][(size_t)sg * a.span] = single;
+            a.ctr->work[0].v = hub ? deg : 0ull;
+            a.ctr->n_hub = hub ? 1u : 0u;
+            if (hub) a.hub[0][0] = single;
+        }
     }
 }
 
+// Several sources: each is placed at level 0, in bitmap 0 and in queue 0; a vertex given more
+// than once is counted once (the lane that sets its bit).  One queue atomic per source.
+__global__ __launch_bounds__(256) void k_bfs_mark(BfsArgs a, const uint32_t* __restrict__ src, uint32_t count) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    unsigned long long hacc = 0;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x - lane); i0 < count;
+         i0 += (uint64_t)gridDim.x * blockDim.x) {  // uniform per wave
+        const uint64_t i = i0 + lane;
+        bool won = false;
+        uint32_t v = 0, deg = 0;
+        if (i < count) {
+            v = src[i];
+            const unsigned long long bit = 1ull << (v & 63);
+            won = !(atomicOr(&a.bm[0][v >> 6], bit) & bit);
+        }
+        if (won) {
+            a.level[v] = 0;
+            deg = a.row_ptr[v + 1] - a.row_ptr[v];
+            const uint32_t sg = v / a.span;
+            const uint32_t off = (uint32_t)atomicAdd(&a.ctr->seg[0][sg].v, bfs_pack(1u, deg));
+            a.q[0][(size_t)sg * a.span + off] = v;
+        }
+        const bool hub = won && deg > a.large;
+        const unsigned long long hm = __ballot(hub);
+        if (hm) {
+            uint32_t hb = 0;
+            if (lane == 0) hb = atomicAdd(&a.ctr->n_hub, (uint32_t)__popcll(hm));
+            hb = __shfl(hb, 0, kWave);
+            if (hub) {
+                a.hub[0][hb + lanes_below(hm)] = v;
+                hacc += deg;
+            }
+        }
+    }
+    bfs_finish<false>(a.ctr, hacc);
+}
+
+// Host-driven superstep t: counters cleared except the frontier's queue segments (parity t % 2).
+__global__ __launch_bounds__(64) void k_zero_counters(BfsCounters* c, uint32_t par_free) {
+    bfs_clear_counters(c, par_free);
+}
+
+// After a push with plain claims: the next frontier = {v : level[v] == step + 1}, as its
+// bitmap (one ballot per 64 vertices), queue, hub queue and counters, in workgroup rounds of
+// kBfsRound consecutive vertices (wave w the chunks w * kRoundChunks + k), whose level loads are
+// issued together.
+__device__ __forceinline__ void compact_level(const BfsArgs& a, int32_t next, unsigned long long* __restrict__ nxt,
+                                              uint32_t* __restrict__ q_next, BfsLine* __restrict__ seg_next,
+                                              uint32_t* __restrict__ hub_next, unsigned long long& hacc,
+                                              BfsAppendLds& L) {
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (uint64_t blk = (uint64_t)blockIdx.x * kBfsRound; blk < a.n; blk += (uint64_t)gridDim.x * kBfsRound) {
+        const uint64_t c0 = blk / kWave + wv * kRoundChunks;  // first chunk of this wave
+        int32_t lv[kRoundChunks];
+#pragma unroll
+        for (uint32_t k = 0; k < kRoundChunks; ++k) {
+            const uint64_t v = (c0 + k) * kWave + lane;
+            lv[k] = v < a.n ? a.level[v] : -1;
+        }
+        unsigned long long jm[kRoundChunks];
+#pragma unroll
+        for (uint32_t k = 0; k < kRoundChunks; ++k) {
+            jm[k] = __ballot(lv[k] == next);
+            if (lane == 0 && c0 + k < a.nwords) nxt[c0 + k] = jm[k];
+        }
+        bfs_append_round(a, jm, blk, q_next, seg_next, hub_next, hacc, L);
+    }
+}
+
+// One launch of the superstep loop, doing what the control block says:
+//   pending: the compaction after a push with plain claims;
+//   pull:    every unvisited vertex; the next frontier's bitmap, queue, hub queue and
+//            counters come out of the same pass;
+//   small:   push with exact claims (the frontier's out-edges fit one queue segment); the
+//            winners are recorded as they are claimed; with at most 64 frontier vertices every
+//            wave takes an equal share of their out-edges (hubs included);
+//   else:    push with plain claims; it sets `pending`, so the next launch compacts.
+// The first hub_wgs workgroups of a push take the hub queue, the rest the frontier queue.  Every
+// expansion clears the bitmap that superstep + 1 builds.  With kDecide the last workgroup to
+// finish a counting phase decides the next superstep; otherwise the host reads the counters.
+template <bool kDecide>
+__global__ __launch_bounds__(256) void k_bfs_super(BfsArgs a) {
+    __shared__ uint32_t s_beg[4][kWave], s_off[4][kWave];
+    __shared__ uint32_t s_pre[kBfsSlots + 1];
+    __shared__ BfsAppendLds s_app;
+    const BfsCtl* ctl = a.ctl;
+    if (ctl->done) return;
+    const int32_t step = ctl->step, next = step + 1;
+    const uint32_t par = (uint32_t)step & 1u, t3 = (uint32_t)step % 3u;
+    unsigned long long* nxt = a.bm[t3 == 2 ? 0 : t3 + 1];
+    uint32_t* q_next = a.q[par ^ 1u];
+    BfsLine* seg_next = a.ctr->seg[par ^ 1u];
+    uint32_t* hub_next = a.hub[par ^ 1u];
+    unsigned long long hacc = 0;
+    if (ctl->pending) {
+        compact_level(a, next, nxt, q_next, seg_next, hub_next, hacc, s_app);
+        if (bfs_finish<kDecide>(a.ctr, hacc)) bfs_decide(a);
+        return;
+    }
+    unsigned long long* clr = a.bm[t3 == 0 ? 2 : t3 - 1];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwords; w += stride) clr[w] = 0;
+    if (ctl->pull) {
+        unsigned long long probes = 0;
+        pull_all(a, a.bm[t3], nxt, q_next, seg_next, hub_next, next, hacc, probes, s_app);
+        if (bfs_finish<kDecide>(a.ctr, hacc + (probes << 32))) bfs_decide(a);
+        return;
+    }
+    const BfsExactOut o{nxt, q_next, seg_next, hub_next};
+    const uint32_t hub = ctl->hub_wgs, small = ctl->small;
+    if (!small && blockIdx.x == 0 && threadIdx.x == 0) a.ctl->pending = 1;  // the only writer of ctl in this launch
+    if (small == 2) {
+        bfs_queue_prefix(a.ctr->seg[par], s_pre);
+        push_tiny(a, a.q[par], s_pre, next, s_beg, s_off, o, hacc);
+    } else if (blockIdx.x < hub) {
+        if (small)
+            push_hubs<true>(a, a.hub[par], ctl->n_large, next, blockIdx.x, hub, o, hacc);
+        else
+            push_hubs<false>(a, a.hub[par], ctl->n_large, next, blockIdx.x, hub, o, hacc);
+    } else {
+        bfs_queue_prefix(a.ctr->seg[par], s_pre);
+        const uint32_t wg = blockIdx.x - hub, nwg = gridDim.x - hub, w0 = wg * 4 + threadIdx.x / kWave;
+        if (small)
+            push_queue<true>(a, a.q[par], s_pre, next, w0, nwg * 4, s_beg, s_off, o, hacc);
+        else
+            push_queue<false>(a, a.q[par], s_pre, next, w0, nwg * 4, s_beg, s_off, o, hacc);
+    }
+    if (small && bfs_finish<kDecide>(a.ctr, hacc)) bfs_decide(a);
+}
+
 // First decision (superstep 0) from the seeded counters; one workgroup.
-__global__ __launch_bounds__(256) void k_bfs_first_decide(BfsCtl* __restrict__ ctl, FrontierCounters* __restrict__ ctr,
-                                                          const float* __restrict__ policy, uint32_t n,
-                                                          unsigned long long m, uint32_t mode, uint32_t grid) {
+__global__ __launch_bounds__(256) void k_bfs_first_decide(BfsArgs a, unsigned long long m, uint32_t mode) {
     __shared__ MlpShared s_mlp;
+    BfsCounters* ctr = a.ctr;
+    unsigned long long joined, work;
+    bfs_sum_counters(ctr, 0, joined, work);
+    const uint32_t n_hub = ctr->n_hub;
     BfsCtl c{};
     c.mode = mode;
-    c.n = n;
+    c.n = a.n;
     c.m = m;
-    c.n_u = n;
-    c.m_u = m;
     c.step = 0;
-    c.n_f = ctr->n_new;
-    c.m_f = ctr->deg_sum;
-    c.n_u -= c.n_f;
-    c.m_u -= c.m_f;
+    c.n_f = (uint32_t)joined;
+    c.m_f = joined >> 32;
+    c.n_u = a.n - c.n_f;
+    c.m_u = m - c.m_f;
     c.reached = c.n_f;
     c.reached_edges = c.m_f;
-    c.n_small = ctr->n_small;
-    c.n_large = ctr->n_large;
-    c.hub_wgs = bfs_hub_wgs(grid, c.n_small, c.n_large, ctr->deg_large, ctr->deg_sum);
+    c.n_small = (uint32_t)c.n_f - n_hub;
+    c.n_large = n_hub;
+    c.hub_wgs = bfs_hub_wgs(a.grid, c.n_small, c.n_large, (uint32_t)work, c.m_f);
     bool pull = false;
     if (c.mode == 2) {
         pull = true;
-    } else if (c.mode == 3 && policy &&
+    } else if (c.mode == 3 && a.policy &&
                bfs_policy_consulted(false, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
                                     Sched::kAlphaBfs, Sched::kBeta)) {
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
-        policy_direction_costs(rec, policy, s_mlp);
+        policy_direction_costs(rec, a.policy, s_mlp);
         pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
     } else if (c.mode == 0 || c.mode == 3) {  // heuristic, or the policy far from its switching point
         pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
     }
     c.pull = pull ? 1u : 0u;
+    c.small = bfs_small(pull, c.n_f, c.m_f, a.exact_edges);
     c.supersteps = 1;
     c.push_steps = pull ? 0 : 1;
     c.pull_steps = pull ? 1 : 0;
     __syncthreads();  // every wave has read the control block and counters
+    if (threadIdx.x < kBfsSlots) {  // the work slots, tickets and hub count; queue 0's counters stay
+        ctr->work[threadIdx.x].v = 0;
+        ctr->sub[threadIdx.x].v = 0;
+    }
     if (threadIdx.x == 0) {
-        *ctl = c;
-        ctr->n_small = 0;
-        ctr->n_large = 0;
-        ctr->n_new = 0;
-        ctr->deg_sum = 0;
-        ctr->probes = 0;
-        ctr->deg_large = 0;
+        ctr->top = 0;
+        ctr->n_hub = 0;
+        *a.ctl = c;
     }
 }
 
@@ -570,47 +930,80 @@ static uint32_t bfs_mode(const medusa_opts& o) {
 // medusa_bfs_noalloc, the caller's level array and the library's static scratch.
 struct BfsBufs {
     int32_t* level;
-    uint32_t *q_small, *q_large, *q_aux;
-    unsigned long long* bitmap;
+    uint32_t* q[2];                 // frontier queues, kBfsSlots * bfs_span(n) entries each
+    uint32_t* hub[2];               // hub queues, n entries each
+    unsigned long long* bm[3];      // frontier bitmaps, (n + 63) / 64 words each
+    const uint32_t* sources;        // several sources (device), or null
     BfsCtl* ctl;
+    BfsCounters* ctr;
 };
+static size_t bfs_queue_cap(uint32_t n) { return (size_t)kBfsSlots * bfs_span(n); }
+// Control block and counters share one allocation: the control block, then the counters.
+constexpr size_t kBfsCtlUnits = 1 + (sizeof(BfsCounters) + sizeof(CtlStorage) - 1) / sizeof(CtlStorage);
 static BfsBufs handle_bufs(medusa_graph* g) {
-    g->bfs_ctl.ensure(1);
+    g->bfs_ctl.ensure(kBfsCtlUnits);
     static_assert(sizeof(BfsCtl) <= sizeof(BfsCtlStorage), "control block storage");
-    return {g->level.p, g->q_small.p, g->q_large.p, g->q_aux.p, g->bitmap.p, reinterpret_cast<BfsCtl*>(g->bfs_ctl.p)};
+    const size_t nwords = ((size_t)g->n + 63) / 64;
+    const size_t qcap = bfs_queue_cap(g->n);
+    return {g->level.p,
+            {g->q_small.p, g->q_small.p + qcap},
+            {g->q_large.p, g->q_large.p + g->n},
+            {g->bitmap.p, g->bitmap.p + nwords, g->bitmap.p + 2 * nwords},
+            g->q_aux.p,
+            reinterpret_cast<BfsCtl*>(g->bfs_ctl.p),
+            reinterpret_cast<BfsCounters*>(g->bfs_ctl.p + 1)};
 }
 
-// Seeding: every source at level 0 (one source by value, several from B.q_aux), then the
-// compaction of level 0 builds the first frontier's bitmap, queues and counters exactly
-// as a superstep would.
-static void bfs_seed(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uint32_t single, uint32_t large) {
-    const uint32_t n = g->n, nwords = (n + 63) / 64;
+// Frontier out-edges up to which a push claims exactly instead of compacting after it.
+static unsigned long long bfs_exact_edges(uint32_t n) { return bfs_span(n); }
+
+static BfsArgs bfs_args(medusa_graph* g, const BfsBufs& B, uint32_t large, uint32_t grid, const float* policy) {
+    BfsArgs a{};
+    a.row_ptr = g->row_ptr.p;
+    a.col = g->col.p;
+    a.cptr = g->cptr.p;
+    a.csrc = g->csrc.p;
+    a.level = B.level;
+    a.q[0] = B.q[0];
+    a.q[1] = B.q[1];
+    a.hub[0] = B.hub[0];
+    a.hub[1] = B.hub[1];
+    for (int i = 0; i < 3; ++i) a.bm[i] = B.bm[i];
+    a.ctr = B.ctr;
+    a.ctl = B.ctl;
+    a.policy = policy;
+    a.n = g->n;
+    a.nwords = (g->n + 63) / 64;
+    a.large = large;
+    a.grid = grid;
+    a.span = bfs_span(g->n);
+    a.exact_edges = bfs_exact_edges(g->n);
+    return a;
+}
+
+// Seeding: every source at level 0 and in bitmap 0 (one source by value, several from
+// B.sources), the first hub queue and the counters of the first frontier.
+static void bfs_seed(medusa_graph* g, const BfsBufs& B, const BfsArgs& a, uint32_t nsrc, uint32_t single) {
     hipStream_t s = g->stream;
-    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
-    hipLaunchKernelGGL(k_bfs_reset, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, B.level,
-                       nsrc == 1 ? single : 0xFFFFFFFFu, ctr);
+    hipLaunchKernelGGL(k_bfs_seed, dim3(grid_for(g->n, 256, 4096)), dim3(256), 0, s, a,
+                       nsrc == 1 ? single : 0xFFFFFFFFu);
     MDS_LAUNCHED();
     if (nsrc > 1) {
-        hipLaunchKernelGGL(k_bfs_mark, dim3(grid_for(nsrc, 256, 1024)), dim3(256), 0, s, (const uint32_t*)B.q_aux,
-                           nsrc, B.level);
+        hipLaunchKernelGGL(k_bfs_mark, dim3(grid_for(nsrc, 256, 1024)), dim3(256), 0, s, a, B.sources, nsrc);
         MDS_LAUNCHED();
     }
-    launch_compact(s, nullptr, n, 0, (const int32_t*)B.level, g->row_ptr.p, large, B.q_small, B.q_large, B.bitmap,
-                   nwords, ctr);
-    MDS_LAUNCHED();
 }
 
-// One device-driven superstep slot: expansion, compaction, decision.
-static void bfs_enqueue_slot(medusa_graph* g, const BfsBufs& B, uint32_t large, uint32_t grid, const float* policy) {
-    const uint32_t n = g->n, nwords = (n + 63) / 64;
-    hipStream_t s = g->stream;
-    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
-    hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)B.ctl, (const uint32_t*)B.q_small,
-                       (const uint32_t*)B.q_large, g->row_ptr.p, g->col.p, g->cptr.p, g->csrc.p,
-                       (const unsigned long long*)B.bitmap, B.level, n, ctr);
-    launch_compact(s, B.ctl, n, 0, (const int32_t*)B.level, g->row_ptr.p, large, B.q_small, B.q_large, B.bitmap,
-                   nwords, ctr);
-    hipLaunchKernelGGL(k_bfs_decide, dim3(1), dim3(256), 0, s, B.ctl, n, nwords, ctr, policy, grid);
+// One device-driven slot: one launch of the superstep kernel (a push with plain claims takes two
+// slots, its expansion and its compaction; every other superstep one).
+static void bfs_enqueue_slot(medusa_graph* g, const BfsArgs& a) {
+    hipLaunchKernelGGL(k_bfs_super<true>, dim3(a.grid), dim3(256), 0, g->stream, a);
+}
+
+// Bytes of the seeding: the level reset (4 per vertex), bitmap 0 (8 per word) and the sources
+// (16 each: id, level, bit OR).
+static uint64_t bfs_seed_bytes(uint32_t n, uint32_t nwords, uint32_t nsrc) {
+    return 4ull * n + 8ull * nwords + 16ull * nsrc;
 }
 
 // Device-driven run: seed, first decision, then batches of superstep slots until the control
@@ -625,17 +1018,16 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
     hipStream_t s = g->stream;
     const uint32_t mode = bfs_mode(o);
     const float* policy = mode == 3 ? policy_device_params(o.policy) : nullptr;
-    BfsCtl* ctl = B.ctl;
-    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
+    const BfsArgs a = bfs_args(g, B, large, grid, policy);
 
     // capture (once per graph, threshold, policy buffer and slot count) a batch of supersteps
     auto captured = [&](GraphCache& gc, uint32_t slots) -> hipGraphExec_t {
-        const uintptr_t key[4] = {(uintptr_t)B.level, (uintptr_t)policy, ((uintptr_t)grid << 32) | large,
-                                  (uintptr_t)B.bitmap ^ ((uintptr_t)slots << 48)};
+        const uintptr_t key[4] = {(uintptr_t)B.level ^ ((uintptr_t)B.hub[0] << 16), (uintptr_t)policy,
+                                  ((uintptr_t)grid << 32) | large, (uintptr_t)B.bm[0] ^ ((uintptr_t)slots << 48)};
         if (gc.exec && std::equal(key, key + 4, gc.key)) return gc.exec;
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (uint32_t k = 0; k < slots; ++k) bfs_enqueue_slot(g, B, large, grid, policy);
+        for (uint32_t k = 0; k < slots; ++k) bfs_enqueue_slot(g, a);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
@@ -648,9 +1040,8 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
     const hipGraphExec_t head_exec = capture ? captured(head == kBfsBatch ? g->bfs_graph : g->bfs_head, head) : nullptr;
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    bfs_seed(g, B, nsrc, single, large);
-    hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, ctl, ctr, policy, n, (unsigned long long)m,
-                       mode, grid);
+    bfs_seed(g, B, a, nsrc, single);
+    hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, a, (unsigned long long)m, mode);
     MDS_LAUNCHED();
     BfsCtl h{};
     for (uint32_t batch = 0;; ++batch) {
@@ -659,16 +1050,16 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
         if (capture) {
             MDS_HIP(hipGraphLaunch(batch == 0 ? head_exec : captured(g->bfs_graph, kBfsBatch), s));
         } else {
-            for (uint32_t k = 0; k < kBfsBatch; ++k) bfs_enqueue_slot(g, B, large, grid, policy);
+            for (uint32_t k = 0; k < kBfsBatch; ++k) bfs_enqueue_slot(g, a);
             MDS_LAUNCHED();
         }
-        MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
+        MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, B.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
         MDS_HIP(hipStreamSynchronize(s));
         std::memcpy(&h, g->h_ctr_raw.p, sizeof(BfsCtl));
         if (h.done) break;
-        MDS_CHECK(batch < (n + kBfsBatch) / kBfsBatch + 2, MEDUSA_ERR_INTERNAL, "bfs: superstep loop did not end");
+        MDS_CHECK(batch < (2ull * n + 2) / kBfsBatch + 2, MEDUSA_ERR_INTERNAL, "bfs: superstep loop did not end");
     }
-    if (capture) g->bfs_last_steps = h.supersteps;
+    if (capture) g->bfs_last_steps = h.slots;
     MDS_HIP(hipEventRecord(g->ev_end, s));
     MDS_HIP(hipStreamSynchronize(s));
     float ms = 0;
@@ -680,12 +1071,38 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
     st.edges_traversed = h.reached_edges;
     st.reached = (uint32_t)h.reached;
     st.kernel_bytes = h.kernel_bytes;
-    st.alg_bytes = h.alg_bytes + 8ull * n + nwords * 8ull + 8ull * nsrc;
+    st.alg_bytes = h.alg_bytes + bfs_seed_bytes(n, nwords, nsrc);
     if (out != B.level) copy_out(g, out, B.level, (size_t)n * 4);
 }
 
-// Host-driven run: the same expansion and compaction kernels, with the host deciding each
-// superstep from counters read back after it (timing and tracing).
+// Sums of the counter slots, read back to the host (waits for the stream).
+struct BfsSums {
+    uint32_t n_new = 0, n_hub = 0;
+    unsigned long long deg_sum = 0, deg_large = 0, probes = 0;
+};
+static BfsSums read_bfs_counters(medusa_graph* g, const BfsCounters* ctr, uint32_t par_next) {
+    g->h_ctr_raw.ensure((sizeof(BfsCounters) + 7) / 8);
+    MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, ctr, sizeof(BfsCounters), hipMemcpyDeviceToHost, g->stream));
+    MDS_HIP(hipStreamSynchronize(g->stream));
+    BfsCounters h;
+    std::memcpy(&h, g->h_ctr_raw.p, sizeof(BfsCounters));
+    unsigned long long joined = 0, work = 0;
+    for (uint32_t i = 0; i < kBfsSlots; ++i) {
+        joined += h.seg[par_next][i].v;
+        work += h.work[i].v;
+    }
+    BfsSums r;
+    r.n_new = (uint32_t)joined;
+    r.deg_sum = joined >> 32;
+    r.deg_large = (uint32_t)work;
+    r.probes = work >> 32;
+    r.n_hub = h.n_hub;
+    return r;
+}
+
+// Host-driven run: the same kernels, with the host deciding each superstep from the counters
+// read back after it (timing and tracing); a superstep's time covers its expansion and, after
+// a push, the compaction.
 static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_t* out, const medusa_opts& o,
                          medusa_stats& st) {
     const uint32_t n = g->n;
@@ -694,23 +1111,22 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
     const uint32_t large = bfs_large(g, o), grid = bfs_grid(g, o);
     hipStream_t s = g->stream;
     const bool timing = o.timing != 0;
-    FrontierCounters* ctr = dev_counters<FrontierCounters>(g);
     const BfsBufs B = handle_bufs(g);
-    BfsCtl* ctl = B.ctl;
+    const BfsArgs a = bfs_args(g, B, large, grid, nullptr);
     g->trace.clear();
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    bfs_seed(g, B, nsrc, single, large);
-    FrontierCounters c = read_counters<FrontierCounters>(g);
+    bfs_seed(g, B, a, nsrc, single);
+    BfsSums c = read_bfs_counters(g, B.ctr, 0);
 
     Frontier f{};
     f.n = n;
     f.m = m;
     f.n_f = c.n_new;
     f.m_f = c.deg_sum;
-    f.n_u = n - c.n_new;
-    f.m_u = m - c.deg_sum;
-    uint64_t reached_edges = c.deg_sum, reached = c.n_new, bytes = 8ull * n + nwords * 8ull + 8ull * nsrc;
+    f.n_u = n - f.n_f;
+    f.m_u = m - f.m_f;
+    uint64_t reached_edges = f.m_f, reached = f.n_f, bytes = bfs_seed_bytes(n, nwords, nsrc);
     bool pull = false;
     Sched sched(o, Sched::kBfs);
     if (timing) ensure_events(g, 2);
@@ -719,22 +1135,23 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
         if (o.direction == MEDUSA_DIR_PUSH) pull = false;
         else if (o.direction == MEDUSA_DIR_PULL) pull = true;
         else pull = sched.bfs_pull(f, pull);
-        const uint32_t hub = bfs_hub_wgs(grid, c.n_small, c.n_large, c.deg_large, c.deg_sum);
-        hipLaunchKernelGGL(k_bfs_set_ctl, dim3(1), dim3(1), 0, s, ctl, (int32_t)t, pull ? 1u : 0u, c.n_small,
-                           c.n_large, hub);
+        const uint32_t n_small = c.n_new - c.n_hub;
+        const uint32_t hub = bfs_hub_wgs(grid, n_small, c.n_hub, c.deg_large, c.deg_sum);
+        const uint32_t small = bfs_small(pull, f.n_f, f.m_f, a.exact_edges);
+        hipLaunchKernelGGL(k_bfs_set_ctl, dim3(1), dim3(1), 0, s, B.ctl, (int32_t)t, pull ? 1u : 0u, small, n_small,
+                           c.n_hub, hub);
         MDS_LAUNCHED();
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, ctr);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(kWave), 0, s, a.ctr, (t & 1u) ^ 1u);
         MDS_LAUNCHED();
         if (timing) MDS_HIP(hipEventRecord(g->events[0], s));
-        hipLaunchKernelGGL(k_bfs_expand_dev, dim3(grid), dim3(256), 0, s, (const BfsCtl*)ctl,
-                           (const uint32_t*)g->q_small.p, (const uint32_t*)g->q_large.p, g->row_ptr.p, g->col.p,
-                           g->cptr.p, g->csrc.p, (const unsigned long long*)g->bitmap.p, g->level.p, n, ctr);
+        hipLaunchKernelGGL(k_bfs_super<false>, dim3(grid), dim3(256), 0, s, a);
         MDS_LAUNCHED();
+        if (!pull && !small) {  // the compaction (the push set `pending`)
+            hipLaunchKernelGGL(k_bfs_super<false>, dim3(grid), dim3(256), 0, s, a);
+            MDS_LAUNCHED();
+        }
         if (timing) MDS_HIP(hipEventRecord(g->events[1], s));
-        launch_compact(s, nullptr, n, (int32_t)(t + 1), (const int32_t*)g->level.p, g->row_ptr.p, large,
-                       g->q_small.p, g->q_large.p, g->bitmap.p, nwords, ctr);
-        MDS_LAUNCHED();
-        c = read_counters<FrontierCounters>(g);
+        c = read_bfs_counters(g, B.ctr, (t & 1u) ^ 1u);
         double kms = 0;
         if (timing) {
             float ms = 0;
@@ -743,19 +1160,20 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
             st.kernel_ms += ms;
             ++st.kernel_launches;
         }
-        // byte model: push = queue + offsets + col + level probe + claims;
-        // pull = 2 bitmap passes + level and offsets of unvisited vertices + probed in-edges
-        // (source + bitmap word) + claims
+        // the byte model of the device-driven loop's decision
         uint64_t kb;
-        if (!pull) {
-            kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * c.n_new;
+        if (pull) {
+            kb = 4ull * n + 8ull * f.n_u + 8ull * c.probes + 16ull * nwords + 16ull * c.n_new;
+            ++st.pull_steps;
+        } else if (small) {
+            kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * nwords + 24ull * c.n_new;
             ++st.push_steps;
         } else {
-            kb = n / 4 + 8ull * f.n_u + 8ull * c.probes + 4ull * c.n_new;
-            ++st.pull_steps;
+            kb = 12ull * f.n_f + 8ull * f.m_f + 4ull * n + 16ull * nwords + 16ull * c.n_new;
+            ++st.push_steps;
         }
         st.kernel_bytes += kb;
-        bytes += kb + 4ull * n + nwords * 8ull + 12ull * c.n_new;
+        bytes += kb;
         sched.record(g, f, pull ? 1.0 : 0.0, kms);
         ++st.supersteps;
         if (c.n_new == 0) break;
@@ -790,10 +1208,10 @@ static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, 
     MDS_CHECK(o.direction >= 0 && o.direction <= 2, MEDUSA_ERR_ARG, "bfs: bad direction %d", o.direction);
     MDS_CHECK(o.grid == 0 || (o.grid >= 2 && o.grid <= 65535), MEDUSA_ERR_ARG, "bfs: grid must be 0 or 2..65535");
     g->level.ensure(n);
-    g->q_small.ensure(n);
-    g->q_large.ensure(n);
+    g->q_small.ensure(2 * bfs_queue_cap(n));
+    g->q_large.ensure(2 * (size_t)n);
     g->q_aux.ensure(std::max(n, count));
-    g->bitmap.ensure((n + 63) / 64);
+    g->bitmap.ensure(3 * (((size_t)n + 63) / 64));
     if (count > 1)
         MDS_HIP(hipMemcpyAsync(g->q_aux.p, sources, (size_t)count * 4, hipMemcpyHostToDevice, g->stream));
     medusa_stats st{};
@@ -803,15 +1221,17 @@ static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, 
 }
 
 // ---- BFS with no device allocation ------------------------------------------------
-// Work buffers of medusa_bfs_noalloc: two frontier queues of kNoAllocMaxVertices entries,
-// the frontier bitmap and a control block, in one array per GPU that lives as long as the
+// Work buffers of medusa_bfs_noalloc: two frontier queues and two hub queues of
+// kNoAllocMaxVertices entries each, three frontier bitmaps, the control block and the counters, in one array per GPU that lives as long as the
 // process.  Host code must not pass `scratch` itself to a kernel (in host code the name is
 // a stand-in, not a device address): the device address is looked up once per device with
 // hipGetSymbolAddress and kept in s_scratch.
 constexpr uint32_t kNoAllocMaxVertices = MEDUSA_NOALLOC_MAX_VERTICES;
 constexpr size_t kNoAllocQueueBytes = (size_t)kNoAllocMaxVertices * 4;
 constexpr size_t kNoAllocBitmapBytes = (size_t)kNoAllocMaxVertices / 8;
-constexpr size_t kNoAllocBytes = 2 * kNoAllocQueueBytes + kNoAllocBitmapBytes + sizeof(CtlStorage);
+static_assert((size_t)kBfsSlots * bfs_span(kNoAllocMaxVertices) == kNoAllocMaxVertices,
+              "the segmented queue of the largest graph fills kNoAllocMaxVertices entries exactly");
+constexpr size_t kNoAllocBytes = 4 * kNoAllocQueueBytes + 3 * kNoAllocBitmapBytes + kBfsCtlUnits * sizeof(CtlStorage);
 static_assert(kNoAllocMaxVertices % 2048 == 0, "queue and bitmap regions stay 256-byte aligned");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -847,11 +1267,18 @@ static void bfs_noalloc(medusa_graph* g, uint32_t source, int32_t* levels, medus
     unsigned char* base = scratch_on(g->device);
     BfsBufs B{};
     B.level = levels;
-    B.q_small = reinterpret_cast<uint32_t*>(base);
-    B.q_large = reinterpret_cast<uint32_t*>(base + kNoAllocQueueBytes);
-    B.bitmap = reinterpret_cast<unsigned long long*>(base + 2 * kNoAllocQueueBytes);
-    B.ctl = reinterpret_cast<BfsCtl*>(base + 2 * kNoAllocQueueBytes + kNoAllocBitmapBytes);
-    B.q_aux = nullptr;  // one source: not used
+    // a graph of n <= kNoAllocMaxVertices vertices needs at most kNoAllocMaxVertices entries per queue
+    MDS_CHECK(bfs_queue_cap(n) <= kNoAllocMaxVertices, MEDUSA_ERR_INTERNAL, "bfs_noalloc: queue layout");
+    for (int i = 0; i < 2; ++i) {
+        B.q[i] = reinterpret_cast<uint32_t*>(base + i * kNoAllocQueueBytes);
+        B.hub[i] = reinterpret_cast<uint32_t*>(base + (2 + i) * kNoAllocQueueBytes);
+    }
+    for (int i = 0; i < 3; ++i)
+        B.bm[i] = reinterpret_cast<unsigned long long*>(base + 4 * kNoAllocQueueBytes + i * kNoAllocBitmapBytes);
+    CtlStorage* ctl = reinterpret_cast<CtlStorage*>(base + 4 * kNoAllocQueueBytes + 3 * kNoAllocBitmapBytes);
+    B.ctl = reinterpret_cast<BfsCtl*>(ctl);
+    B.ctr = reinterpret_cast<BfsCounters*>(ctl + 1);
+    B.sources = nullptr;  // one source: not used
     static_assert(sizeof(BfsCtl) <= sizeof(CtlStorage), "control block storage");
     medusa_opts o{};  // heuristic direction, default or tuned launch shape
     medusa_stats st{};

@@ -70,7 +70,8 @@ __device__ inline void mlp_stage(const float* __restrict__ P, float* sP) {
 // units of all rows are computed in parallel, one (row, unit) per thread; out[r] is written
 // for every row.  The FP32 operations and their order per row are those of mlp_eval_host,
 // so host and device pick the same action.  Every thread of the block must call it; out[]
-// is valid afterwards.
+// is valid afterwards.  The dot products are unrolled by 4 only: kernels that evaluate the
+// policy in their last workgroup keep their register budget (and occupancy) for the main body.
 __device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, const float* x, int rows, float* s_a1,
                                      float* s_a2, float* out, bool staged = false) {
     const int t = threadIdx.x, nt = blockDim.x;
@@ -84,6 +85,7 @@ __device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, con
     for (int j = t; j < rows * kMlpH; j += nt) {
         const int r = j / kMlpH, i = j % kMlpH;
         float z = b1[i];
+#pragma unroll 4
         for (int k = 0; k < kMlpIn; ++k) z = fmaf(W1[i * kMlpIn + k], x[r * kMlpIn + k], z);
         s_a1[j] = z > 0.f ? z : z * kMlpSlope;
     }
@@ -91,12 +93,14 @@ __device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, con
     for (int j = t; j < rows * kMlpH; j += nt) {
         const int r = j / kMlpH, i = j % kMlpH;
         float z = b2[i];
+#pragma unroll 4
         for (int k = 0; k < kMlpH; ++k) z = fmaf(W2[i * kMlpH + k], s_a1[r * kMlpH + k], z);
         s_a2[j] = z > 0.f ? z : z * kMlpSlope;
     }
     __syncthreads();
     for (int r = t; r < rows; r += nt) {
         float o = W3[kMlpH];
+#pragma unroll 4
         for (int i = 0; i < kMlpH; ++i) o = fmaf(W3[i], s_a2[r * kMlpH + i], o);
         out[r] = o;
     }
