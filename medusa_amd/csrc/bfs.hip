@@ -699,10 +699,72 @@ __device__ __forceinline__ void bfs_decide(const BfsArgs& a) {
 
 // ---- kernels (both drivers) ---------------------------------------------------------
 
+// First decision (superstep 0) from the first frontier's counts (one workgroup; every thread
+// passes the same values): the control block is written and the work slots, tickets and hub
+// count are cleared (queue 0's counters stay: superstep 0 reads them).
+__device__ __forceinline__ void bfs_first_decision(const BfsArgs& a, unsigned long long joined,
+                                                   unsigned long long work, uint32_t n_hub, unsigned long long m,
+                                                   uint32_t mode) {
+    __shared__ MlpShared s_mlp;
+    BfsCounters* ctr = a.ctr;
+    BfsCtl c{};
+    c.mode = mode;
+    c.n = a.n;
+    c.m = m;
+    c.step = 0;
+    c.n_f = (uint32_t)joined;
+    c.m_f = joined >> 32;
+    c.n_u = a.n - c.n_f;
+    c.m_u = m - c.m_f;
+    c.reached = c.n_f;
+    c.reached_edges = c.m_f;
+    c.n_small = (uint32_t)c.n_f - n_hub;
+    c.n_large = n_hub;
+    c.hub_wgs = bfs_hub_wgs(a.grid, c.n_small, c.n_large, (uint32_t)work, c.m_f);
+    bool pull = false;
+    if (c.mode == 2) {
+        pull = true;
+    } else if (c.mode == 3 && a.policy &&
+               bfs_policy_consulted(false, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
+                                    Sched::kAlphaBfs, Sched::kBeta)) {
+        const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
+                                                 (double)c.n, (double)c.m, 0.0, 0.0};
+        policy_direction_costs(rec, a.policy, s_mlp);
+        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
+    } else if (c.mode == 0 || c.mode == 3) {  // heuristic, or the policy far from its switching point
+        pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
+    }
+    c.pull = pull ? 1u : 0u;
+    c.small = bfs_small(pull, c.n_f, c.m_f, a.exact_edges);
+    c.supersteps = 1;
+    c.push_steps = pull ? 0 : 1;
+    c.pull_steps = pull ? 1 : 0;
+    __syncthreads();  // every wave has read the counters
+    if (threadIdx.x < kBfsSlots) {
+        ctr->work[threadIdx.x].v = 0;
+        ctr->sub[threadIdx.x].v = 0;
+    }
+    if (threadIdx.x == 0) {
+        ctr->top = 0;
+        ctr->n_hub = 0;
+        *a.ctl = c;
+    }
+}
+
+// First decision after k_bfs_mark placed several sources; one workgroup.
+__global__ __launch_bounds__(256) void k_bfs_first_decide(BfsArgs a, unsigned long long m, uint32_t mode) {
+    unsigned long long joined, work;
+    bfs_sum_counters(a.ctr, 0, joined, work);
+    bfs_first_decision(a, joined, work, a.ctr->n_hub, m, mode);
+}
+
 // Level reset, counters and bitmap 1 cleared, and the first frontier when there is one source:
-// with `single` < n that vertex is placed at level 0, in bitmap 0 and in queue 0 here;
-// otherwise k_bfs_mark places the sources.
-__global__ void k_bfs_seed(BfsArgs a, uint32_t single) {
+// with `single` < n that vertex is placed at level 0, in bitmap 0 and in queue 0 here, and with
+// kDecide workgroup 0 also makes the first decision (the device-driven loop); otherwise
+// k_bfs_mark places the sources.
+template <bool kDecide>
+__global__ __launch_bounds__(256) void k_bfs_seed(BfsArgs a, uint32_t single, unsigned long long m, uint32_t mode) {
+    __shared__ uint32_t s_deg;
     const uint32_t n = a.n;
     const bool one = single < n;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -712,20 +774,30 @@ __global__ void k_bfs_seed(BfsArgs a, uint32_t single) {
         a.bm[0][w] = one && w == single / kWave ? 1ull << (single & 63) : 0ull;
         a.bm[1][w] = 0;  // superstep 0's next bitmap (superstep 0 clears bitmap 2 itself)
     }
-    if (blockIdx.x == 0) {
-        bfs_clear_counters(a.ctr, 0);
-        bfs_clear_counters(a.ctr, 1);
-        __syncthreads();
-        if (threadIdx.x == 0 && one) {
-            const uint32_t deg = a.row_ptr[single + 1] - a.row_ptr[single];
-            const bool hub = deg > a.large;
+    if (blockIdx.x != 0) return;
+    bfs_clear_counters(a.ctr, 0);
+    bfs_clear_counters(a.ctr, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t deg = one ? a.row_ptr[single + 1] - a.row_ptr[single] : 0u;
+        const bool hub = one && deg > a.large;
+        s_deg = deg;
+        if (one) {
             const uint32_t sg = single / a.span;
             a.ctr->seg[0][sg].v = bfs_pack(1u, deg);
             a.q[0][(size_t)sg * a.span] = single;
+        }
+        if (hub) a.hub[0][0] = single;
+        if (!kDecide) {  // the host reads the first frontier's counters
             a.ctr->work[0].v = hub ? deg : 0ull;
             a.ctr->n_hub = hub ? 1u : 0u;
-            if (hub) a.hub[0][0] = single;
         }
+    }
+    if constexpr (kDecide) {
+        __syncthreads();
+        const uint32_t deg = s_deg;
+        const bool hub = deg > a.large;
+        bfs_first_decision(a, bfs_pack(1u, deg), hub ? deg : 0ull, hub ? 1u : 0u, m, mode);
     }
 }
 
@@ -859,57 +931,6 @@ __global__ __launch_bounds__(256) void k_bfs_super(BfsArgs a) {
     if (small && bfs_finish<kDecide>(a.ctr, hacc)) bfs_decide(a);
 }
 
-// First decision (superstep 0) from the seeded counters; one workgroup.
-__global__ __launch_bounds__(256) void k_bfs_first_decide(BfsArgs a, unsigned long long m, uint32_t mode) {
-    __shared__ MlpShared s_mlp;
-    BfsCounters* ctr = a.ctr;
-    unsigned long long joined, work;
-    bfs_sum_counters(ctr, 0, joined, work);
-    const uint32_t n_hub = ctr->n_hub;
-    BfsCtl c{};
-    c.mode = mode;
-    c.n = a.n;
-    c.m = m;
-    c.step = 0;
-    c.n_f = (uint32_t)joined;
-    c.m_f = joined >> 32;
-    c.n_u = a.n - c.n_f;
-    c.m_u = m - c.m_f;
-    c.reached = c.n_f;
-    c.reached_edges = c.m_f;
-    c.n_small = (uint32_t)c.n_f - n_hub;
-    c.n_large = n_hub;
-    c.hub_wgs = bfs_hub_wgs(a.grid, c.n_small, c.n_large, (uint32_t)work, c.m_f);
-    bool pull = false;
-    if (c.mode == 2) {
-        pull = true;
-    } else if (c.mode == 3 && a.policy &&
-               bfs_policy_consulted(false, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
-                                    Sched::kAlphaBfs, Sched::kBeta)) {
-        const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
-                                                 (double)c.n, (double)c.m, 0.0, 0.0};
-        policy_direction_costs(rec, a.policy, s_mlp);
-        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
-    } else if (c.mode == 0 || c.mode == 3) {  // heuristic, or the policy far from its switching point
-        pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
-    }
-    c.pull = pull ? 1u : 0u;
-    c.small = bfs_small(pull, c.n_f, c.m_f, a.exact_edges);
-    c.supersteps = 1;
-    c.push_steps = pull ? 0 : 1;
-    c.pull_steps = pull ? 1 : 0;
-    __syncthreads();  // every wave has read the control block and counters
-    if (threadIdx.x < kBfsSlots) {  // the work slots, tickets and hub count; queue 0's counters stay
-        ctr->work[threadIdx.x].v = 0;
-        ctr->sub[threadIdx.x].v = 0;
-    }
-    if (threadIdx.x == 0) {
-        ctr->top = 0;
-        ctr->n_hub = 0;
-        *a.ctl = c;
-    }
-}
-
 // Degree threshold of the hub queue and workgroups of the expansion: the caller's value,
 // else the handle's autotuned one, else the default.
 static uint32_t bfs_large(const medusa_graph* g, const medusa_opts& o) {
@@ -982,15 +1003,25 @@ static BfsArgs bfs_args(medusa_graph* g, const BfsBufs& B, uint32_t large, uint3
 }
 
 // Seeding: every source at level 0 and in bitmap 0 (one source by value, several from
-// B.sources), the first hub queue and the counters of the first frontier.
-static void bfs_seed(medusa_graph* g, const BfsBufs& B, const BfsArgs& a, uint32_t nsrc, uint32_t single) {
+// B.sources), the first queue, hub queue and counters; with `decide` also the first decision
+// (inside the seed kernel when there is one source).
+static void bfs_seed(medusa_graph* g, const BfsBufs& B, const BfsArgs& a, uint32_t nsrc, uint32_t single, bool decide,
+                     uint32_t mode) {
     hipStream_t s = g->stream;
-    hipLaunchKernelGGL(k_bfs_seed, dim3(grid_for(g->n, 256, 4096)), dim3(256), 0, s, a,
-                       nsrc == 1 ? single : 0xFFFFFFFFu);
+    const dim3 grid(grid_for(g->n, 256, 4096));
+    const unsigned long long m = g->m;
+    if (nsrc == 1 && decide)
+        hipLaunchKernelGGL(k_bfs_seed<true>, grid, dim3(256), 0, s, a, single, m, mode);
+    else
+        hipLaunchKernelGGL(k_bfs_seed<false>, grid, dim3(256), 0, s, a, nsrc == 1 ? single : 0xFFFFFFFFu, m, mode);
     MDS_LAUNCHED();
     if (nsrc > 1) {
         hipLaunchKernelGGL(k_bfs_mark, dim3(grid_for(nsrc, 256, 1024)), dim3(256), 0, s, a, B.sources, nsrc);
         MDS_LAUNCHED();
+        if (decide) {
+            hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, a, m, mode);
+            MDS_LAUNCHED();
+        }
     }
 }
 
@@ -1039,10 +1070,15 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
     const uint32_t head = head_slots(g->bfs_last_steps, kBfsBatch);
     const hipGraphExec_t head_exec = capture ? captured(head == kBfsBatch ? g->bfs_graph : g->bfs_head, head) : nullptr;
 
+    // A device output is copied behind every batch, so the usual run (one batch) waits for the
+    // stream once; a run that takes more batches copies again after the next one.
+    hipPointerAttribute_t attr{};
+    const bool out_dev = out != B.level && hipPointerGetAttributes(&attr, out) == hipSuccess &&
+                         attr.type == hipMemoryTypeDevice;
+    (void)hipGetLastError();  // host pointers unknown to HIP report an error
+
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    bfs_seed(g, B, a, nsrc, single);
-    hipLaunchKernelGGL(k_bfs_first_decide, dim3(1), dim3(256), 0, s, a, (unsigned long long)m, mode);
-    MDS_LAUNCHED();
+    bfs_seed(g, B, a, nsrc, single, true, mode);
     BfsCtl h{};
     for (uint32_t batch = 0;; ++batch) {
         // the tail batch is captured after the timed region has started only on a handle's
@@ -1053,6 +1089,8 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
             for (uint32_t k = 0; k < kBfsBatch; ++k) bfs_enqueue_slot(g, a);
             MDS_LAUNCHED();
         }
+        MDS_HIP(hipEventRecord(g->ev_end, s));
+        if (out_dev) MDS_HIP(hipMemcpyAsync(out, B.level, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
         MDS_HIP(hipMemcpyAsync(g->h_ctr_raw.p, B.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, s));
         MDS_HIP(hipStreamSynchronize(s));
         std::memcpy(&h, g->h_ctr_raw.p, sizeof(BfsCtl));
@@ -1060,8 +1098,6 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
         MDS_CHECK(batch < (2ull * n + 2) / kBfsBatch + 2, MEDUSA_ERR_INTERNAL, "bfs: superstep loop did not end");
     }
     if (capture) g->bfs_last_steps = h.slots;
-    MDS_HIP(hipEventRecord(g->ev_end, s));
-    MDS_HIP(hipStreamSynchronize(s));
     float ms = 0;
     MDS_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
     st.device_ms = ms;
@@ -1072,7 +1108,7 @@ static void bfs_run_device(medusa_graph* g, const BfsBufs& B, uint32_t nsrc, uin
     st.reached = (uint32_t)h.reached;
     st.kernel_bytes = h.kernel_bytes;
     st.alg_bytes = h.alg_bytes + bfs_seed_bytes(n, nwords, nsrc);
-    if (out != B.level) copy_out(g, out, B.level, (size_t)n * 4);
+    if (out != B.level && !out_dev) copy_out(g, out, B.level, (size_t)n * 4);
 }
 
 // Sums of the counter slots, read back to the host (waits for the stream).
@@ -1116,7 +1152,7 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
     g->trace.clear();
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
-    bfs_seed(g, B, a, nsrc, single);
+    bfs_seed(g, B, a, nsrc, single, false, 0);
     BfsSums c = read_bfs_counters(g, B.ctr, 0);
 
     Frontier f{};
