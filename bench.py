@@ -42,8 +42,9 @@ def parse():
                    help="PageRank superstep kernel variant (all give bit-identical ranks); default = the "
                         "engine's choice (binned)")
     p.add_argument("--profile", action="store_true",
-                   help="short run for hardware counters: 3 PageRank runs (non-graph launches) and one "
-                        "256 MiB device copy as the byte-count calibration; prints no JSON")
+                   help="short run for hardware counters: 3 PageRank runs (non-graph launches), or with "
+                        "--workload bfs_rmat20 / bfs_er 4 BFS runs launched kernel by kernel, and one 256 MiB "
+                        "device copy as the byte-count calibration; prints no JSON")
     return p.parse_args()
 
 
@@ -53,11 +54,16 @@ def profile_pass(G, out, torch, variant):
     for _ in range(3):
         G.pagerank(0.85, STEPS_PR, out=out, variant=variant)
     torch.cuda.synchronize()
-    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+    _calibration_copy(torch)
+    print("profile pass done: 90 supersteps, copy of 268435456 bytes", flush=True)
+
+
+def _calibration_copy(torch):
+    """A plain device copy of 256 MiB: the counters' bytes-per-unit calibration."""
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     b.copy_(a)
     torch.cuda.synchronize()
-    print("profile pass done: 90 supersteps, copy of 268435456 bytes", flush=True)
 
 
 def traffic_from_profiles(name):
@@ -277,15 +283,22 @@ def _timed(fn, steps, warmup, torch):
     return time.perf_counter() - t0, res
 
 
-def _roofline(stats, kernel):
+def _roofline(stats, kernel, traffic=None):
+    """HIP events around every superstep (all its launches, with the gaps between them) of
+    host-driven timing runs on the library's stream; bytes = the engine's algorithmic model."""
     kms = sum(s.kernel_ms for s in stats)
     kb = sum(s.kernel_bytes for s in stats)
     launches = sum(s.kernel_launches for s in stats)
     achieved = kb / (kms / 1e3) / 1e9 if kms > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(traffic) if traffic else None, "kernel": kernel,
             "kernel_avg_us": round(kms / max(1, launches) * 1e3, 2),
-            "alg_bytes_per_launch": int(kb / max(1, launches))}
+            "alg_bytes_per_launch": int(kb / max(1, launches)),
+            "launch": "one superstep (every launch of it, with the gaps between them)"}
+
+
+BFS_KERNEL = "k_bfs_super (superstep: pull, exact-claim push, or plain push + compaction)"
 
 
 def _emit(a, metric_cfg, value, ms_per_step, roofline, cpu, extra):
@@ -375,8 +388,8 @@ def bfs_block(md, wl, torch, np, rp, col, src, steps, warmup, oracle_check=False
     d = {"gteps": round(steps * m_r / wall / 1e9, 3), "ms": round(wall / steps * 1e3, 4), "source": src,
          "supersteps": res[-1].supersteps, "push_pull_steps": [res[-1].push_steps, res[-1].pull_steps],
          "edges_reached": m_r,
-         "roofline": _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)],
-                               "k_bfs_expand_dev (push or pull superstep)"),
+         "roofline": _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)], BFS_KERNEL,
+                               "bfs_rmat20_traffic.md"),
          "gteps_learned": round(steps * m_r / wl_ / 1e9, 3), "learned_over_plain": round(wall / wl_, 3),
          "levels_identical_to_plain": bool(np.array_equal(plain, out.cpu().numpy()))}
     if oracle_check:
@@ -485,6 +498,14 @@ def run_workload(a, md, wl, torch, np):
         cfg["dtype"] = "int32"
         G = md.Graph(rp, col)
         out = torch.empty(n, dtype=torch.int32, device="cuda")
+        if a.profile:  # counter pass: 4 runs launched kernel by kernel (no graph), then the calibration copy
+            for _ in range(4):
+                G.bfs_noalloc(src, out)
+            torch.cuda.synchronize()
+            _calibration_copy(torch)
+            print("profile pass done: 4 BFS runs, copy of 268435456 bytes", flush=True)
+            G.close()
+            return
         wall, res = _timed(lambda: G.bfs(src, out=out)[1], a.steps, a.warmup, torch)
         m_r = res[-1].edges_traversed
         extra = {"supersteps": res[-1].supersteps, "push_steps": res[-1].push_steps,
@@ -501,7 +522,8 @@ def run_workload(a, md, wl, torch, np):
                       "levels_identical_to_plain": bool(np.array_equal(plain_levels, out.cpu().numpy())),
                       "policy": "direction per superstep by the MLP, trained on the GPU from push-only/pull-only "
                                 "traces of smaller RMAT, Erdos-Renyi and Chung-Lu graphs (not this graph)"})
-        roof = _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)], "k_bfs_push / k_bfs_pull")
+        roof = _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)], BFS_KERNEL,
+                         "bfs_rmat20_traffic.md" if w == "bfs_rmat20" else None)
         if w == "chung_lu":
             pout = torch.empty(n, dtype=torch.float32, device="cuda")
             wpr, _ = _timed(lambda: G.pagerank(0.85, 20, out=pout, use_graph=1), a.steps, 1, torch)

@@ -130,6 +130,7 @@ struct BfsArgs {
     uint32_t* q[2];    // segmented queues, kBfsSlots * span entries each
     uint32_t* hub[2];  // hub queues, n entries each
     unsigned long long* bm[3];
+    unsigned long long* visited;  // one bit per vertex that has a level (n / 8 bytes, stays in L2)
     BfsCounters* ctr;
     BfsCtl* ctl;
     const float* policy;  // learned direction policy (device parameters) or null
@@ -306,18 +307,23 @@ struct BfsExactOut {
     uint32_t* hub_next;
 };
 
-// A push claim of v for level `next`.  Plain: `if (level[v] < 0) level[v] = next`, a plain
+// A push claim of v for level `next`; a vertex whose visited bit is set is skipped (the bitmap
+// is small enough to stay in L2, the level array is not).  Plain: `level[v] = next`, a plain
 // store; every writer stores the same value, so the race is benign (as in the reference) and
 // the compaction after the push finds the vertices that joined.  Exact: a compare-and-swap
-// from -1 picks one winner per vertex, which is recorded at once (bfs_record_round).
+// from -1 picks one winner per vertex, which sets its visited bit and is recorded at once
+// (bfs_record_round).
 template <bool kExact>
-__device__ __forceinline__ bool bfs_claim(int32_t* __restrict__ level, uint32_t v, int32_t next) {
-    if (level[v] >= 0) return false;
+__device__ __forceinline__ bool bfs_claim(const BfsArgs& a, uint32_t v, int32_t next) {
+    const unsigned long long bit = 1ull << (v & 63);
+    if (a.visited[v >> 6] & bit) return false;
     if constexpr (!kExact) {
-        level[v] = next;
+        a.level[v] = next;
         return false;
     } else {
-        return atomicCAS(&level[v], -1, next) == -1;
+        if (atomicCAS(&a.level[v], -1, next) != -1) return false;
+        atomicOr(&a.visited[v >> 6], bit);
+        return true;
     }
 }
 
@@ -419,7 +425,7 @@ __device__ __forceinline__ void push_queue(const BfsArgs& a, const uint32_t* __r
             bool won[kPushUnroll];
 #pragma unroll
             for (uint32_t j = 0; j < kPushUnroll; ++j)
-                won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<kExact>(a.level, v[j], next);
+                won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<kExact>(a, v[j], next);
             if constexpr (kExact) bfs_record_round(a, o, won, v, hacc);
         }
         __builtin_amdgcn_wave_barrier();
@@ -463,7 +469,7 @@ __device__ __forceinline__ void push_tiny(const BfsArgs& a, const uint32_t* __re
     }
     bool won[kPushUnroll];
 #pragma unroll
-    for (uint32_t j = 0; j < kPushUnroll; ++j) won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<true>(a.level, v[j], next);
+    for (uint32_t j = 0; j < kPushUnroll; ++j) won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<true>(a, v[j], next);
     bfs_record_round(a, o, won, v, hacc);
 }
 
@@ -492,7 +498,7 @@ __device__ __forceinline__ void push_hubs(const BfsArgs& a, const uint32_t* __re
             }
 #pragma unroll
             for (uint32_t j = 0; j < kPushUnroll; ++j)
-                won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<kExact>(a.level, v[j], next);
+                won[j] = v[j] != 0xFFFFFFFFu && bfs_claim<kExact>(a, v[j], next);
             if constexpr (kExact) bfs_record_round(a, o, won, v, hacc);
         }
     }
@@ -502,7 +508,7 @@ constexpr uint32_t kPullLane = 32;  // in-degree up to which one lane scans a ve
 constexpr uint32_t kPullGroup = 4;  // in-edges a lane probes per round
 
 // Bottom-up over 64 consecutive vertices from `base` (one per lane; the wave is converged): an
-// unvisited vertex joins if any in-neighbour is in the frontier bitmap.  A lane scans a vertex
+// unvisited vertex (visited bitmap) joins if any in-neighbour is in the frontier bitmap.  A lane scans a vertex
 // of in-degree <= kPullLane alone; above that the whole wave scans it 64 in-edges at a time and
 // stops at the first hit (ballot), so one high in-degree vertex does not hold its wave for
 // hundreds of sequential probes.  Returns the join mask, which is also written as the chunk's
@@ -515,9 +521,10 @@ __device__ __forceinline__ unsigned long long pull_chunk(const BfsArgs& a, const
     const uint32_t n = a.n;
     if (base >= n) return 0;  // uniform across the wave
     const uint32_t v = (uint32_t)base + lane;
+    const unsigned long long vis = a.visited[base / kWave];
     uint32_t b = 0, e = 0;
     bool todo = false, joined = false;
-    if (v < n && a.level[v] < 0) {
+    if (v < n && !((vis >> lane) & 1ull)) {
         b = a.cptr[v];
         e = a.cptr[v + 1];
         todo = b < e;
@@ -570,7 +577,10 @@ __device__ __forceinline__ unsigned long long pull_chunk(const BfsArgs& a, const
         }
     }
     const unsigned long long jm = __ballot(joined) | bigjoin;
-    if (lane == 0) nxt[base / kWave] = jm;
+    if (lane == 0) {
+        nxt[base / kWave] = jm;
+        a.visited[base / kWave] = vis | jm;  // the wave owns this word in a pull
+    }
     return jm;
 }
 
@@ -771,7 +781,7 @@ __global__ __launch_bounds__(256) void k_bfs_seed(BfsArgs a, uint32_t single, un
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
         a.level[v] = v == single ? 0 : -1;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwords; w += stride) {
-        a.bm[0][w] = one && w == single / kWave ? 1ull << (single & 63) : 0ull;
+        a.bm[0][w] = a.visited[w] = one && w == single / kWave ? 1ull << (single & 63) : 0ull;
         a.bm[1][w] = 0;  // superstep 0's next bitmap (superstep 0 clears bitmap 2 itself)
     }
     if (blockIdx.x != 0) return;
@@ -815,6 +825,7 @@ __global__ __launch_bounds__(256) void k_bfs_mark(BfsArgs a, const uint32_t* __r
             v = src[i];
             const unsigned long long bit = 1ull << (v & 63);
             won = !(atomicOr(&a.bm[0][v >> 6], bit) & bit);
+            if (won) atomicOr(&a.visited[v >> 6], bit);
         }
         if (won) {
             a.level[v] = 0;
@@ -844,7 +855,8 @@ __global__ __launch_bounds__(64) void k_zero_counters(BfsCounters* c, uint32_t p
 }
 
 // After a push with plain claims: the next frontier = {v : level[v] == step + 1}, as its
-// bitmap (one ballot per 64 vertices), queue, hub queue and counters, in workgroup rounds of
+// bitmap (one ballot per 64 vertices), queue, hub queue and counters, and the visited bitmap
+// rebuilt from the levels, in workgroup rounds of
 // kBfsRound consecutive vertices (wave w the chunks w * kRoundChunks + k), whose level loads are
 // issued together.
 __device__ __forceinline__ void compact_level(const BfsArgs& a, int32_t next, unsigned long long* __restrict__ nxt,
@@ -864,7 +876,11 @@ __device__ __forceinline__ void compact_level(const BfsArgs& a, int32_t next, un
 #pragma unroll
         for (uint32_t k = 0; k < kRoundChunks; ++k) {
             jm[k] = __ballot(lv[k] == next);
-            if (lane == 0 && c0 + k < a.nwords) nxt[c0 + k] = jm[k];
+            const unsigned long long vm = __ballot(lv[k] >= 0);
+            if (lane == 0 && c0 + k < a.nwords) {
+                nxt[c0 + k] = jm[k];
+                a.visited[c0 + k] = vm;
+            }
         }
         bfs_append_round(a, jm, blk, q_next, seg_next, hub_next, hacc, L);
     }
@@ -954,6 +970,7 @@ struct BfsBufs {
     uint32_t* q[2];                 // frontier queues, kBfsSlots * bfs_span(n) entries each
     uint32_t* hub[2];               // hub queues, n entries each
     unsigned long long* bm[3];      // frontier bitmaps, (n + 63) / 64 words each
+    unsigned long long* visited;    // visited bitmap, (n + 63) / 64 words
     const uint32_t* sources;        // several sources (device), or null
     BfsCtl* ctl;
     BfsCounters* ctr;
@@ -970,6 +987,7 @@ static BfsBufs handle_bufs(medusa_graph* g) {
             {g->q_small.p, g->q_small.p + qcap},
             {g->q_large.p, g->q_large.p + g->n},
             {g->bitmap.p, g->bitmap.p + nwords, g->bitmap.p + 2 * nwords},
+            g->bitmap.p + 3 * nwords,
             g->q_aux.p,
             reinterpret_cast<BfsCtl*>(g->bfs_ctl.p),
             reinterpret_cast<BfsCounters*>(g->bfs_ctl.p + 1)};
@@ -990,6 +1008,7 @@ static BfsArgs bfs_args(medusa_graph* g, const BfsBufs& B, uint32_t large, uint3
     a.hub[0] = B.hub[0];
     a.hub[1] = B.hub[1];
     for (int i = 0; i < 3; ++i) a.bm[i] = B.bm[i];
+    a.visited = B.visited;
     a.ctr = B.ctr;
     a.ctl = B.ctl;
     a.policy = policy;
@@ -1247,7 +1266,7 @@ static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, 
     g->q_small.ensure(2 * bfs_queue_cap(n));
     g->q_large.ensure(2 * (size_t)n);
     g->q_aux.ensure(std::max(n, count));
-    g->bitmap.ensure(3 * (((size_t)n + 63) / 64));
+    g->bitmap.ensure(4 * (((size_t)n + 63) / 64));
     if (count > 1)
         MDS_HIP(hipMemcpyAsync(g->q_aux.p, sources, (size_t)count * 4, hipMemcpyHostToDevice, g->stream));
     medusa_stats st{};
@@ -1258,16 +1277,17 @@ static void bfs_entry(medusa_graph* g, const uint32_t* sources, uint32_t count, 
 
 // ---- BFS with no device allocation ------------------------------------------------
 // Work buffers of medusa_bfs_noalloc: two frontier queues and two hub queues of
-// kNoAllocMaxVertices entries each, three frontier bitmaps, the control block and the counters, in one array per GPU that lives as long as the
-// process.  Host code must not pass `scratch` itself to a kernel (in host code the name is
-// a stand-in, not a device address): the device address is looked up once per device with
-// hipGetSymbolAddress and kept in s_scratch.
+// kNoAllocMaxVertices entries each, three frontier bitmaps and the visited bitmap, the control
+// block and the counters, in one array per GPU that lives as long as the process.  Host code
+// must not pass `scratch` itself to a kernel (in host code the name is a stand-in, not a device
+// address): the device address is looked up once per device with hipGetSymbolAddress and kept
+// in s_scratch.
 constexpr uint32_t kNoAllocMaxVertices = MEDUSA_NOALLOC_MAX_VERTICES;
 constexpr size_t kNoAllocQueueBytes = (size_t)kNoAllocMaxVertices * 4;
 constexpr size_t kNoAllocBitmapBytes = (size_t)kNoAllocMaxVertices / 8;
 static_assert((size_t)kBfsSlots * bfs_span(kNoAllocMaxVertices) == kNoAllocMaxVertices,
               "the segmented queue of the largest graph fills kNoAllocMaxVertices entries exactly");
-constexpr size_t kNoAllocBytes = 4 * kNoAllocQueueBytes + 3 * kNoAllocBitmapBytes + kBfsCtlUnits * sizeof(CtlStorage);
+constexpr size_t kNoAllocBytes = 4 * kNoAllocQueueBytes + 4 * kNoAllocBitmapBytes + kBfsCtlUnits * sizeof(CtlStorage);
 static_assert(kNoAllocMaxVertices % 2048 == 0, "queue and bitmap regions stay 256-byte aligned");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -1311,7 +1331,8 @@ static void bfs_noalloc(medusa_graph* g, uint32_t source, int32_t* levels, medus
     }
     for (int i = 0; i < 3; ++i)
         B.bm[i] = reinterpret_cast<unsigned long long*>(base + 4 * kNoAllocQueueBytes + i * kNoAllocBitmapBytes);
-    CtlStorage* ctl = reinterpret_cast<CtlStorage*>(base + 4 * kNoAllocQueueBytes + 3 * kNoAllocBitmapBytes);
+    B.visited = reinterpret_cast<unsigned long long*>(base + 4 * kNoAllocQueueBytes + 3 * kNoAllocBitmapBytes);
+    CtlStorage* ctl = reinterpret_cast<CtlStorage*>(base + 4 * kNoAllocQueueBytes + 4 * kNoAllocBitmapBytes);
     B.ctl = reinterpret_cast<BfsCtl*>(ctl);
     B.ctr = reinterpret_cast<BfsCounters*>(ctl + 1);
     B.sources = nullptr;  // one source: not used
