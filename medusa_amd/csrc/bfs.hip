@@ -627,95 +627,120 @@ __device__ __forceinline__ void bfs_clear_counters(BfsCounters* __restrict__ ctr
     }
 }
 
-// The next superstep from this one's counters (one workgroup, after every workgroup of the
-// superstep has added its counters): direction by the heuristic or the learned policy, hub
-// queue split and the termination vote, written into the control block; the counters are
-// cleared for the next superstep.
+// The next superstep from this one's counters (the deciding workgroup's first wave, after
+// every workgroup of the superstep has added its counters): direction by the heuristic or the
+// learned policy, hub queue split and the termination vote, written into the control block;
+// the counters are cleared for the next superstep.  The updated control block waits in LDS
+// while the policy is evaluated, so it holds no registers there.
 __device__ __forceinline__ void bfs_decide(const BfsArgs& a) {
-    __shared__ MlpShared s_mlp;
+    __shared__ BfsCtl s_c;
+    if (threadIdx.x >= kWave) return;
     BfsCounters* ctr = a.ctr;
-    BfsCtl c = load_now_block(a.ctl);
-    const uint32_t par = (uint32_t)c.step & 1u;
-    unsigned long long joined, work;
-    bfs_sum_counters(ctr, par ^ 1u, joined, work);
-    const uint32_t n_hub = load_now(&ctr->n_hub);
-    const uint32_t n = a.n, n_new = (uint32_t)joined;
-    const unsigned long long deg_sum = joined >> 32, deg_large = (uint32_t)work, probes = work >> 32;
-    // bytes of the superstep just finished (every expansion clears one bitmap: 8 per word):
-    // push = queue entry and out-offsets per frontier vertex (12) + target id and level probe
-    // per out-edge (8); exact claims then add compare-and-swap, bit OR, out-offsets and queue
-    // entry per join (24), plain claims the store per join (4) and the compaction's level pass
-    // (4n), next bitmap (8 per word), out-offsets and queue entry per join (12); pull = level of
-    // every vertex (4n) + in-offsets of unvisited vertices (8) + source id and bitmap word per
-    // probed in-edge (8) + next bitmap (8 per word) + level, out-offsets and queue entry per
-    // join (16)
-    const unsigned long long kb =
-        c.pull    ? 4ull * n + 8ull * c.n_u + 8ull * probes + 16ull * a.nwords + 16ull * n_new
-        : c.small ? 12ull * c.n_f + 8ull * c.m_f + 8ull * a.nwords + 24ull * n_new
-                  : 12ull * c.n_f + 8ull * c.m_f + 4ull * n + 16ull * a.nwords + 16ull * n_new;
-    c.kernel_bytes += kb;
-    c.alg_bytes += kb;
-    c.slots += c.pending ? 2u : 1u;
-    c.pending = 0;
-    if (n_new == 0) {
-        c.done = 1;
-    } else {
-        c.step += 1;
-        c.n_f = n_new;
-        c.m_f = deg_sum;
-        c.n_u -= n_new;
-        c.m_u -= deg_sum;
-        c.reached += n_new;
-        c.reached_edges += deg_sum;
-        c.n_small = n_new - n_hub;
-        c.n_large = n_hub;
-        c.hub_wgs = bfs_hub_wgs(a.grid, c.n_small, n_hub, deg_large, deg_sum);
-    }
-    bool pull = c.pull != 0;
-    if (!c.done) {
-        if (c.mode == 1) {
-            pull = false;
-        } else if (c.mode == 2) {
-            pull = true;
-        } else if (c.mode == 3 && a.policy &&
-                   bfs_policy_consulted(pull, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
-                                        Sched::kAlphaBfs, Sched::kBeta)) {
-            const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.step, (double)c.n_f, (double)c.m_f, (double)c.n_u,
-                                                     (double)c.m_u, (double)c.n, (double)c.m, 0.0, 0.0};
-            policy_direction_costs(rec, a.policy, s_mlp);
-            // the heuristic's choice stands unless the policy predicts a clear gain
-            const bool plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs
-                                     : !((double)c.n_f < (double)c.n / Sched::kBeta);
-            pull = policy_pull(s_mlp.out[0], s_mlp.out[1], plain);
+    const uint32_t lane = threadIdx.x;
+    uint32_t par;
+    bool pull, ask = false, plain = false, done;
+    double rec[MEDUSA_TRACE_FIELDS] = {};
+    {
+        BfsCtl c = load_now_block(a.ctl);
+        par = (uint32_t)c.step & 1u;
+        unsigned long long joined, work;
+        bfs_sum_counters(ctr, par ^ 1u, joined, work);
+        const uint32_t n_hub = load_now(&ctr->n_hub);
+        const uint32_t n = a.n, n_new = (uint32_t)joined;
+        const unsigned long long deg_sum = joined >> 32, deg_large = (uint32_t)work, probes = work >> 32;
+        // bytes of the superstep just finished (every expansion clears one bitmap: 8 per word):
+        // push = queue entry and out-offsets per frontier vertex (12) + target id and visited bit
+        // per out-edge (8); exact claims then add compare-and-swap (4), two bit ORs (16),
+        // out-offsets (8) and queue entry (4) per join, plain claims the store per join (4) and the compaction's level
+        // pass (4n), next and visited bitmaps (16 per word), out-offsets and queue entry per join
+        // (12); pull = visited word per 64 vertices and in-offsets of unvisited vertices (8) +
+        // source id and bitmap word per probed in-edge (8) + next and visited bitmaps (16 per
+        // word) + level, out-offsets and queue entry per join (16)
+        const unsigned long long kb =
+            c.pull    ? 8ull * c.n_u + 8ull * probes + 32ull * a.nwords + 16ull * n_new
+            : c.small ? 12ull * c.n_f + 8ull * c.m_f + 8ull * a.nwords + 32ull * n_new
+                      : 12ull * c.n_f + 8ull * c.m_f + 4ull * n + 24ull * a.nwords + 16ull * n_new;
+        c.kernel_bytes += kb;
+        c.alg_bytes += kb;
+        c.slots += c.pending ? 2u : 1u;
+        c.pending = 0;
+        if (n_new == 0) {
+            c.done = 1;
         } else {
-            // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
-            if (!pull)
-                pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
-            else
-                pull = !((double)c.n_f < (double)c.n / Sched::kBeta);
+            c.step += 1;
+            c.n_f = n_new;
+            c.m_f = deg_sum;
+            c.n_u -= n_new;
+            c.m_u -= deg_sum;
+            c.reached += n_new;
+            c.reached_edges += deg_sum;
+            c.n_small = n_new - n_hub;
+            c.n_large = n_hub;
+            c.hub_wgs = bfs_hub_wgs(a.grid, c.n_small, n_hub, deg_large, deg_sum);
         }
-        c.pull = pull ? 1u : 0u;
-        c.small = bfs_small(pull, c.n_f, c.m_f, a.exact_edges);
-        c.supersteps += 1;
-        if (pull)
-            c.pull_steps += 1;
-        else
-            c.push_steps += 1;
+        done = c.done != 0;
+        pull = c.pull != 0;
+        if (!done) {
+            if (c.mode == 1) {
+                pull = false;
+            } else if (c.mode == 2) {
+                pull = true;
+            } else if (c.mode == 3 && a.policy &&
+                       bfs_policy_consulted(pull, (double)c.n_f, (double)c.m_f, (double)c.m_u, (double)c.n,
+                                            Sched::kAlphaBfs, Sched::kBeta)) {
+                ask = true;
+                // the heuristic's choice stands unless the policy predicts a clear gain
+                plain = !pull ? (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs
+                              : !((double)c.n_f < (double)c.n / Sched::kBeta);
+                rec[0] = (double)c.step;
+                rec[1] = (double)c.n_f;
+                rec[2] = (double)c.m_f;
+                rec[3] = (double)c.n_u;
+                rec[4] = (double)c.m_u;
+                rec[5] = (double)c.n;
+                rec[6] = (double)c.m;
+            } else {
+                // direction-optimising heuristic (Beamer, Asanovic, Patterson, SC'12)
+                if (!pull)
+                    pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
+                else
+                    pull = !((double)c.n_f < (double)c.n / Sched::kBeta);
+            }
+        }
+        if (lane == 0) s_c = c;
     }
-    __syncthreads();  // every wave has read the control block and counters
-    bfs_clear_counters(ctr, par);
-    if (threadIdx.x == 0) *a.ctl = c;
+    if (ask) {  // uniform across the wave
+        __shared__ float s_params[kMlpParams];
+        float c_push, c_pull;
+        policy_direction_costs_wave(rec, a.policy, s_params, c_push, c_pull);
+        pull = policy_pull(c_push, c_pull, plain);
+    }
+    bfs_clear_counters(ctr, par);  // this wave read them above
+    if (lane == 0) {
+        BfsCtl c = s_c;
+        if (!done) {
+            c.pull = pull ? 1u : 0u;
+            c.small = bfs_small(pull, c.n_f, c.m_f, a.exact_edges);
+            c.supersteps += 1;
+            if (pull)
+                c.pull_steps += 1;
+            else
+                c.push_steps += 1;
+        }
+        *a.ctl = c;
+    }
 }
 
 // ---- kernels (both drivers) ---------------------------------------------------------
 
-// First decision (superstep 0) from the first frontier's counts (one workgroup; every thread
-// passes the same values): the control block is written and the work slots, tickets and hub
-// count are cleared (queue 0's counters stay: superstep 0 reads them).
+// First decision (superstep 0) from the first frontier's counts, by the calling workgroup's
+// first wave (the others return; every thread passes the same values): the control block is
+// written and the work slots, tickets and hub count are cleared (queue 0's counters stay:
+// superstep 0 reads them).
 __device__ __forceinline__ void bfs_first_decision(const BfsArgs& a, unsigned long long joined,
                                                    unsigned long long work, uint32_t n_hub, unsigned long long m,
                                                    uint32_t mode) {
-    __shared__ MlpShared s_mlp;
+    if (threadIdx.x >= kWave) return;
     BfsCounters* ctr = a.ctr;
     BfsCtl c{};
     c.mode = mode;
@@ -739,8 +764,10 @@ __device__ __forceinline__ void bfs_first_decision(const BfsArgs& a, unsigned lo
                                     Sched::kAlphaBfs, Sched::kBeta)) {
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
-        policy_direction_costs(rec, a.policy, s_mlp);
-        pull = policy_pull(s_mlp.out[0], s_mlp.out[1], (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
+        __shared__ float s_params[kMlpParams];
+        float c_push, c_pull;
+        policy_direction_costs_wave(rec, a.policy, s_params, c_push, c_pull);
+        pull = policy_pull(c_push, c_pull, (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);
     } else if (c.mode == 0 || c.mode == 3) {  // heuristic, or the policy far from its switching point
         pull = (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs;
     }
@@ -749,7 +776,6 @@ __device__ __forceinline__ void bfs_first_decision(const BfsArgs& a, unsigned lo
     c.supersteps = 1;
     c.push_steps = pull ? 0 : 1;
     c.pull_steps = pull ? 1 : 0;
-    __syncthreads();  // every wave has read the counters
     if (threadIdx.x < kBfsSlots) {
         ctr->work[threadIdx.x].v = 0;
         ctr->sub[threadIdx.x].v = 0;
@@ -1050,10 +1076,10 @@ static void bfs_enqueue_slot(medusa_graph* g, const BfsArgs& a) {
     hipLaunchKernelGGL(k_bfs_super<true>, dim3(a.grid), dim3(256), 0, g->stream, a);
 }
 
-// Bytes of the seeding: the level reset (4 per vertex), bitmap 0 (8 per word) and the sources
-// (16 each: id, level, bit OR).
+// Bytes of the seeding: the level reset (4 per vertex), bitmaps 0 and 1 and the visited bitmap
+// (24 per word) and the sources (16 each: id, level, bit OR).
 static uint64_t bfs_seed_bytes(uint32_t n, uint32_t nwords, uint32_t nsrc) {
-    return 4ull * n + 8ull * nwords + 16ull * nsrc;
+    return 4ull * n + 24ull * nwords + 16ull * nsrc;
 }
 
 // Device-driven run: seed, first decision, then batches of superstep slots until the control
@@ -1218,13 +1244,13 @@ static void bfs_run_host(medusa_graph* g, uint32_t nsrc, uint32_t single, int32_
         // the byte model of the device-driven loop's decision
         uint64_t kb;
         if (pull) {
-            kb = 4ull * n + 8ull * f.n_u + 8ull * c.probes + 16ull * nwords + 16ull * c.n_new;
+            kb = 8ull * f.n_u + 8ull * c.probes + 32ull * nwords + 16ull * c.n_new;
             ++st.pull_steps;
         } else if (small) {
-            kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * nwords + 24ull * c.n_new;
+            kb = 12ull * f.n_f + 8ull * f.m_f + 8ull * nwords + 32ull * c.n_new;
             ++st.push_steps;
         } else {
-            kb = 12ull * f.n_f + 8ull * f.m_f + 4ull * n + 16ull * nwords + 16ull * c.n_new;
+            kb = 12ull * f.n_f + 8ull * f.m_f + 4ull * n + 24ull * nwords + 16ull * c.n_new;
             ++st.push_steps;
         }
         st.kernel_bytes += kb;

@@ -106,5 +106,59 @@ __device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, con
     }
     __syncthreads();
 }
+
+// Parameters copied into LDS by one wave: every lane issues its 16-byte loads before its first
+// store (one memory round trip), and LDS keeps one wave's accesses in order, so the wave may
+// read sP right after (no barrier; other waves must not use sP).
+constexpr int kMlpVec = (int)((kMlpParams + 3) / 4);           // 16-byte vectors of the parameters
+constexpr int kMlpVecPerLane = (kMlpVec + 63) / 64;
+__device__ inline void mlp_stage_wave(const float* __restrict__ P, float* sP) {
+    const int lane = threadIdx.x & 63;
+    const float4* P4 = reinterpret_cast<const float4*>(P);
+    float4 r[kMlpVecPerLane];
+#pragma unroll
+    for (int j = 0; j < kMlpVecPerLane; ++j) {
+        const int v = lane + 64 * j;
+        if (v < kMlpVec - 1) {
+            r[j] = P4[v];
+        } else if (v == kMlpVec - 1) {  // the last vector may be partial
+            r[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int e = 0; e < 4 && 4 * v + e < (int)kMlpParams; ++e) (&r[j].x)[e] = P[4 * v + e];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kMlpVecPerLane; ++j) {
+        const int v = lane + 64 * j;
+        if (v < kMlpVec) reinterpret_cast<float4*>(sP)[v] = r[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One wave evaluates the MLP on two input rows with the parameters in LDS (mlp_stage_wave):
+// lanes 0-31 pass row 0 in x, lanes 32-63 row 1, and lane 32 * r + i computes hidden unit i of
+// row r; activations move by shuffles, so there is no barrier.  The FP32 operations and their
+// order per row are those of mlp_eval_host.  Every lane gets both outputs; the calling wave
+// must be converged.
+__device__ inline void mlp_eval_wave2(const float* sP, const float* x, float& out0, float& out1) {
+    const int lane = threadIdx.x & 63, r = lane >> 5, i = lane & 31;
+    const float* W1 = sP;
+    const float* b1 = W1 + kMlpH * kMlpIn;
+    const float* W2 = b1 + kMlpH;
+    const float* b2 = W2 + kMlpH * kMlpH;
+    const float* W3 = b2 + kMlpH;
+    float z = b1[i];
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) z = fmaf(W1[i * kMlpIn + k], x[k], z);
+    const float a1 = z > 0.f ? z : z * kMlpSlope;
+    float z2 = b2[i];
+#pragma unroll 8
+    for (int k = 0; k < kMlpH; ++k) z2 = fmaf(W2[i * kMlpH + k], __shfl(a1, r * kMlpH + k, 64), z2);
+    const float a2 = z2 > 0.f ? z2 : z2 * kMlpSlope;
+    float o = W3[kMlpH];
+#pragma unroll 8
+    for (int k = 0; k < kMlpH; ++k) o = fmaf(W3[k], __shfl(a2, r * kMlpH + k, 64), o);
+    out0 = __shfl(o, 0, 64);
+    out1 = __shfl(o, kMlpH, 64);
+}
 }  // namespace medusa
 #endif

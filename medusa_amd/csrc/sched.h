@@ -105,6 +105,23 @@ struct MlpShared {
     float out[12];
 };
 
+// Predicted costs of push and pull for the superstep described by `rec`, by one converged wave
+// with `sP` (kMlpParams floats of LDS) to itself: the parameters are staged while lanes 0-6
+// compute the logarithms, then lanes 0-31 evaluate the push row and lanes 32-63 the pull row;
+// the rows are the same as direction_row's.
+__device__ inline void policy_direction_costs_wave(const double* rec, const float* __restrict__ policy, float* sP,
+                                                   float& c_push, float& c_pull) {
+    const int lane = threadIdx.x & 63;
+    mlp_stage_wave(policy, sP);
+    const float lg = lane < kDirectionLogs ? direction_log(rec, lane) : 0.f;
+    float L[kDirectionLogs];
+#pragma unroll
+    for (int i = 0; i < kDirectionLogs; ++i) L[i] = __shfl(lg, i, 64);
+    float x[MEDUSA_POLICY_INPUTS];
+    direction_fill(rec, L, (double)(lane >> 5), x);
+    mlp_eval_wave2(sP, x, c_push, c_pull);
+}
+
 // Predicted costs of push (out[0]) and pull (out[1]) for the superstep described by `rec`.
 // Every thread of the block calls it; `staged`: the kernel already ran mlp_stage into m.P.
 // The logarithms are computed by seven threads at once (in double, a long dependent chain
