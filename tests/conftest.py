@@ -41,3 +41,43 @@ def oracle():
     import oracle as orc
     orc.lib()
     return orc
+
+
+# Trained policies of the learned scheduler, shared by the GPU tests (trained once per session
+# on the GPU, from graphs other than the ones the tests run them on).
+@pytest.fixture(scope="session")
+def direction_policy():
+    import policies
+    pol = policies.train_direction_policy()
+    yield pol
+    pol.close()
+
+
+@pytest.fixture(scope="session")
+def mbfs_policy():
+    import policies
+    pol = policies.train_mbfs_policy()
+    yield pol
+    pol.close()
+
+
+@pytest.fixture(scope="session")
+def delta_policy():
+    import policies
+    pol = policies.train_delta_policy()
+    yield pol
+    pol.close()
+
+
+@pytest.fixture(scope="session")
+def variant_data():
+    import policies
+    return policies.variant_training_set()
+
+
+@pytest.fixture(scope="session")
+def variant_policy(variant_data):
+    import policies
+    pol = policies.train_variant_policy(*variant_data)
+    yield pol
+    pol.close()

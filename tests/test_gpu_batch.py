@@ -148,19 +148,29 @@ def test_set_batch_rejects_crossing_edges():
         assert list(G.batch()) == [0, 4, 6]
 
 
-def test_chung_lu_batch_config(oracle):
+def test_chung_lu_batch_config(oracle, direction_policy, variant_policy):
     """The 128-graph Chung-Lu batch (n log-uniform in 2^10..2^18, edge factor 8, seed 5):
-    direction-optimising BFS per graph bit-exact, 20-step PageRank per graph within
-    L1 1e-6 of FP64 (bit-exact against the FP32 restatement on every 8th graph)."""
+    direction-optimising BFS per graph bit-exact (heuristic, learned direction, autotuned launch
+    shape), 20-step PageRank per graph within L1 1e-6 of FP64 (bit-exact against the FP32
+    restatement on every 8th graph; the learned kernel-variant policy gives the same bits)."""
     graphs, sizes = wl.chung_lu_batch()
     assert len(graphs) == 128 and sizes.min() >= 2 ** 10 - 1 and sizes.max() <= 2 ** 18
     with md.Graph.packed(graphs) as G:
         begin = G.batch()
         local = [wl.max_outdeg_vertex(grp) for grp, _ in graphs]
-        lv, st = G.bfs_multi(np.array([begin[k] + s for k, s in enumerate(local)], np.uint32))
+        sources = np.array([begin[k] + s for k, s in enumerate(local)], np.uint32)
+        lv, st = G.bfs_multi(sources)
         for k, (grp, gcol) in enumerate(graphs):
             assert np.array_equal(lv[begin[k]:begin[k + 1]], oracle.bfs_queue(grp, gcol, local[k])), k
+        lv_l, _ = G.bfs_multi(sources, policy=direction_policy)
+        assert np.array_equal(lv_l, lv), "learned direction"
+        G.autotune(md.TUNE_BFS, sources, reps=1)
+        for pol in (None, direction_policy):
+            lv_t, _ = G.bfs_multi(sources, policy=pol)
+            assert np.array_equal(lv_t, lv), ("autotuned", pol is not None)
         r, _ = G.pagerank(0.85, 20)
+        r_v, _ = G.pagerank(0.85, 20, policy=variant_policy)
+        assert np.array_equal(_bits(r_v), _bits(r)), "variant policy"
         worst = 0.0
         for k, (grp, gcol) in enumerate(graphs):
             part = r[begin[k]:begin[k + 1]]

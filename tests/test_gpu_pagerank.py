@@ -142,20 +142,33 @@ def test_pagerank_rmat20_config(oracle):
         assert 0 < st.residual_l1 < 1e-2
 
 
-def test_chung_lu_config(oracle):
+def test_chung_lu_config(oracle, direction_policy, variant_policy):
     """The Chung-Lu configuration at full size: 2.4 M vertices, 5 M arcs, Zipf(2.0) out-weights
-    capped at 100,000, 40 % weight 0 (seed 5): direction-optimising BFS bit-exact, 20-step
-    PageRank bit-exact against the FP32 restatement and within L1 1e-6 of FP64."""
+    capped at 100,000, 40 % weight 0 (seed 5): direction-optimising BFS bit-exact with the
+    heuristic, the learned direction and the autotuned launch shape; 20-step PageRank with the
+    default variant and with the learned kernel-variant policy bit-exact against the FP32
+    restatement and within L1 1e-6 of FP64."""
     rp, col = wl.chung_lu_graph()
     src = wl.max_outdeg_vertex(rp)
+    want = oracle.bfs_queue(rp, col, src)
+    want32 = oracle.pagerank_f32(rp, col, 0.85, 20)
     with md.Graph(rp, col) as G:
         lv, st = G.bfs(src)
-        assert np.array_equal(lv, oracle.bfs_queue(rp, col, src))
+        assert np.array_equal(lv, want)
         assert st.push_steps > 0 and st.pull_steps > 0
+        lv, _ = G.bfs(src, policy=direction_policy)
+        assert np.array_equal(lv, want), "learned direction"
+        G.autotune(md.TUNE_BFS, [src], reps=1)
+        for pol in (None, direction_policy):
+            lv, _ = G.bfs(src, policy=pol)
+            assert np.array_equal(lv, want), ("autotuned", pol is not None)
         r, _ = G.pagerank(0.85, 20, use_graph=1)
-        assert np.array_equal(_bits(r), _bits(oracle.pagerank_f32(rp, col, 0.85, 20)))
+        assert np.array_equal(_bits(r), _bits(want32))
         l1 = np.abs(r.astype(np.float64) - oracle.pagerank_f64(rp, col, 0.85, 20)).sum()
         assert l1 <= L1_TOL, l1
+        for use_graph in (0, 1):
+            r, _ = G.pagerank(0.85, 20, use_graph=use_graph, policy=variant_policy)
+            assert np.array_equal(_bits(r), _bits(want32)), ("variant policy", use_graph)
 
 
 def test_pagerank_more_than_2_24_vertices(oracle):

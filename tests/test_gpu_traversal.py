@@ -165,15 +165,22 @@ def test_sssp_lattice_small(oracle):
 
 
 @pytest.mark.slow
-def test_sssp_lattice_config(oracle):
+def test_sssp_lattice_config(oracle, delta_policy):
+    """The 1400x1400 lattice at full size: the default width, a fixed width and the learned width
+    (the policy trained on smaller lattices; device- and host-driven) all give Dijkstra's
+    distances."""
     rp, col, w = wl.lattice_graph()
     src = wl.lattice_source()
     want = oracle.sssp_dijkstra(rp, col, w, src)
+    deg = np.diff(rp.astype(np.int64))
     with md.Graph(rp, col, w) as G:
         for delta in (0, 2000):
             d, st = G.sssp(src, delta=delta)
             assert np.array_equal(d, want), delta
-            deg = np.diff(rp.astype(np.int64))
+            assert st.edges_traversed == deg[want != U32MAX].sum()
+        for timing in (0, 1):
+            d, st = G.sssp(src, policy=delta_policy, timing=timing)
+            assert np.array_equal(d, want), ("learned width", timing)
             assert st.edges_traversed == deg[want != U32MAX].sum()
 
 
@@ -200,15 +207,27 @@ def test_mbfs_matches_plain_bfs(oracle):
 
 
 @pytest.mark.slow
-def test_mbfs_rmat20_config(oracle):
+def test_mbfs_rmat20_config(oracle, mbfs_policy):
+    """RMAT-20 with 64 sources at full size: the heuristic engine, the learned direction (policy
+    trained on smaller RMAT graphs) and the autotuned launch shape (with and without the policy)
+    all give the oracle's per-source levels."""
     rp, col = wl.rmat_graph()
     srcs = wl.mbfs_sources(rp, 64, seed=4)
     want = oracle.mbfs(rp, col, srcs)
+    deg = np.diff(rp.astype(np.int64))
+    reached = sum(deg[want[i] != 255].sum() for i in range(len(srcs)))
     with md.Graph(rp, col) as G:
         out, st = G.mbfs(srcs)
         assert np.array_equal(out, want)
-        deg = np.diff(rp.astype(np.int64))
-        assert st.edges_traversed == sum(deg[want[i] != 255].sum() for i in range(len(srcs)))
+        assert st.edges_traversed == reached
+        out, st = G.mbfs(srcs, policy=mbfs_policy)
+        assert np.array_equal(out, want), "learned direction"
+        t = G.autotune(md.TUNE_MBFS, srcs, reps=1)
+        assert t.large_degree > 0 and t.grid > 0
+        for pol in (None, mbfs_policy):
+            out, st = G.mbfs(srcs, policy=pol)
+            assert np.array_equal(out, want), ("autotuned", pol is not None)
+            assert st.edges_traversed == reached
 
 
 @pytest.mark.parametrize("far_limit", [1, 7, 64])
