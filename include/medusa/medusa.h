@@ -16,7 +16,8 @@
 //   D_Vertex / D_Edge / D_Message / EdgeList with get_<field>/set_<field> accessors
 //       (Algorithm/DeviceDataStructure.h:33-233)
 //   Medusa_Continue / Medusa_Break (MedusaRT/SystemLibGPU.h:14-22)
-//   FunctorHolder<EDGE|VERTEX|EDGELIST|MESSAGE|EDGE_HT>::Run(op) (MultipleGPU/MultiAPIHoster.h:649-754)
+//   FunctorHolder<EDGE|VERTEX|EDGELIST|MESSAGE|EDGE_HT>::Run(op) (MultipleGPU/MultiAPIHoster.h:649-754),
+//   MGFunctorHolder<MG_EDGE|MGMH_EDGE|MG_VERTEX|MGMH_VERTEX>::Run(op) on one GPU (MultiAPIHoster.h:28-640)
 //   InitConfig, InitHostDS, CheckOutgoingEdge, InitDeviceDS, ResetGraphData, InitMessageBuffer,
 //   NULL_Combine, ResetToExecute, RetriveToExecute (MultipleGPU/MultiPublicAPI.h:16-401,
 //   MultiUtilities.h:48-69), MGLOBAL::com.combineAllDevice() (MedusaRT/Combiner.cu:82-142)
@@ -25,6 +26,9 @@
 //   QFunctorHolder<EQ2EQ|EQ2VQ|EQConsume|VQ2EQ|VQ2VQ|VQConsume|VA2EQ|VA2AQ|EA2EQ|EA2AQ>::Run
 //       over MedusaQueue (the reference's planned queue API, MultipleGPU/MultiAPIHoster.h:18-23,
 //       MedusaRT/APIKernelWrappers.h:53-140).
+//
+// Addition: Medusa_Run(Medusa_Exec[, max_supersteps, vote]) runs the BSP loop on the device
+// (captured batches of supersteps, the vote read once per batch).
 //
 // Storage comes from libmedusa_amd.so (medusa_graph_create: CSR + CSC + slot map in HBM);
 // user fields are device arrays allocated here.  Every call runs on the current GPU; the
@@ -70,7 +74,9 @@ enum MedusaDS { DS_MEG, DS_AA, DS_HY, DS_ELL };
 enum MedusaCombinerType { MEDUSA_COMBINE_FLOAT, MEDUSA_COMBINE_INT, MEDUSA_COMBINE_UINT };
 enum MedusaCombinerOp { MEDUSA_COMBINE_ADD, MEDUSA_COMBINE_MIN, MEDUSA_COMBINE_MAX, MEDUSA_COMBINE_MUL };
 // functor kinds (MultipleGPU/MultiAPIHoster.h:18)
-enum EMVAPIType { EDGE, EDGELIST, MESSAGE, MESSAGELIST, VERTEX, EDGE_HT };
+// functor kinds, with the reference's names and order (MultipleGPU/MultiAPIHoster.h:18); the MG_
+// and MGMH_ kinds are the multi-GPU holders, which on one GPU do what EDGE / VERTEX do
+enum EMVAPIType { EDGE, EDGELIST, MESSAGE, MESSAGELIST, VERTEX, MG_VERTEX, EDGE_HT, MG_EDGE, MGMH_EDGE, MGMH_VERTEX };
 
 #define MEDUSA_HIP_OK(expr)                                                                                \
     do {                                                                                                   \
@@ -227,6 +233,12 @@ static __constant__ D_MessageArray d_messageArray;
 static __constant__ MVT* d_messageArrayBuf;  // combined message per vertex
 static __device__ bool d_toExecute;           // BSP vote
 static __device__ MVT d_init_msg_val;
+// Medusa_Run's loop on the device: running, the superstep being run, its last superstep
+// (exclusive) and whether the vote ends it
+static __device__ int d_active;
+static __device__ int d_super_step;
+static __device__ int d_max_step;
+static __device__ int d_vote;
 }  // namespace GRAPH_STORAGE_GPU
 
 // ---- system calls for functors (MedusaRT/SystemLibGPU.h:14-22) ----------------------
@@ -327,11 +339,19 @@ inline MedusaCombinerOp combiner_operator = MEDUSA_COMBINE_ADD;
 inline MessageMode message_mode = NO_MESSAGE;
 inline MedusaDS graph_ds_type = DS_AA;
 inline int num_gpu_to_use = 1;
+inline hipStream_t stream = nullptr;  // every launch of the API; blocking, so it orders with hipMemcpy
+inline bool device_loop = false;      // Medusa_Run is capturing: wrappers read the loop state
+inline const int* d_vertex_order = nullptr;  // vertices by out-degree class, heaviest first (EDGELIST)
+inline unsigned run_generation = 1;   // bumped when captured launches may hold stale pointers
+inline hipGraphExec_t run_exec = nullptr;
+inline unsigned run_exec_generation = 0;
+inline const void* run_exec_fn = nullptr;
 inline int max_hop = 0;
 inline int* total_edge_count = nullptr;
 }  // namespace MGLOBAL
 
 namespace medusa_api {
+inline hipStream_t stream() { return MGLOBAL::stream; }
 inline int grid_size() { return MGLOBAL::gpu_def ? 8 * MGLOBAL::gpu_def[0].device_prop.multiProcessorCount : 2048; }
 inline void check_launch(const char* what) {
     hipError_t e = hipGetLastError();
@@ -354,24 +374,60 @@ inline void upload_symbols() {
 }
 
 // ---- functor wrappers (MedusaRT/APIKernelWrappers.h:8-218), grid-stride ------------------
-template <class O>
+// kLoop: launched inside Medusa_Run's captured loop; the superstep number comes from the device
+// and a superstep after the loop ended returns at once.
+template <bool kLoop>
+__device__ inline bool bsp_live(int& super_step) {
+    if constexpr (kLoop) {
+        if (!GRAPH_STORAGE_GPU::d_active) return false;
+        super_step = GRAPH_STORAGE_GPU::d_super_step;
+    }
+    return true;
+}
+template <bool kLoop, class O>
 __global__ void vertexProcWrapper(int super_step, O op, int n) {
+    if (!bsp_live<kLoop>(super_step)) return;
     for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) op(D_Vertex(v), super_step);
 }
-template <class O>
+template <bool kLoop, class O>
 __global__ void edgeProcWrapper(int super_step, O op, int m) {
+    if (!bsp_live<kLoop>(super_step)) return;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) op(D_Edge(e), super_step);
 }
-template <class O>
-__global__ void edgeListProcWrapper(int super_step, O op, int n) {
-    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+// EDGELIST: one thread per vertex with its whole out-edge list (the functor's contract), in
+// the order of `order`: vertices grouped by out-degree class, heaviest class first, so the
+// lanes of a wave walk lists of similar length and the long lists start first.
+template <bool kLoop, class O>
+__global__ void edgeListProcWrapper(int super_step, O op, int n, const int* __restrict__ order) {
+    if (!bsp_live<kLoop>(super_step)) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int v = order[i];
         EdgeList el(v);
         op(D_Vertex(v), el, super_step);
     }
 }
-template <class O>
+template <bool kLoop, class O>
 __global__ void messageProcWrapper(int super_step, O op, int slots) {
+    if (!bsp_live<kLoop>(super_step)) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += gridDim.x * blockDim.x) op(D_Message(i), super_step);
+}
+// BSP vote and loop control, each one thread on the API's stream.
+__global__ inline void bsp_reset_vote(bool v) { GRAPH_STORAGE_GPU::d_toExecute = v; }
+__global__ inline void bsp_set_init_msg(MVT v) { GRAPH_STORAGE_GPU::d_init_msg_val = v; }
+__global__ inline void bsp_start(int step, int max_step, int vote) {
+    GRAPH_STORAGE_GPU::d_active = step < max_step ? 1 : 0;
+    GRAPH_STORAGE_GPU::d_super_step = step;
+    GRAPH_STORAGE_GPU::d_max_step = max_step;
+    GRAPH_STORAGE_GPU::d_vote = vote;
+}
+__global__ inline void bsp_step_begin() {
+    if (GRAPH_STORAGE_GPU::d_active) GRAPH_STORAGE_GPU::d_toExecute = false;
+}
+__global__ inline void bsp_step_end() {
+    if (!GRAPH_STORAGE_GPU::d_active) return;
+    const int s = ++GRAPH_STORAGE_GPU::d_super_step;
+    if ((GRAPH_STORAGE_GPU::d_vote && !GRAPH_STORAGE_GPU::d_toExecute) || s >= GRAPH_STORAGE_GPU::d_max_step)
+        GRAPH_STORAGE_GPU::d_active = 0;
 }
 template <class T>
 __global__ void fill_kernel(T* p, T v, int n) {
@@ -456,13 +512,24 @@ __global__ void combine_kernel(int n, const int* __restrict__ msg_index, const T
 template <EMVAPIType f>
 struct FunctorHolder {};
 
+// Launches a wrapper on the API's stream: the loop form inside Medusa_Run's capture, else the
+// plain one.
+#define MDS_LAUNCH_FUNCTOR(WRAPPER, OP, ...)                                                             \
+    do {                                                                                                 \
+        if (MGLOBAL::device_loop)                                                                        \
+            hipLaunchKernelGGL((medusa_api::WRAPPER<true, OP>), dim3(medusa_api::grid_size()), dim3(256), 0, \
+                               medusa_api::stream(), MGLOBAL::super_step, op, __VA_ARGS__);             \
+        else                                                                                             \
+            hipLaunchKernelGGL((medusa_api::WRAPPER<false, OP>), dim3(medusa_api::grid_size()), dim3(256), 0, \
+                               medusa_api::stream(), MGLOBAL::super_step, op, __VA_ARGS__);             \
+        medusa_api::check_launch(#WRAPPER);                                                              \
+    } while (0)
+
 template <>
 struct FunctorHolder<EDGE> {
     template <class OP>
     static void Run(OP op) {
-        hipLaunchKernelGGL(medusa_api::edgeProcWrapper<OP>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
-                           MGLOBAL::super_step, op, MGLOBAL::gpu_def[0].edgeArray.size);
-        medusa_api::check_launch("edgeProcWrapper");
+        MDS_LAUNCH_FUNCTOR(edgeProcWrapper, OP, MGLOBAL::gpu_def[0].edgeArray.size);
     }
 };
 template <>
@@ -471,27 +538,64 @@ template <>
 struct FunctorHolder<VERTEX> {
     template <class OP>
     static void Run(OP op) {
-        hipLaunchKernelGGL(medusa_api::vertexProcWrapper<OP>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
-                           MGLOBAL::super_step, op, MGLOBAL::gpu_def[0].vertexArray.size);
-        medusa_api::check_launch("vertexProcWrapper");
+        MDS_LAUNCH_FUNCTOR(vertexProcWrapper, OP, MGLOBAL::gpu_def[0].vertexArray.size);
     }
 };
 template <>
 struct FunctorHolder<EDGELIST> {
     template <class OP>
     static void Run(OP op) {
-        hipLaunchKernelGGL(medusa_api::edgeListProcWrapper<OP>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
-                           MGLOBAL::super_step, op, MGLOBAL::gpu_def[0].vertexArray.size);
-        medusa_api::check_launch("edgeListProcWrapper");
+        MDS_LAUNCH_FUNCTOR(edgeListProcWrapper, OP, MGLOBAL::gpu_def[0].vertexArray.size, MGLOBAL::d_vertex_order);
     }
 };
 template <>
 struct FunctorHolder<MESSAGE> {
     template <class OP>
     static void Run(OP op) {
-        hipLaunchKernelGGL(medusa_api::messageProcWrapper<OP>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
-                           MGLOBAL::super_step, op, MGLOBAL::gpu_def[0].d_messageArray.size);
-        medusa_api::check_launch("messageProcWrapper");
+        MDS_LAUNCH_FUNCTOR(messageProcWrapper, OP, MGLOBAL::gpu_def[0].d_messageArray.size);
+    }
+};
+
+// Multi-GPU holders (MultipleGPU/MultiAPIHoster.h:28-640).  One GPU has no replicas to gather
+// or scatter, so each does what its single-GPU kind does; more GPUs are refused.
+template <EMVAPIType f>
+struct MGFunctorHolder {};
+namespace medusa_api {
+inline void require_one_gpu(const char* what) {
+    if (MGLOBAL::num_gpu_to_use != 1)
+        throw std::runtime_error(std::string(what) + ": this engine runs on one GPU (num_gpu_to_use must be 1)");
+}
+}  // namespace medusa_api
+template <>
+struct MGFunctorHolder<MG_EDGE> {
+    template <class OP>
+    static void Run(OP op) {
+        medusa_api::require_one_gpu("MGFunctorHolder<MG_EDGE>");
+        FunctorHolder<EDGE>::Run(op);
+    }
+};
+template <>
+struct MGFunctorHolder<MGMH_EDGE> {
+    template <class OP>
+    static void Run(OP op) {
+        medusa_api::require_one_gpu("MGFunctorHolder<MGMH_EDGE>");
+        FunctorHolder<EDGE>::Run(op);
+    }
+};
+template <>
+struct MGFunctorHolder<MG_VERTEX> {
+    template <class OP>
+    static void Run(OP op) {
+        medusa_api::require_one_gpu("MGFunctorHolder<MG_VERTEX>");
+        FunctorHolder<VERTEX>::Run(op);
+    }
+};
+template <>
+struct MGFunctorHolder<MGMH_VERTEX> {
+    template <class OP>
+    static void Run(OP op) {
+        medusa_api::require_one_gpu("MGFunctorHolder<MGMH_VERTEX>");
+        FunctorHolder<VERTEX>::Run(op);
     }
 };
 
@@ -628,7 +732,7 @@ __global__ __launch_bounds__(256) void queueExpandWrapper(int step, O op, const 
 }
 
 inline void queue_set_count(unsigned long long* c, unsigned long long v) {
-    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, 0, c, v, 1);
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, stream(), c, v, 1);
     check_launch("queue count");
 }
 
@@ -663,10 +767,10 @@ inline void run_queue_op(O op, const MedusaQueue* in, MedusaQueue* out, const ch
     const int* in_items = kFromQueue ? in->d_items : nullptr;
     const unsigned long long* in_count = kFromQueue ? in->d_count : nullptr;
     if constexpr (kExpand)
-        hipLaunchKernelGGL((queueExpandWrapper<kEdges, kFromQueue, O>), dim3(grid_size()), dim3(256), 0, 0,
+        hipLaunchKernelGGL((queueExpandWrapper<kEdges, kFromQueue, O>), dim3(grid_size()), dim3(256), 0, stream(),
                            MGLOBAL::super_step, op, in_items, in_count, in_n, out_items, out_count, out_cap);
     else
-        hipLaunchKernelGGL((queueFilterWrapper<kEdges, kFromQueue, O>), dim3(grid_size()), dim3(256), 0, 0,
+        hipLaunchKernelGGL((queueFilterWrapper<kEdges, kFromQueue, O>), dim3(grid_size()), dim3(256), 0, stream(),
                            MGLOBAL::super_step, op, in_items, in_count, in_n, out_items, out_count, out_cap);
     check_launch(what);
 }
@@ -759,18 +863,38 @@ inline void Medusa_Combiner::init(MedusaCombinerType dt, MedusaCombinerOp op, in
 inline void Medusa_Combiner::combineAllDevice() {
     GPUDef& G = MGLOBAL::gpu_def[0];
     if (MGLOBAL::message_mode != EDGE_MESSAGE) throw std::runtime_error("combineAllDevice needs EDGE_MESSAGE mode");
-    hipLaunchKernelGGL(medusa_api::combine_kernel<MVT>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
+    hipLaunchKernelGGL(medusa_api::combine_kernel<MVT>, dim3(medusa_api::grid_size()), dim3(256), 0, medusa_api::stream(),
                        G.vertexArray.size, G.d_vertexArray.d_msg_index, (const MVT*)G.d_messageArray.d_val,
                        G.d_messageArrayBuf.d_val, (int)MGLOBAL::combiner_operator);
     medusa_api::check_launch("combine_kernel");
 }
 
 // ---- PublicAPI (MultipleGPU/MultiPublicAPI.h) ------------------------------------------------
+// The graph layout is chosen at compile time as in the reference (define one of MEG, AA, HY,
+// ELL; AA when none is).  AA, MEG and HY all run on the CSR/CSC built here: MEG and HY sort
+// vertices by degree in the reference, and here every EDGELIST functor visits vertices grouped
+// by out-degree class whatever the layout, with vertex ids unchanged, so results are the same.
+// The reference's InitHostDS has no ELL builder (it exits), so ELL is refused here too.
+inline MedusaDS compiled_ds() {
+#if defined(ELL)
+    return DS_ELL;
+#elif defined(HY)
+    return DS_HY;
+#elif defined(MEG)
+    return DS_MEG;
+#else
+    return DS_AA;
+#endif
+}
+
 inline void InitConfig(const char* gpus, const char* hops, MessageMode mm, bool exe) {
     MGLOBAL::num_gpu_to_use = gpus ? atoi(gpus) : 1;
     MGLOBAL::max_hop = hops ? atoi(hops) : 0;
     if (MGLOBAL::num_gpu_to_use != 1)
         throw std::runtime_error("InitConfig: this engine runs on one GPU (multi-GPU partitioning is not built)");
+    MGLOBAL::graph_ds_type = compiled_ds();
+    if (MGLOBAL::graph_ds_type == DS_ELL)
+        throw std::runtime_error("InitConfig: the ELL layout has no builder (as in the reference); use AA, MEG or HY");
     MGLOBAL::message_mode = mm;
     MGLOBAL::toExecute = exe;
     MGLOBAL::super_step = 0;
@@ -779,6 +903,7 @@ inline void InitConfig(const char* gpus, const char* hops, MessageMode mm, bool 
     int dev = 0;
     MEDUSA_HIP_OK(hipGetDevice(&dev));
     MEDUSA_HIP_OK(hipGetDeviceProperties(&MGLOBAL::gpu_def[0].device_prop, dev));
+    if (!MGLOBAL::stream) MEDUSA_HIP_OK(hipStreamCreate(&MGLOBAL::stream));
 }
 
 // Builds host SoA arrays and the HBM graph (CSR in the reference's edge order, CSC, slot map).
@@ -871,10 +996,30 @@ inline void D_EdgeArray_Fill(D_EdgeArray& D, const EdgeArray& E) {
 
 // Uploads vertex/edge fields, allocates the message buffers and binds the device symbols
 // (MultiPublicAPI.h:144-308).
+// Vertices grouped by out-degree class (floor(log2(degree)) + 1, 0 for none), heaviest class
+// first, ids ascending inside a class: the visiting order of EDGELIST functors.
+inline std::vector<int> degree_order(const VertexArray& V) {
+    constexpr int kClasses = 33;
+    std::vector<int> cls(V.size), count(kClasses + 1, 0);
+    for (int v = 0; v < V.size; ++v) {
+        const unsigned d = (unsigned)V.edge_count[v];
+        cls[v] = d ? 32 - __builtin_clz(d) : 0;
+        ++count[kClasses - 1 - cls[v] + 1];
+    }
+    for (int c = 1; c <= kClasses; ++c) count[c] += count[c - 1];
+    std::vector<int> order(V.size);
+    for (int v = 0; v < V.size; ++v) order[count[kClasses - 1 - cls[v]]++] = v;
+    return order;
+}
+
 inline void InitDeviceDS() {
     GPUDef& G = MGLOBAL::gpu_def[0];
     D_VertexArray_Fill(G.d_vertexArray, G.vertexArray);
     D_EdgeArray_Fill(G.d_edgeArray, G.edgeArray);
+    if (MGLOBAL::d_vertex_order) MEDUSA_HIP_OK(hipFree((void*)MGLOBAL::d_vertex_order));
+    const std::vector<int> order = degree_order(G.vertexArray);
+    MGLOBAL::d_vertex_order = medusa_api::dev_alloc_copy(order.data(), order.size());
+    ++MGLOBAL::run_generation;
     int slots = 0, buf = 0;
     if (MGLOBAL::message_mode == EDGE_MESSAGE) {
         slots = G.edgeArray.size;
@@ -898,6 +1043,7 @@ inline void InitDeviceDS() {
 // Re-uploads host vertex/edge fields for another run (MultiPublicAPI.h:310-326).
 inline void ResetGraphData() {
     GPUDef& G = MGLOBAL::gpu_def[0];
+    ++MGLOBAL::run_generation;
     D_VertexArray_Fill(G.d_vertexArray, G.vertexArray);
     D_EdgeArray_Fill(G.d_edgeArray, G.edgeArray);
     medusa_api::upload_symbols();
@@ -919,11 +1065,13 @@ inline void DumpVertexData() {
 inline void InitMessageBuffer(Message init_val) {
     GPUDef& G = MGLOBAL::gpu_def[0];
     if (G.d_messageArray.size) {
-        hipLaunchKernelGGL(medusa_api::fill_kernel<MVT>, dim3(medusa_api::grid_size()), dim3(256), 0, 0,
+        hipLaunchKernelGGL(medusa_api::fill_kernel<MVT>, dim3(medusa_api::grid_size()), dim3(256), 0, medusa_api::stream(),
                            G.d_messageArray.d_val, init_val.val, G.d_messageArray.size);
         medusa_api::check_launch("InitMessageBuffer");
     }
-    MEDUSA_HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(GRAPH_STORAGE_GPU::d_init_msg_val), &init_val.val, sizeof(MVT)));
+    // a kernel argument rather than a host copy, so Medusa_Run can capture it
+    hipLaunchKernelGGL(medusa_api::bsp_set_init_msg, dim3(1), dim3(1), 0, medusa_api::stream(), init_val.val);
+    medusa_api::check_launch("InitMessageBuffer");
 }
 
 // Swaps the message array and its buffer, so messages become readable without combining
@@ -934,16 +1082,82 @@ inline void NULL_Combine() {
         throw std::runtime_error("NULL_Combine: message array and buffer differ in size (use VERTEX_MESSAGE)");
     std::swap(G.d_messageArray, G.d_messageArrayBuf);
     medusa_api::upload_symbols();
+    ++MGLOBAL::run_generation;
 }
 
-// BSP vote (MultipleGPU/MultiUtilities.h:48-69).
+// BSP vote (MultipleGPU/MultiUtilities.h:48-69): the reset is a one-thread kernel on the API's
+// stream and the read one copy, so a host-driven superstep waits for the GPU once.
 inline void ResetToExecute() {
-    MEDUSA_HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(GRAPH_STORAGE_GPU::d_toExecute), &MGLOBAL::toExecute, sizeof(bool)));
+    hipLaunchKernelGGL(medusa_api::bsp_reset_vote, dim3(1), dim3(1), 0, medusa_api::stream(), MGLOBAL::toExecute);
+    medusa_api::check_launch("ResetToExecute");
 }
 inline bool RetriveToExecute() {
     bool v = false;
-    MEDUSA_HIP_OK(hipMemcpyFromSymbol(&v, HIP_SYMBOL(GRAPH_STORAGE_GPU::d_toExecute), sizeof(bool)));
+    MEDUSA_HIP_OK(hipMemcpyFromSymbolAsync(&v, HIP_SYMBOL(GRAPH_STORAGE_GPU::d_toExecute), sizeof(bool), 0,
+                                           hipMemcpyDeviceToHost, medusa_api::stream()));
+    MEDUSA_HIP_OK(hipStreamSynchronize(medusa_api::stream()));
     return v;
+}
+
+// The BSP loop on the device (no reference counterpart; the reference's host loop is
+// ResetToExecute / Medusa_Exec / RetriveToExecute per superstep, which still works).  Runs
+// supersteps of `exec` (the application's Medusa_Exec) until the vote ends the run (with
+// `vote`; functors call Medusa_Continue to go on) or `max_supersteps` have run.  Batches of
+// kRunBatch supersteps are captured once as a hipGraph, each superstep bracketed by one-thread
+// kernels that reset the vote and advance the superstep on the device; supersteps after the
+// end return at once, and the host reads the loop state once per batch.  `exec` must enqueue
+// the same work every superstep through this API (FunctorHolder, MGFunctorHolder,
+// InitMessageBuffer, combineAllDevice); work that synchronises with the host cannot be
+// captured and makes Medusa_Run throw (use the host loop for it).  Returns the supersteps run;
+// MGLOBAL::super_step is advanced by as many.
+constexpr int kRunBatch = 8;
+inline int Medusa_Run(void (*exec)(), int max_supersteps = INT_MAX, bool vote = true) {
+    hipStream_t s = medusa_api::stream();
+    const int start = MGLOBAL::super_step;
+    const long long last = (long long)start + (max_supersteps > 0 ? max_supersteps : 0);
+    const int max_step = last > INT_MAX ? INT_MAX : (int)last;
+    if (!MGLOBAL::run_exec || MGLOBAL::run_exec_generation != MGLOBAL::run_generation ||
+        MGLOBAL::run_exec_fn != (const void*)exec) {
+        if (MGLOBAL::run_exec) (void)hipGraphExecDestroy(MGLOBAL::run_exec);
+        MGLOBAL::run_exec = nullptr;
+        MEDUSA_HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        std::string err;
+        MGLOBAL::device_loop = true;
+        try {
+            for (int k = 0; k < kRunBatch; ++k) {
+                hipLaunchKernelGGL(medusa_api::bsp_step_begin, dim3(1), dim3(1), 0, s);
+                exec();
+                hipLaunchKernelGGL(medusa_api::bsp_step_end, dim3(1), dim3(1), 0, s);
+            }
+        } catch (const std::exception& e) {
+            err = e.what();
+        }
+        MGLOBAL::device_loop = false;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess && err.empty()) e = hipGraphInstantiate(&MGLOBAL::run_exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        if (!err.empty() || e != hipSuccess)
+            throw std::runtime_error("Medusa_Run: the superstep could not be captured (" +
+                                     (err.empty() ? std::string(hipGetErrorString(e)) : err) +
+                                     "); Medusa_Exec must only enqueue work through the API");
+        MGLOBAL::run_exec_generation = MGLOBAL::run_generation;
+        MGLOBAL::run_exec_fn = (const void*)exec;
+    }
+    hipLaunchKernelGGL(medusa_api::bsp_start, dim3(1), dim3(1), 0, s, start, max_step, vote ? 1 : 0);
+    medusa_api::check_launch("Medusa_Run");
+    int state[2] = {1, start};  // active, superstep
+    while (state[0]) {
+        MEDUSA_HIP_OK(hipGraphLaunch(MGLOBAL::run_exec, s));
+        MEDUSA_HIP_OK(hipMemcpyFromSymbolAsync(&state[0], HIP_SYMBOL(GRAPH_STORAGE_GPU::d_active), sizeof(int), 0,
+                                               hipMemcpyDeviceToHost, s));
+        MEDUSA_HIP_OK(hipMemcpyFromSymbolAsync(&state[1], HIP_SYMBOL(GRAPH_STORAGE_GPU::d_super_step), sizeof(int), 0,
+                                               hipMemcpyDeviceToHost, s));
+        MEDUSA_HIP_OK(hipStreamSynchronize(s));
+    }
+    MGLOBAL::super_step = state[1];
+    return state[1] - start;
 }
 
 // Frees the device state of the run.
@@ -963,6 +1177,11 @@ inline void ReleaseDeviceDS() {
     if (G.d_messageArrayBuf.d_val) (void)hipFree(G.d_messageArrayBuf.d_val);
     if (G.graph) medusa_graph_destroy(G.graph);
     G.graph = nullptr;
+    if (MGLOBAL::d_vertex_order) (void)hipFree((void*)MGLOBAL::d_vertex_order);
+    MGLOBAL::d_vertex_order = nullptr;
+    if (MGLOBAL::run_exec) (void)hipGraphExecDestroy(MGLOBAL::run_exec);
+    MGLOBAL::run_exec = nullptr;
+    ++MGLOBAL::run_generation;
     G.d_vertexArray = D_VertexArray{};
     G.d_edgeArray = D_EdgeArray{};
     G.d_messageArray = D_MessageArray{};

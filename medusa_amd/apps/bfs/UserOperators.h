@@ -14,4 +14,14 @@ struct ExpandLevel {
     }
 };
 
-inline void Medusa_Exec() { FunctorHolder<EDGE>::Run(ExpandLevel()); }
+// The reference's dispatch (Algorithm/UserOperators.h:44-53): multi-hop and multi-GPU holders
+// when more than one GPU is used, the single-GPU holder otherwise.
+inline void Medusa_Exec() {
+    ExpandLevel ul;
+    if (MGLOBAL::num_gpu_to_use > 1 && MGLOBAL::max_hop >= 1)
+        MGFunctorHolder<MGMH_EDGE>::Run(ul);
+    else if (MGLOBAL::num_gpu_to_use > 1)
+        MGFunctorHolder<MG_EDGE>::Run(ul);
+    else
+        FunctorHolder<EDGE>::Run(ul);
+}

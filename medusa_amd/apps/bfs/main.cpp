@@ -22,16 +22,10 @@ int main(int argc, char** argv) {
         for (int v = 0; v < V.size; ++v) V.level[v] = MVT_Init_Value;
         V.level[root] = 0;
         InitDeviceDS();
-        bool more = true;
-        while (more) {
-            ResetToExecute();
-            Medusa_Exec();
-            more = RetriveToExecute();
-            ++MGLOBAL::super_step;
-        }
+        const double ms = RunLoop(Medusa_Exec, INT_MAX, true);
         DumpVertexData();
         WriteArray(argv[2], V.level, V.size);
-        printf("bfs root=%d supersteps=%d\n", root, MGLOBAL::super_step);
+        printf("bfs root=%d supersteps=%d loop_ms=%.4f\n", root, MGLOBAL::super_step, ms);
         ReleaseDeviceDS();
     } catch (const std::exception& e) {
         fprintf(stderr, "bfs_app: %s\n", e.what());

@@ -6,7 +6,9 @@
 // Binary arcs are added in CSR order to the GraphIR.
 #pragma once
 
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -60,5 +62,40 @@ inline int MaxOutDegreeVertex(const VertexArray& V) {
     int best = 0;
     for (int v = 1; v < V.size; ++v)
         if (V.edge_count[v] > V.edge_count[best]) best = v;
+    return best;
+}
+
+// The application's BSP loop over `exec` (its Medusa_Exec), from superstep 0 and the initial
+// data InitDeviceDS uploaded: on the device with Medusa_Run, or, with MEDUSA_HOST_LOOP=1 in
+// the environment, the reference's host loop (ResetToExecute / exec / RetriveToExecute per
+// superstep).  `vote`: the functors' vote ends the run; otherwise exactly `max_supersteps`
+// run.  With MEDUSA_APP_REPEAT=r the run is repeated r times, the data re-uploaded before
+// each; returns the fastest run's milliseconds (the GPU idle before and after each).
+inline double RunLoop(void (*exec)(), int max_supersteps, bool vote) {
+    const char* hl = std::getenv("MEDUSA_HOST_LOOP");
+    const char* rp = std::getenv("MEDUSA_APP_REPEAT");
+    const bool host = hl && std::atoi(hl) == 1;
+    const int reps = rp && std::atoi(rp) > 1 ? std::atoi(rp) : 1;
+    double best = 1e300;
+    for (int r = 0; r < reps; ++r) {
+        if (r) ResetGraphData();
+        MGLOBAL::super_step = 0;
+        MEDUSA_HIP_OK(hipDeviceSynchronize());
+        const auto t0 = std::chrono::steady_clock::now();
+        if (host) {
+            bool more = true;
+            while (more && MGLOBAL::super_step < max_supersteps) {
+                if (vote) ResetToExecute();
+                exec();
+                more = vote ? RetriveToExecute() : true;
+                ++MGLOBAL::super_step;
+            }
+        } else {
+            Medusa_Run(exec, max_supersteps, vote);
+        }
+        MEDUSA_HIP_OK(hipDeviceSynchronize());
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms < best) best = ms;
+    }
     return best;
 }

@@ -31,16 +31,10 @@ int main(int argc, char** argv) {
         MEDUSA_HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(g_levels), &d_levels, sizeof(d_levels)));
         MEDUSA_HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(g_n), &n, sizeof(int)));
         InitDeviceDS();
-        bool more = true;
-        while (more) {
-            ResetToExecute();
-            Medusa_Exec();
-            more = RetriveToExecute();
-            ++MGLOBAL::super_step;
-        }
+        const double ms = RunLoop(Medusa_Exec, INT_MAX, true);
         MEDUSA_HIP_OK(hipMemcpy(levels.data(), d_levels, levels.size(), hipMemcpyDeviceToHost));
         WriteArray(argv[2], levels.data(), levels.size());
-        printf("mbfs sources=%d supersteps=%d\n", k, MGLOBAL::super_step);
+        printf("mbfs sources=%d supersteps=%d loop_ms=%.4f\n", k, MGLOBAL::super_step, ms);
         (void)hipFree(d_levels);
         ReleaseDeviceDS();
     } catch (const std::exception& e) {

@@ -21,13 +21,10 @@ int main(int argc, char** argv) {
         VertexArray& V = MGLOBAL::gpu_def[0].vertexArray;
         for (int v = 0; v < V.size; ++v) V.level[v] = 1.0f;
         InitDeviceDS();
-        while (MGLOBAL::super_step < steps) {
-            Medusa_Exec();
-            ++MGLOBAL::super_step;
-        }
+        const double ms = RunLoop(Medusa_Exec, steps, false);
         DumpVertexData();
         WriteArray(argv[2], V.level, V.size);
-        printf("pagerank supersteps=%d\n", MGLOBAL::super_step);
+        printf("pagerank supersteps=%d loop_ms=%.4f\n", MGLOBAL::super_step, ms);
         ReleaseDeviceDS();
     } catch (const std::exception& e) {
         fprintf(stderr, "pagerank_app: %s\n", e.what());

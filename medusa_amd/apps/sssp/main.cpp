@@ -27,16 +27,10 @@ int main(int argc, char** argv) {
         V.distance[root] = 0;
         V.updated[root] = true;
         InitDeviceDS();
-        bool more = true;
-        while (more) {
-            ResetToExecute();
-            Medusa_Exec();
-            more = RetriveToExecute();
-            ++MGLOBAL::super_step;
-        }
+        const double ms = RunLoop(Medusa_Exec, INT_MAX, true);
         DumpVertexData();
         WriteArray(argv[2], V.distance, V.size);
-        printf("sssp root=%d supersteps=%d\n", root, MGLOBAL::super_step);
+        printf("sssp root=%d supersteps=%d loop_ms=%.4f\n", root, MGLOBAL::super_step, ms);
         ReleaseDeviceDS();
     } catch (const std::exception& e) {
         fprintf(stderr, "sssp_app: %s\n", e.what());

@@ -55,10 +55,16 @@ APPS = HERE / "apps"
 APP_BIN = APPS / "bin"
 
 
+# Extra builds of an application with the reference's compile-time layout choice
+# (Algorithm/Configuration.h: one of MEG, AA, HY, ELL is defined): name -> (source app, defines).
+LAYOUT_APPS = {"bfs_meg": ("bfs", ["-DMEG"]), "bfs_hy": ("bfs", ["-DHY"]), "bfs_ell": ("bfs", ["-DELL"])}
+
+
 def _build_app(name: str, verbose: bool) -> pathlib.Path:
     """Compiles apps/<name>/main.cpp (Medusa's user API, header-only) against the library."""
+    src, defines = LAYOUT_APPS.get(name, (name, []))
     out = APP_BIN / f"{name}_app"
-    cmd = [hipcc(), "-x", "hip", *HIP_FLAGS, *COMMON, str(APPS / name / "main.cpp"), "-o", str(out),
+    cmd = [hipcc(), "-x", "hip", *HIP_FLAGS, *COMMON, *defines, str(APPS / src / "main.cpp"), "-o", str(out),
            f"-L{HERE}", "-lmedusa_amd", "-Wl,-rpath,$ORIGIN/../.."]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -70,7 +76,7 @@ def _build_app(name: str, verbose: bool) -> pathlib.Path:
 
 def build_apps(verbose: bool = False):
     APP_BIN.mkdir(exist_ok=True)
-    names = sorted(d.name for d in APPS.iterdir() if (d / "main.cpp").exists())
+    names = sorted(d.name for d in APPS.iterdir() if (d / "main.cpp").exists()) + sorted(LAYOUT_APPS)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(names))) as ex:
         return list(ex.map(lambda n: _build_app(n, verbose), names))
 

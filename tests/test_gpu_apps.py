@@ -40,10 +40,14 @@ def write_gtgraph(path, rp, col, w):
                 f.write(f"a {u + 1} {col[e] + 1} {w[e]}\n")
 
 
-def run(app, *args):
-    r = subprocess.run([str(BIN / app), *map(str, args)], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    return r.stdout
+def run(app, *args, env=None, ok=True):
+    import os
+    e = dict(os.environ, **(env or {}))
+    r = subprocess.run([str(BIN / app), *map(str, args)], capture_output=True, text=True, timeout=120, env=e)
+    if ok:
+        assert r.returncode == 0, r.stderr
+        return r.stdout
+    return r
 
 
 def test_bfs_app(tmp_path, oracle):
@@ -213,3 +217,50 @@ def test_queue_operations_app(tmp_path, oracle, seed, hub):
     odd = want_ea2eq[wc[want_ea2eq] % 2 == 1]
     assert np.array_equal(srt(eq2vq), srt(col[odd].astype(np.int64)))
     assert np.array_equal(hits, np.bincount(esrc[want_ea2eq], minlength=n).astype(np.int32))
+
+
+def test_bfs_app_loops_and_layouts(tmp_path, oracle):
+    """The BFS app's Medusa_Exec has the reference's branch structure (MGMH_EDGE / MG_EDGE /
+    EDGE by GPU count and hops).  Its levels equal the oracle's with the device loop
+    (Medusa_Run), with the reference's host loop (ResetToExecute / RetriveToExecute per
+    superstep), over repeated runs, and built for the MEG and HY layouts; an ELL build is refused
+    at InitConfig, as the reference refuses it."""
+    rp, col, _ = random_graph(20000, 200000, 21)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col)
+    root = 5
+    want = oracle.bfs_queue(rp, col, root)
+    steps = None
+    for app, env in (("bfs_app", {}), ("bfs_app", {"MEDUSA_HOST_LOOP": "1"}),
+                     ("bfs_app", {"MEDUSA_APP_REPEAT": "3"}), ("bfs_app", {"MEDUSA_HOST_LOOP": "1", "MEDUSA_APP_REPEAT": "2"}),
+                     ("bfs_meg_app", {}), ("bfs_hy_app", {})):
+        out = tmp_path / f"{app}_{len(env)}.bin"
+        text = run(app, g, out, root, env=env)
+        assert np.array_equal(np.fromfile(out, np.int32), want), (app, env)
+        s = int(text.split("supersteps=")[1].split()[0])
+        assert steps in (None, s), (app, env, text)  # every loop runs the same supersteps
+        steps = s
+        assert "loop_ms=" in text
+    r = run("bfs_ell_app", g, tmp_path / "ell.bin", root, ok=False)
+    assert r.returncode != 0 and "ELL" in r.stderr
+
+
+@pytest.mark.parametrize("env", [{}, {"MEDUSA_HOST_LOOP": "1"}, {"MEDUSA_APP_REPEAT": "2"}])
+def test_sssp_and_pagerank_apps_every_loop(tmp_path, oracle, env):
+    """SSSP (vote-ended) and PageRank (fixed supersteps) through the device loop, the host
+    loop and repeated runs give the same results as the oracle."""
+    rp, col, w = random_graph(6000, 48000, 23)
+    w = np.random.default_rng(23).integers(1, 100, len(col)).astype(np.uint32)
+    g = tmp_path / "g.csr"
+    write_csr(g, rp, col, w)
+    out = tmp_path / "d.bin"
+    run("sssp_app", g, out, 0, env=env)
+    want = oracle.sssp_dijkstra(rp, col, w, 0)
+    wantf = np.where(want == 0xFFFFFFFF, np.float32(np.finfo(np.float32).max), want.astype(np.float32))
+    assert np.array_equal(np.fromfile(out, np.float32), wantf)
+    g2 = tmp_path / "g2.csr"
+    write_csr(g2, rp, col)
+    out = tmp_path / "r.bin"
+    text = run("pagerank_app", g2, out, 20, env=env)
+    assert "supersteps=20" in text
+    np.testing.assert_allclose(np.fromfile(out, np.float32), oracle.pagerank_ref(rp, col, 20), rtol=2e-5)
