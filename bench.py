@@ -36,8 +36,9 @@ def parse():
     p.add_argument("--scale", type=int, default=20)
     p.add_argument("--workload", default="pagerank_rmat20",
                    choices=["pagerank_rmat20", "bfs_er", "bfs_rmat20", "sssp_lattice", "mbfs_rmat20",
-                            "chung_lu", "chung_lu_batch", "sssp_rmat20"],
-                   help="pagerank_rmat20 is the headline line; the others measure the remaining configurations")
+                            "chung_lu", "chung_lu_batch", "sssp_rmat20", "apps_bfs_er", "apps_pagerank_rmat20"],
+                   help="pagerank_rmat20 is the headline line; the others measure the remaining configurations "
+                        "(apps_*: the example applications written against the user API)")
     p.add_argument("--variant", default="default", choices=["default", "pull", "push", "binned"],
                    help="PageRank superstep kernel variant (all give bit-identical ranks); default = the "
                         "engine's choice (binned)")
@@ -471,9 +472,117 @@ def mbfs_block(md, wl, torch, np, rp, col, srcs, steps, warmup):
     return d
 
 
+APP_BIN = ROOT / "medusa_amd" / "apps" / "bin"
+
+
+def _write_csr(path, rp, col, np):
+    with open(path, "wb") as f:
+        f.write(b"MDSCSR01")
+        f.write(np.uint32(len(rp) - 1).tobytes())
+        f.write(np.uint64(len(col)).tobytes())
+        f.write(np.uint32(0).tobytes())
+        f.write(np.asarray(rp, np.uint32).tobytes())
+        f.write(np.asarray(col, np.uint32).tobytes())
+
+
+def _run_app(app, args, reps, host_loop=False):
+    """Runs an example application (a separate process) with `reps` repeated runs; returns
+    (stdout key=value fields, median loop ms of the runs after the first)."""
+    import subprocess
+
+    env = dict(os.environ, MEDUSA_APP_REPEAT=str(reps))
+    if host_loop:
+        env["MEDUSA_HOST_LOOP"] = "1"
+    r = subprocess.run([str(APP_BIN / app), *map(str, args)], capture_output=True, text=True, timeout=600, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"{app} failed: {r.stderr}")
+    fields = dict(kv.split("=", 1) for kv in r.stdout.split() if "=" in kv)
+    return fields, float(fields["loop_ms"])
+
+
+def run_apps_workload(a, md, wl, torch, np):
+    """The example applications built on the user API (include/medusa/medusa.h), timed in their
+    own process over the device-driven loop (Medusa_Run), with the reference's host loop
+    (ResetToExecute / RetriveToExecute per superstep) beside it; results checked against the
+    oracle."""
+    import tempfile
+
+    import oracle
+
+    threads = _cpu_threads()
+    reps = a.steps + 1
+    with tempfile.TemporaryDirectory() as td:
+        g, out = pathlib.Path(td) / "g.csr", pathlib.Path(td) / "out.bin"
+        if a.workload == "apps_bfs_er":
+            rp, col = wl.er_graph()
+            n, m = len(rp) - 1, len(col)
+            _write_csr(g, rp, col, np)
+            f, ms = _run_app("bfs_app", [g, out, 0], reps)
+            levels = np.fromfile(out, np.int32)
+            _, ms_host = _run_app("bfs_app", [g, out, 0], reps, host_loop=True)
+            want = oracle.bfs_queue(rp, col, 0)
+            steps = int(f["supersteps"])
+            deg = np.diff(rp.astype(np.int64))
+            m_r = int(deg[want >= 0].sum())
+            n_r = int((want >= 0).sum())
+            # per superstep every edge reads its source id and the source's level (8m); over the
+            # run, the reached out-edges read destination id and level (8) and claims write 4
+            alg = steps * 8 * m + 8 * m_r + 4 * n_r
+            value = m_r / (ms / 1e3) / 1e9
+            cfg = {"workload": "apps_bfs_er", "app": "bfs_app (FunctorHolder<EDGE>, reference dispatch branches)",
+                   "vertices": n, "arcs": m, "draws": 1048576, "source": 0,
+                   "graph": "Erdos-Renyi G(n,m), self-loops dropped, duplicates kept, seed 1", "dtype": "int32"}
+            extra = {"supersteps": steps, "edges_reached": m_r,
+                     "levels_equal_oracle": bool(np.array_equal(levels, want))}
+            kernel = "edgeProcWrapper<ExpandLevel> + one-thread vote kernels (whole BFS loop)"
+            cpu_fn = lambda: oracle.bfs_sweep(rp, col, 0, threads)  # noqa: E731
+            cpu_units = m_r
+            cpu_sample = "reference CPU path restated (all-edge sweep per superstep, OpenMP)"
+        else:
+            rp, col = wl.rmat_graph()
+            n, m = len(rp) - 1, len(col)
+            _write_csr(g, rp, col, np)
+            steps = STEPS_PR
+            f, ms = _run_app("pagerank_app", [g, out, steps], reps)
+            ranks = np.fromfile(out, np.float32)
+            _, ms_host = _run_app("pagerank_app", [g, out, steps], reps, host_loop=True)
+            want = oracle.pagerank_ref(rp, col, steps, threads)
+            # per superstep: message fill (4m), edge functor: source id, rank and out-degree,
+            # slot and message (20m), combiner: offsets, messages, sums (4m + 8n), vertex
+            # functor: combined message and rank (8n)
+            alg = steps * (28 * m + 20 * n)
+            value = steps * m / (ms / 1e3) / 1e9
+            cfg = {"workload": "apps_pagerank_rmat20",
+                   "app": "pagerank_app (EDGE functor + ADD combiner + VERTEX functor, reference formula)",
+                   "vertices": n, "arcs": m, "supersteps": steps, "graph": "RMAT scale 20, edge factor 16, seed 2",
+                   "dtype": "fp32"}
+            rel = float(np.max(np.abs(ranks.astype(np.float64) - want) / np.maximum(np.abs(want), 1e-30)))
+            extra = {"max_rel_err_vs_oracle": rel, "ranks_match_oracle_rtol_2e-5": bool(rel <= 2e-5)}
+            kernel = "fill + edgeProcWrapper<ShareRank> + combine_kernel + vertexProcWrapper<GatherRank> (whole loop)"
+            cpu_fn = lambda: oracle.pagerank_ref(rp, col, steps, threads)  # noqa: E731
+            cpu_units = steps * m
+            cpu_sample = "reference CPU path restated (message-array push + per-vertex segment sum, OpenMP)"
+    extra.update({"host_loop_ms": round(ms_host, 4), "device_loop_over_host_loop": round(ms_host / ms, 3),
+                  "loop": "Medusa_Run: supersteps captured as a hipGraph, the vote read once per 8 supersteps"})
+    roof = {"bound": "hbm", "achieved": round(alg / (ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "kernel_avg_us": round(ms / steps * 1e3, 2), "alg_bytes_per_launch": int(alg / steps),
+            "launch": "one superstep of the application's loop (its kernels and the gaps between them)"}
+    cpu = None
+    if not a.no_cpu:
+        creps, tc = _cpu_repeat(cpu_fn)
+        cpu = {"value": round(creps * cpu_units / tc / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+               "sample": f"{cpu_sample}: {creps} full runs ({tc:.1f} s)"}
+    _emit(a, cfg, value, ms, roof, cpu, extra)
+
+
 def run_workload(a, md, wl, torch, np):
     """The non-headline configurations: one JSON line each, same schema as the headline."""
     import oracle
+
+    if a.workload.startswith("apps_"):
+        run_apps_workload(a, md, wl, torch, np)
+        return
 
     threads = _cpu_threads()
     w = a.workload

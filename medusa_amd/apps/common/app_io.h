@@ -6,6 +6,7 @@
 // Binary arcs are added in CSR order to the GraphIR.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -70,13 +71,14 @@ inline int MaxOutDegreeVertex(const VertexArray& V) {
 // the environment, the reference's host loop (ResetToExecute / exec / RetriveToExecute per
 // superstep).  `vote`: the functors' vote ends the run; otherwise exactly `max_supersteps`
 // run.  With MEDUSA_APP_REPEAT=r the run is repeated r times, the data re-uploaded before
-// each; returns the fastest run's milliseconds (the GPU idle before and after each).
+// each; returns the median milliseconds of the runs after the first (which also captures the
+// device loop), or of the one run (the GPU idle before and after each).
 inline double RunLoop(void (*exec)(), int max_supersteps, bool vote) {
     const char* hl = std::getenv("MEDUSA_HOST_LOOP");
     const char* rp = std::getenv("MEDUSA_APP_REPEAT");
     const bool host = hl && std::atoi(hl) == 1;
     const int reps = rp && std::atoi(rp) > 1 ? std::atoi(rp) : 1;
-    double best = 1e300;
+    std::vector<double> times;
     for (int r = 0; r < reps; ++r) {
         if (r) ResetGraphData();
         MGLOBAL::super_step = 0;
@@ -94,8 +96,9 @@ inline double RunLoop(void (*exec)(), int max_supersteps, bool vote) {
             Medusa_Run(exec, max_supersteps, vote);
         }
         MEDUSA_HIP_OK(hipDeviceSynchronize());
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (ms < best) best = ms;
+        times.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
-    return best;
+    if (times.size() > 1) times.erase(times.begin());
+    std::sort(times.begin(), times.end());
+    return times[times.size() / 2];
 }
