@@ -1108,14 +1108,20 @@ inline bool RetriveToExecute() {
 // end return at once, and the host reads the loop state once per batch.  `exec` must enqueue
 // the same work every superstep through this API (FunctorHolder, MGFunctorHolder,
 // InitMessageBuffer, combineAllDevice); work that synchronises with the host cannot be
-// captured and makes Medusa_Run throw (use the host loop for it).  Returns the supersteps run;
-// MGLOBAL::super_step is advanced by as many.
+// captured and makes Medusa_Run throw (use the host loop for it).  Without `vote` there is
+// nothing to wait for: the supersteps are enqueued back to back and waited for once.  Returns
+// the supersteps run; MGLOBAL::super_step is advanced by as many.
 constexpr int kRunBatch = 8;
 inline int Medusa_Run(void (*exec)(), int max_supersteps = INT_MAX, bool vote = true) {
     hipStream_t s = medusa_api::stream();
     const int start = MGLOBAL::super_step;
     const long long last = (long long)start + (max_supersteps > 0 ? max_supersteps : 0);
     const int max_step = last > INT_MAX ? INT_MAX : (int)last;
+    if (!vote) {  // a fixed number of supersteps: nothing to wait for, so they go back to back
+        for (; MGLOBAL::super_step < max_step; ++MGLOBAL::super_step) exec();
+        MEDUSA_HIP_OK(hipStreamSynchronize(s));
+        return MGLOBAL::super_step - start;
+    }
     if (!MGLOBAL::run_exec || MGLOBAL::run_exec_generation != MGLOBAL::run_generation ||
         MGLOBAL::run_exec_fn != (const void*)exec) {
         if (MGLOBAL::run_exec) (void)hipGraphExecDestroy(MGLOBAL::run_exec);
