@@ -145,15 +145,15 @@ struct PrPart {
     uint32_t first_block, nblocks, n, pad;
 };
 struct PrBins {
-    uint32_t nbins = 0, nhubs = 0, ntiles = 0, max_runs = 0;
+    uint32_t nbins = 0, nhubs = 0, ntiles = 0;
     uint64_t nmsg = 0;     // messages per superstep (in-edges of non-hub destinations)
     uint64_t nentry = 0;   // message-array entries (runs padded for aligned access)
     DBuf<PrItem> items;    // combine items, bin by bin; then one kPrWave item per hub
     DBuf<PrBin> bins;
-    DBuf<PrMsgRun> runs;   // per bin: its runs and a sentinel
     DBuf<PrTile> tiles;
     DBuf<uint16_t> srcloc; // message order: source offset of every entry in its chunk
     DBuf<uint16_t> slot;   // bin order: LDS slot of every entry
+    DBuf<uint32_t> qmsg;   // bin order: message-array position of every group of 4 entries, / 4
     DBuf<float> msg;       // the message array
 };
 

@@ -277,19 +277,19 @@ bool build_pr_bins(medusa_graph* g, bool required) {
     B->nbins = (uint32_t)P.bins.size();
     B->nhubs = P.nhubs;
     B->ntiles = (uint32_t)P.tiles.size();
-    B->max_runs = P.max_runs;
     B->nmsg = P.nmsg;
     B->nentry = P.nentry;
     B->items.upload(P.items.data(), P.items.size(), s);
     B->bins.upload(P.bins.data(), P.bins.size(), s);
-    B->runs.upload(P.runs.data(), P.runs.size(), s);
     B->tiles.upload(P.tiles.data(), P.tiles.size(), s);
     if (P.nentry) {
         B->srcloc.upload(P.srcloc.data(), P.nentry, s);
         B->slot.upload(P.slot.data(), P.nentry, s);
+        B->qmsg.upload(P.qmsg.data(), P.qmsg.size(), s);
     } else {  // no message: every in-edge goes to a hub (or there are none)
         B->srcloc.alloc(1);
         B->slot.alloc(1);
+        B->qmsg.alloc(1);
     }
     B->msg.alloc(std::max<uint64_t>(P.nentry, 1));
     MDS_HIP(hipStreamSynchronize(s));

@@ -455,12 +455,12 @@ constexpr uint32_t kCombineWaves = kCombineThreads / kWave;
 constexpr uint32_t kCombineUnroll = 4;  // groups of four entries per thread in flight
 
 // Binned combine (one launch per superstep).  Workgroup b < nbins:
-//   1. stages its run table (message start and first entry number of every run) in LDS and
-//      zeroes its LDS slots, while every wave loads the records of its items (items w, w+8, ...
+//   1. zeroes its LDS slots while every wave loads the records of its items (items w, w+8, ...
 //      for wave w, one per lane) into registers;
-//   2. reads its entries four at a time: a binary search over the staged table gives a group
-//      its run, so the group's four messages (16 bytes) and four slots (8 bytes) load together;
-//      each message goes to its LDS slot (padding entries to a dummy slot);
+//   2. reads its entries four at a time: a group's message-array position (qmsg, planned on
+//      the host; a group never straddles two runs) and its four slots (8 bytes) load together,
+//      then its four messages (16 bytes); each message goes to its LDS slot (padding entries to
+//      a dummy slot);
 //   3. after the barrier, wave w sums its items from LDS in the canonical order.  A lane
 //      vertex sums as many slots as the largest in-degree of its group: the slots past its own
 //      in-degree hold +0.0, which merges as the exact identity the canonical tree uses for
@@ -470,19 +470,15 @@ constexpr uint32_t kCombineUnroll = 4;  // groups of four entries per thread in 
 // dangling blocks, whose base every wave merges here from the send's block pairs; otherwise
 // base[part] was written by k_pr_base.  kContrib: also write r / outdeg (the next superstep
 // runs another variant, which reads it).
-// Dynamic LDS after the slots: max_runs run starts and max_runs + 1 run numbers.
 template <bool kFold, bool kContrib>
 __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
     const PrBin* __restrict__ bins, uint32_t nbins, const PrItem* __restrict__ items,
-    const PrItem* __restrict__ hubs, uint32_t nhubs, const PrMsgRun* __restrict__ runs, uint32_t max_runs,
+    const PrItem* __restrict__ hubs, uint32_t nhubs, const uint32_t* __restrict__ qmsg,
     const float* __restrict__ msg, const uint16_t* __restrict__ slot, const uint32_t* __restrict__ cptr,
     const uint32_t* __restrict__ crow, const float* __restrict__ rank_in, const uint32_t* __restrict__ outdeg,
     const float* __restrict__ base_p, const float* __restrict__ partials, uint32_t nblk, uint32_t n, float d,
     float* __restrict__ rank_out, float* __restrict__ contrib_out) {
     __shared__ float lds[kPrBinValues + 4];  // + the dummy slot of padding entries
-    extern __shared__ uint32_t s_runs[];
-    uint32_t* s_rm = s_runs;
-    uint32_t* s_rk = s_runs + max_runs;
     using u16x4 = unsigned short __attribute__((ext_vector_type(4)));
     using f32x4 = float __attribute__((ext_vector_type(4)));
     const uint32_t lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
@@ -525,35 +521,26 @@ __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
         ps = partials[2 * lane];
         pc = partials[2 * lane + 1];
     }
-    const uint32_t nr = B.run_end - B.run_begin;
-    for (uint32_t j = threadIdx.x; j <= nr; j += kCombineThreads) {
-        const PrMsgRun R = runs[B.run_begin + j];  // j = nr: the sentinel, k = the bin's length
-        s_rk[j] = R.k;
-        if (j < nr) s_rm[j] = R.msg;
-    }
-    __syncthreads();  // the run table is staged
     // 2. entries into their slots: the first round's loads are issued before the slots are
     // zeroed, so they fly during the zeroing and its barrier
     const u16x4* __restrict__ s4 = reinterpret_cast<const u16x4*>(slot + B.slot_begin);
+    const uint32_t* __restrict__ g4 = qmsg + B.slot_begin / 4;
     const f32x4* __restrict__ m4 = reinterpret_cast<const f32x4*>(msg);
-    const uint32_t nq = s_rk[nr] / 4;
+    const uint32_t nq = B.entries / 4;
     f32x4 x[kCombineUnroll];
     u16x4 t[kCombineUnroll];
     auto fetch = [&](uint32_t q0) {
+        uint32_t at[kCombineUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kCombineUnroll; ++u) {
             const uint32_t q = q0 + u * kCombineThreads + threadIdx.x;
             if (q >= nq) continue;
-            const uint32_t k = 4 * q;
-            uint32_t lo = 0, hi = nr;  // s_rk[lo] <= k < s_rk[hi]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_rk[mid] <= k) lo = mid;
-                else hi = mid;
-            }
-            x[u] = m4[(s_rm[lo] + (k - s_rk[lo])) / 4];
+            at[u] = g4[q];
             t[u] = s4[q];
         }
+#pragma unroll
+        for (uint32_t u = 0; u < kCombineUnroll; ++u)
+            if (q0 + u * kCombineThreads + threadIdx.x < nq) x[u] = m4[at[u]];
     };
     auto drop = [&](uint32_t q0) {
 #pragma unroll
@@ -783,13 +770,12 @@ static void pr_run(medusa_graph* g, float damping, uint32_t steps, float* out, c
                 }
                 const bool keep = t + 1 < steps && var[t + 1] != MEDUSA_PR_BINNED;
                 const uint32_t cwg = Bn->nbins + (Bn->nhubs + kCombineWaves - 1) / kCombineWaves;
-                const size_t run_lds = (2 * (size_t)Bn->max_runs + 1) * 4;
                 const PrItem* hubs = Bn->items.p + (Bn->items.n - Bn->nhubs);
                 auto kern = fold ? (keep ? k_pr_combine<true, true> : k_pr_combine<true, false>)
                                  : (keep ? k_pr_combine<false, true> : k_pr_combine<false, false>);
-                hipLaunchKernelGGL(kern, dim3(cwg), dim3(kCombineThreads), run_lds, s, (const PrBin*)Bn->bins.p,
+                hipLaunchKernelGGL(kern, dim3(cwg), dim3(kCombineThreads), 0, s, (const PrBin*)Bn->bins.p,
                                    Bn->nbins, (const PrItem*)Bn->items.p, hubs, Bn->nhubs,
-                                   (const PrMsgRun*)Bn->runs.p, Bn->max_runs, (const float*)Bn->msg.p,
+                                   (const uint32_t*)Bn->qmsg.p, (const float*)Bn->msg.p,
                                    (const uint16_t*)Bn->slot.p, L.cptr.p, L.crow.p, r_in, L.outdeg.p,
                                    (const float*)L.base.p, (const float*)L.partials.p, nblk, n, d_f, r_out, c_out);
                 MDS_LAUNCHED();

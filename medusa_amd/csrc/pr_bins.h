@@ -62,7 +62,9 @@ struct PrItem {
 struct PrBin {
     uint32_t item_begin, item_end;
     uint32_t run_begin, run_end;
-    uint32_t slot_begin, pad[3];
+    uint32_t slot_begin;
+    uint32_t entries;  // the bin's entries (its runs' total length, a multiple of kPrRunAlign)
+    uint32_t pad[2];
 };
 // A run of entries from one chunk to one bin: `msg` = its first entry in the message array,
 // `k` = its first entry's number among the bin's entries (both multiples of kPrRunAlign); the
@@ -84,6 +86,8 @@ struct PrBinsPlan {
     std::vector<PrTile> tiles;
     std::vector<uint16_t> srcloc;  // message order: source offset in the chunk (0 for padding)
     std::vector<uint16_t> slot;    // bin order: LDS slot (kPrDummySlot for padding)
+    std::vector<uint32_t> qmsg;    // bin order, per group of kPrRunAlign entries: its message-array
+                                   // position / kPrRunAlign (a group never straddles two runs)
     uint32_t nhubs = 0;
     uint32_t max_runs = 0;  // most runs of any bin
     uint64_t nmsg = 0;      // messages (in-edges of non-hub destinations)
@@ -104,7 +108,7 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
     uint32_t group_max = 0;
     auto close_bin = [&] {
         if (P.items.size() > item_begin) {
-            P.bins.push_back({item_begin, (uint32_t)P.items.size(), 0, 0, 0, {0, 0, 0}});
+            P.bins.push_back({item_begin, (uint32_t)P.items.size(), 0, 0, 0, 0, {0, 0}});
             P.bin_slots.push_back(used);
         }
         item_begin = (uint32_t)P.items.size();
@@ -214,7 +218,18 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
         Bn.run_end = (uint32_t)P.runs.size();
         if (Bn.run_end - Bn.run_begin > kPrBinRuns) return "binned PageRank: a bin receives messages from too many chunks";
         P.max_runs = std::max(P.max_runs, Bn.run_end - Bn.run_begin);
-        P.runs.push_back({0u, slot_begin[b + 1] - slot_begin[b]});  // sentinel: the bin's length
+        Bn.entries = slot_begin[b + 1] - slot_begin[b];
+        P.runs.push_back({0u, Bn.entries});  // sentinel: the bin's length
+    }
+    P.qmsg.assign(P.nentry / kPrRunAlign, 0u);
+    for (uint64_t b = 0; b < B; ++b) {
+        const PrBin& Bn = P.bins[b];
+        for (uint32_t r = Bn.run_begin; r < Bn.run_end; ++r) {
+            const PrMsgRun R = P.runs[r];
+            const uint32_t e = r + 1 < Bn.run_end ? P.runs[r + 1].k : Bn.entries;
+            for (uint32_t k = R.k; k < e; k += kPrRunAlign)
+                P.qmsg[(Bn.slot_begin + k) / kPrRunAlign] = (R.msg + (k - R.k)) / kPrRunAlign;
+        }
     }
     for (uint64_t c = 0; c < C; ++c) {
         const uint32_t src = (uint32_t)(c * kPrChunkSources);
@@ -231,7 +246,9 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
 // addresses.  Returns "" or the first violation.
 inline std::string check_pr_bins(uint32_t n, const uint32_t* cptr, const PrBinsPlan& P) {
     const uint64_t nb = P.bins.size();
-    if (P.srcloc.size() != P.nentry || P.slot.size() != P.nentry) return "bins: array sizes";
+    if (P.srcloc.size() != P.nentry || P.slot.size() != P.nentry || P.qmsg.size() * kPrRunAlign != P.nentry ||
+        P.nentry % kPrRunAlign)
+        return "bins: array sizes";
     uint64_t real = 0;
     for (uint64_t b = 0; b < nb; ++b) {
         const PrBin& B = P.bins[b];
@@ -241,6 +258,7 @@ inline std::string check_pr_bins(uint32_t n, const uint32_t* cptr, const PrBinsP
             return "bins: run range";
         if (P.bin_slots[b] > kPrBinValues) return "bins: LDS slots";
         const uint32_t K = P.runs[B.run_end].k;  // the sentinel
+        if (K != B.entries) return "bins: entry count";
         if ((uint64_t)B.slot_begin + K > P.nentry || B.slot_begin % kPrRunAlign || K % kPrRunAlign)
             return "bins: slot range";
         if (B.run_begin < B.run_end && P.runs[B.run_begin].k != 0) return "bins: first run";
@@ -249,6 +267,9 @@ inline std::string check_pr_bins(uint32_t n, const uint32_t* cptr, const PrBinsP
             const uint32_t e = r + 1 < B.run_end ? P.runs[r + 1].k : K;
             if (R.k >= e || R.msg % kPrRunAlign || R.k % kPrRunAlign || e % kPrRunAlign) return "bins: run shape";
             if ((uint64_t)R.msg + (e - R.k) > P.nentry) return "bins: run bounds";
+            for (uint32_t k = R.k; k < e; k += kPrRunAlign)
+                if ((uint64_t)P.qmsg[(B.slot_begin + k) / kPrRunAlign] * kPrRunAlign != (uint64_t)R.msg + (k - R.k))
+                    return "bins: group position";
         }
         for (uint32_t i = 0; i < K; ++i) {
             const uint16_t t = P.slot[B.slot_begin + i];
