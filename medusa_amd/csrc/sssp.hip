@@ -17,8 +17,9 @@
 //
 // Near-queue entries are (vertex, slice) pairs: a vertex enters with one entry per
 // kSlice of its out-edges, so a hub's relaxation spreads over as many workgroups as it
-// has slices.  A workgroup relaxes 256 entries at a time through a block-wide prefix scan
-// of their degrees, and stages its queue appends in LDS (one atomic per flush).
+// has slices.  A workgroup relaxes up to 256 entries at a time (fewer when the queue is short,
+// so that every workgroup gets some) through a block-wide prefix scan of their degrees, and
+// stages its queue appends in LDS (one atomic per flush).
 //
 // While the near queue is short (a thin frontier), a workgroup keeps the vertices it lowered
 // below the threshold in LDS and relaxes them itself in further rounds of the same launch,
@@ -300,10 +301,20 @@ __device__ __forceinline__ unsigned long long local_rounds(LocalList* loc, uint3
     return relax;
 }
 
-// Relaxes near entries q[0, qn) with the whole block: workgroup b takes the 256 entries of
-// chunks b, b + grid, ....  With a local list and rounds > 0, the block then runs up to
-// `rounds` further tiles over the vertices it lowered itself.  Returns (in every thread) the
-// edges relaxed by this block.  kSpec as for relax_tile.
+// Entries per workgroup tile: 256, or fewer when the queue is short, so that every workgroup
+// of the grid gets a tile (a queue of 1000 entries, hub slices of 128 out-edges among them,
+// would otherwise give its edges to 4 workgroups).  With local rounds each workgroup then
+// continues from the vertices its own tile lowered, so a short queue also spreads those
+// rounds over the whole grid.
+__device__ __forceinline__ uint32_t relax_tile_entries(uint32_t qn) {
+    const uint32_t per = (uint32_t)(((uint64_t)qn + gridDim.x - 1) / gridDim.x);
+    return per < 1u ? 1u : (per > 256u ? 256u : per);
+}
+
+// Relaxes near entries q[0, qn) with the whole block: workgroup b takes the T entries of
+// chunks b, b + grid, ... (T = relax_tile_entries).  With a local list and rounds > 0, the
+// block then runs up to `rounds` further tiles over the vertices it lowered itself.  Returns
+// (in every thread) the edges relaxed by this block.  kSpec as for relax_tile.
 template <bool kSpec>
 __device__ __forceinline__ unsigned long long relax_block(const unsigned long long* __restrict__ q, uint32_t qn,
                                                           const uint32_t* __restrict__ col,
@@ -315,10 +326,11 @@ __device__ __forceinline__ unsigned long long relax_block(const unsigned long lo
     if (!loc) rounds = 0;
     unsigned long long relax = 0;
     uint32_t fmin = 0xFFFFFFFFu;  // not needed after a relaxation superstep
-    for (uint32_t chunk = blockIdx.x; (uint64_t)chunk * 256 < qn; chunk += gridDim.x) {  // uniform per block
-        const uint64_t i = (uint64_t)chunk * 256 + t;
+    const uint32_t T = relax_tile_entries(qn);
+    for (uint32_t chunk = blockIdx.x; (uint64_t)chunk * T < qn; chunk += gridDim.x) {  // uniform per block
+        const uint64_t i = (uint64_t)chunk * T + t;
         uint32_t beg = 0, deg = 0, du = 0;
-        if (i < qn) {
+        if (t < T && i < qn) {
             const unsigned long long ent = q[i];
             const uint32_t u = (uint32_t)ent, slice = (uint32_t)(ent >> 32);
             const uint32_t b = k.row_ptr[u], e = k.row_ptr[u + 1];
@@ -978,7 +990,7 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         if (timing) MDS_HIP(hipEventRecord(g->events[0], s));  // brackets the relaxation launch alone
         // 256 entries per workgroup chunk; the far queue keeps growing at far_a[n_far..)
         const uint32_t rounds = n_near <= kLocalQueue ? local_cfg : 0u;
-        hipLaunchKernelGGL(rounds ? k_sssp_relax<true> : k_sssp_relax<false>, dim3(grid_for(n_near, 256, kSsspGrid)),
+        hipLaunchKernelGGL(rounds ? k_sssp_relax<true> : k_sssp_relax<false>, dim3(grid_for(n_near, 1, kSsspGrid)),
                            dim3(256), 0, s, near_cur, n_near, g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p,
                            epoch, thr, near_nxt, (uint32_t)near_cap, far_a, (uint32_t)far_cap, C, rounds);
         MDS_LAUNCHED();
