@@ -352,27 +352,37 @@ def _train_mbfs_policy(md, wl, np):
 
 def _train_delta_policy(md, wl, np, family):
     """Delta-width cost model trained on the GPU from traced SSSP runs of other graphs of the
-    same family (two lattices of other seeds, one of them the measured size, or RMAT scale 14-16
-    for RMAT), over a sweep of fixed widths."""
+    same family and size (two lattices of other seeds, one of them the measured size, or three
+    RMAT-20 graphs of other seeds), over a sweep of fixed widths."""
     if family == "sssp_lattice":
         graphs = [wl.lattice_graph(r, c, seed=sd) + (wl.lattice_source(r, c),)
                   for r, c, sd in ((1000, 1000, 32), (1400, 1400, 31))]
     else:
         graphs = []
-        for sc in (14, 15, 16):
-            rp, col = wl.rmat_graph(scale=sc, edge_factor=16, seed=40 + sc)
-            graphs.append((rp, col, wl.weighted(rp, col, seed=sc), wl.max_outdeg_vertex(rp)))
+        for sd in (41, 42, 43):
+            rp, col = wl.rmat_graph(scale=20, edge_factor=16, seed=sd)
+            graphs.append((rp, col, wl.weighted(rp, col, seed=sd + 100), wl.max_outdeg_vertex(rp)))
     Xs, ys = [], []
     for rp, col, wt, src in graphs:
         with md.Graph(rp, col, wt) as H:
-            for d in (250, 500, 1000, 2000, 4000, 8000, 16000, 32000):
-                H.sssp(src, delta=d, trace=1)
-                X, y = md.delta_samples(H.trace())  # cost to the end of the run per vertex left
-                Xs.append(X)
-                ys.append(y)
+            H.sssp(src, trace=1)
+            # every width the policy chooses among, two runs each; one row per run (features at
+            # the run's start, whole-run cost per vertex): the policy picks a run's width there
+            for d in md.delta_candidates(H.trace()[0, 2]):
+                for _ in range(2):
+                    H.sssp(src, delta=d, trace=1)
+                    x, y = md.delta_run_sample(H.trace())
+                    Xs.append(x)
+                    ys.append(y)
     pol = md.Policy(md.POLICY_DELTA, seed=3)
-    pol.train(np.concatenate(Xs), np.concatenate(ys), iters=3000, lr=3e-3)
+    pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
     return pol
+
+
+def _first_width(G, src, pol):
+    """The bucket width the policy picks for a run (phase 0 of a traced run)."""
+    G.sssp(src, policy=pol, trace=1)
+    return G.trace()[0, 7]
 
 
 def bfs_block(md, wl, torch, np, rp, col, src, steps, warmup, oracle_check=False):
@@ -418,6 +428,7 @@ def sssp_block(md, wl, torch, np, rp, col, wt, src, family, steps, warmup, oracl
          "roofline": _roofline([G.sssp(src, out=out, timing=1)[1] for _ in range(2)],
                                "k_sssp_relax (relaxation superstep)"),
          "gteps_learned": round(steps * m_r / wl_ / 1e9, 4), "supersteps_learned": rl_[-1].supersteps,
+         "width_plain": int(_first_width(G, src, None)), "width_learned": int(_first_width(G, src, pol)),
          "learned_over_plain": round(wp / wl_, 3),
          "distances_identical_to_plain": bool(np.array_equal(plain, out.cpu().numpy()))}
     if oracle_check:
@@ -670,8 +681,10 @@ def run_workload(a, md, wl, torch, np):
         m_r = res[-1].edges_traversed
         wp, rp_ = _timed(lambda: G.sssp(src, out=out)[1], a.steps, 1, torch)
         extra = {"supersteps": res[-1].supersteps, "phases": res[-1].reserved[1], "edges_reached": m_r,
-                 "policy": "delta width per threshold advance by the MLP, trained on the GPU from traced runs of "
-                           "other graphs of the same family (other seeds; not this graph)",
+                 "policy": "bucket width chosen once per run by the MLP from the first phase's features, trained "
+                           "on the GPU from traced fixed-width runs of other graphs of the same family and size "
+                           "(other seeds; not this graph)",
+                 "width_learned": int(_first_width(G, src, pol)),
                  "gteps_plain": round(a.steps * m_r / wp / 1e9, 4), "supersteps_plain": rp_[-1].supersteps,
                  "learned_over_plain": round(wp / wall, 3),
                  "distances_identical_to_plain": bool(np.array_equal(learned, out.cpu().numpy()))}

@@ -6,6 +6,7 @@ here computes a result on the CPU: if the library is missing or no GPU is
 visible, the calls raise.
 """
 import ctypes
+import math
 import pathlib
 
 import numpy as np
@@ -433,6 +434,20 @@ def delta_samples(trace):
         X[j] = policy_features(POLICY_DELTA, r, r[7])
     y = np.log2(ms_left[keep] * 1e6 / settled_left[keep]).astype(np.float32)
     return X, y
+
+
+def delta_candidates(wmean):
+    """The bucket widths the width policy chooses among for a graph of (sampled) mean weight
+    `wmean`, as the engine computes them: wmean * 2^k, k = -2..8, within [1, 2^32 - 2]."""
+    return [int(min(4294967294.0, max(1.0, math.ldexp(float(wmean), k)))) for k in range(-2, 9)]
+
+
+def delta_run_sample(trace):
+    """The width policy's training row of one fixed-width SSSP run: the features at the run's
+    start with its width, and the cost of the whole run per settled vertex (log2 ns), i.e. row 0
+    of delta_samples.  The engine consults the policy once per run, at that point."""
+    X, y = delta_samples(trace)
+    return X[0], y[0]
 
 
 class Policy:

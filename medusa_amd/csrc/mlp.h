@@ -17,11 +17,15 @@ constexpr float kMlpSlope = 1.0f / 64.0f;
 // be cheaper by more than a margin (log2 units of cost); otherwise the plain engine's choice
 // stands, so an uncertain model falls back to it.  Direction: 1.0 (2x), because the tuned
 // direction heuristics are close to the best choice per superstep on the graphs measured;
-// bucket width: 1.0, because the width model, trained on smaller graphs, errs by more than
-// 19 % on large lattices; kernel variant: 0.25 (about 19 %).
+// kernel variant: 0.25 (about 19 %); bucket width: 0.1 (7 %; the width model's fit error on
+// its training runs is about 0.06, and the best width on RMAT-20 beats the default by
+// 12-24 %).  The bucket width is chosen once per run, from the features at its first phase,
+// whose training rows carry the cost of whole runs at each fixed width; switching widths at
+// later phases on the network's predictions lost 5-22 % on 1400x1400 lattices (its training
+// runs never switch).
 constexpr float kPolicyMargin = 0.25f;
 constexpr float kDirectionMargin = 1.0f;
-constexpr float kWidthMargin = 1.0f;
+constexpr float kWidthMargin = 0.1f;
 
 // Direction from the two predicted costs and the plain engine's choice.
 inline __host__ __device__ bool policy_pull(float c_push, float c_pull, bool plain_pull) {

@@ -202,9 +202,10 @@ struct Sched {
         const float c_pull = mlp_eval_host(policy_params(o.policy), x);
         return policy_pull(c_push, c_pull, heuristic_pull(f, cur_pull));
     }
-    // SSSP bucket width at a threshold advance: the learned policy picks among
-    // wmean * 2^k (k = -2..8) the width with the lowest predicted time per unit of distance,
-    // if it beats the predicted cost of the default width by the policy margin.
+    // SSSP bucket width of a run, from the features at its first phase: the learned policy
+    // picks among wmean * 2^k (k = -2..8) the width with the lowest predicted cost to the end
+    // of the run per vertex, if it beats the predicted cost of the default width by
+    // kWidthMargin.
     uint32_t sssp_delta(const Frontier& f, uint32_t dflt, uint32_t wmean, uint32_t thr) const {
         if (!o.policy || policy_kind(o.policy) != MEDUSA_POLICY_DELTA || o.delta) return dflt;
         double rec[MEDUSA_TRACE_FIELDS];

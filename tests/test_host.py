@@ -173,3 +173,21 @@ def test_delta_samples_charge_the_rest_of_the_run():
     assert np.allclose(y, want, rtol=1e-6), (y, want)
     for j in range(3):
         assert np.array_equal(X[j], md.policy_features(md.POLICY_DELTA, tr[j], tr[j, 7]))
+
+
+def test_delta_run_sample_and_candidates():
+    """A run's single training row is row 0 of the per-phase rows (the whole run's cost per
+    settled vertex); the candidate widths are mean x 2^k, k = -2..8, clamped below 2^32 - 1."""
+    tr = np.zeros((3, md.TRACE_FIELDS))
+    tr[:, 5], tr[:, 6] = 1000, 3000
+    tr[:, 2] = 100
+    tr[:, 3] = [0, 400, 800]
+    tr[:, 4] = [5, 5, 10]
+    tr[:, 7] = 400
+    tr[:, 8] = [1.0, 2.0, 1.0]
+    x, y = md.delta_run_sample(tr)
+    assert np.array_equal(x, md.policy_features(md.POLICY_DELTA, tr[0], 400))
+    assert np.isclose(y, np.log2(4e6 / 20))
+    assert md.delta_candidates(100) == [25, 50, 100, 200, 400, 800, 1600, 3200, 6400, 12800, 25600]
+    assert md.delta_candidates(1)[:3] == [1, 1, 1]
+    assert max(md.delta_candidates(2 ** 28)) == 4294967294
