@@ -223,7 +223,7 @@ struct TraceRec {
 // An instantiated hipGraph of captured supersteps and the buffers it was captured with.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    uintptr_t key[5] = {0, 0, 0, 0, 0};
+    uintptr_t key[6] = {0, 0, 0, 0, 0, 0};
     GraphCache() = default;
     GraphCache(const GraphCache&) = delete;
     GraphCache& operator=(const GraphCache&) = delete;

@@ -451,17 +451,17 @@ __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__
 }
 
 // ---- device-driven loop ----------------------------------------------------------------
-// Each slot of the captured loop is two kernels:
+// Each slot of the captured loop is one work kernel:
 //   work:   if a split is pending, move far entries below the new threshold to the near queue
 //           (a full rescan of the distances if the far queue overflowed); otherwise relax the
 //           current near queue, and, while that queue is short (at most kLocalQueue entries),
 //           let each workgroup relax up to `local_cfg` more rounds of the vertices it lowered
 //           itself, so a thin frontier (a road-like graph) advances several hops per superstep;
-//   decide: one workgroup folds the counters into the control block; after a relaxation
-//           that left the near queue empty the phase ends: it raises the threshold by the
-//           run's bucket width (default, fixed, or chosen by the learned policy before the
-//           run) and marks a split pending; after a split, an empty bucket raises the
-//           threshold past the smallest far distance.
+//   decide: the last workgroup to finish (by ticket) folds the counters into the control
+//           block; after a relaxation that left the near queue empty the phase ends: it
+//           raises the threshold by the run's bucket width (default, fixed, or chosen by the
+//           learned policy before the run) and marks a split pending; after a split, an empty
+//           bucket raises the threshold past the smallest far distance.
 // kSsspBatch slots are captured once as a hipGraph and replayed until done.
 
 constexpr uint32_t kSsspBatch = 32;
@@ -489,90 +489,36 @@ __device__ __forceinline__ void phase_mark(unsigned long long* marks, uint32_t i
     marks[3 * i + 2] = (unsigned long long)wall_clock64();
 }
 
+// Tickets of the last-workgroup election: workgroup b takes a ticket on line b mod kSsspSlots,
+// and the last of a line's workgroups one on `top`, so no address sees more than
+// grid / kSsspSlots atomics (same-address device atomics serialise).
+constexpr uint32_t kSsspSlots = 32;
+struct SsspTicket {
+    alignas(128) uint32_t v;
+};
 struct SsspDevCounters {
     MDS_COUNTER(uint32_t, n_near);   // near entries written by this slot's work kernel
     MDS_COUNTER(uint32_t, n_far);    // far queue length (relax appends after the current entries)
     MDS_COUNTER(uint32_t, far_out);  // far entries kept by a split
     MDS_COUNTER(uint32_t, far_min);  // smallest distance kept by a split
     MDS_COUNTER(unsigned long long, relax);
+    SsspTicket sub[kSsspSlots];
+    MDS_COUNTER(uint32_t, top);
 };
+// storage units of the device-driven loop: the control block, then the counters
+constexpr size_t kSsspCtlUnits = 1 + (sizeof(SsspDevCounters) + sizeof(CtlStorage) - 1) / sizeof(CtlStorage);
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
     const unsigned long long s = (unsigned long long)a + b;
     return s >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)s;
 }
 
-template <bool kLoc>
-__global__ __launch_bounds__(256) void k_sssp_work(const SsspCtl* __restrict__ ctl,
-                                                   unsigned long long* __restrict__ near0,
-                                                   unsigned long long* __restrict__ near1, uint32_t near_cap,
-                                                   uint32_t* __restrict__ far0, uint32_t* __restrict__ far1,
-                                                   uint32_t far_cap, const uint32_t* __restrict__ row_ptr,
-                                                   const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
-                                                   uint32_t* dist, uint32_t* __restrict__ stamp,
-                                                   SsspDevCounters* __restrict__ ctr) {
-    __shared__ Stage st;
-    LocalList* loc = nullptr;
-    if constexpr (kLoc) {
-        __shared__ LocalList s_loc;
-        loc = &s_loc;
-    }
-    const SsspCtl c = *ctl;
-    // every field this kernel reads, loaded in one round trip
-    keep_here(c.done);
-    keep_here(c.local_rounds);
-    keep_here(c.split_pending);
-    keep_here(c.n_near);
-    keep_here(c.epoch);
-    keep_here(c.thr);
-    keep_here(c.old_thr);
-    keep_here(c.near_sel);
-    keep_here(c.far_sel);
-    keep_here(c.overflow);
-    keep_here(c.n_far);
-    keep_here((uint32_t)c.n);
-    if (c.done) return;
-    const uint32_t epoch = c.epoch, thr = c.thr;
-    unsigned long long ndeg = 0;
-    if (c.split_pending) {
-        const bool rescan = c.overflow != 0;
-        const uint32_t nin = rescan ? (uint32_t)c.n : c.n_far;
-        const Sinks k{c.near_sel ? near1 : near0, &ctr->n_near, near_cap, c.far_sel ? far0 : far1,
-                      &ctr->far_out, far_cap, row_ptr};
-        LocalList* sl = kLoc && c.local_rounds ? loc : nullptr;
-        if (sl && threadIdx.x == 0) sl->n[0] = 0;
-        stage_init(st);  // its barrier publishes sl->n[0]
-        uint32_t fmin = split_block(c.far_sel ? far1 : far0, nin, rescan, dist, stamp, epoch, c.old_thr, thr,
-                                    st, k, ndeg, sl);
-        if (sl) {
-            // the moved vertices this block kept are relaxed at once; their stamps use the next
-            // epoch, so a vertex this split kept far still enters the near queue if it is lowered
-            // below the threshold
-            const unsigned long long relax =
-                local_rounds<kLoc>(sl, c.local_rounds, col, w, dist, stamp, epoch + 1, thr, st, k, ndeg, fmin);
-            if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
-        }
-        stage_flush(st, k, true);
-        fmin = wave_min_u32(fmin);
-        if ((threadIdx.x & (kWave - 1)) == 0 && fmin != 0xFFFFFFFFu) atomicMin(&ctr->far_min, fmin);
-        return;
-    }
-    const uint32_t N = c.n_near;
-    if (N == 0) return;
-    const Sinks k{c.near_sel ? near0 : near1, &ctr->n_near, near_cap, c.far_sel ? far1 : far0, &ctr->n_far,
-                  far_cap, row_ptr};
-    if (loc && threadIdx.x == 0) loc->n[0] = 0;
-    stage_init(st);  // its barrier publishes loc->n[0]
-    const unsigned long long relax = relax_block<kLoc>(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch,
-                                                       thr, st, k, ndeg, loc, kLoc ? c.local_rounds : 0u);
-    stage_flush(st, k, true);
-    if (threadIdx.x == 0 && relax) atomicAdd(&ctr->relax, relax);
-}
-
-// One workgroup after the work kernel (its counters are visible at the kernel boundary).
-__global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, SsspDevCounters* __restrict__ ctr,
-                                                     uint32_t far_cap, uint32_t near_cap,
-                                                     unsigned long long* __restrict__ marks) {
+// The slot's decision, by one thread of the work kernel's last workgroup (every other
+// workgroup's counter atomics were performed before its ticket): folds the counters into the
+// control block and clears them and the tickets for the next slot.  The next work kernel reads
+// both after the kernel boundary.
+__device__ void sssp_decide(SsspCtl* ctl, SsspDevCounters* ctr, uint32_t far_cap, uint32_t near_cap,
+                            unsigned long long* __restrict__ marks) {
     SsspDevCounters k;
     k.n_near = load_now(&ctr->n_near);
     k.n_far = load_now(&ctr->n_far);
@@ -580,7 +526,6 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
     k.far_min = load_now(&ctr->far_min);
     k.relax = load_now(&ctr->relax);
     SsspCtl c = load_now_block(ctl);
-    if (c.done) return;
     if (k.n_near > near_cap) {
         // entries past the queue were dropped by the writers (near_capacity rules this out):
         // stop, and let the host report it, rather than read past the queue
@@ -664,16 +609,117 @@ __global__ __launch_bounds__(256) void k_sssp_decide(SsspCtl* __restrict__ ctl, 
         }
     }
     c.local_rounds = c.n_near <= kLocalQueue ? c.local_cfg : 0u;
-    __syncthreads();  // every wave has read the control block and counters
+    if (c.trace && mark != 0xFFFFFFFFu) phase_mark(marks, mark, mark_phase, mark_far, mark_thr, mark_delta);
+    *ctl = c;
+    ctr->n_near = 0;
+    ctr->n_far = n_far;
+    ctr->far_out = 0;
+    ctr->far_min = 0xFFFFFFFFu;
+    ctr->relax = 0;
+    for (uint32_t i = 0; i < kSsspSlots; ++i) ctr->sub[i].v = 0;
+    ctr->top = 0;
+}
+
+// End of a work kernel's workgroup: thread 0 takes the tickets after every counter atomic of
+// the block has returned (the callers wait for their results before the barrier here, and
+// `dep` carries thread 0's last one), so the workgroup that takes the last ticket sees every
+// other workgroup's counts.  Returns true in every thread of that workgroup.
+__device__ __forceinline__ bool sssp_last_block(SsspDevCounters* __restrict__ ctr, unsigned long long dep) {
+    __shared__ uint32_t s_last;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        if (c.trace && mark != 0xFFFFFFFFu) phase_mark(marks, mark, mark_phase, mark_far, mark_thr, mark_delta);
-        *ctl = c;
-        ctr->n_near = 0;
-        ctr->n_far = n_far;
-        ctr->far_out = 0;
-        ctr->far_min = 0xFFFFFFFFu;
-        ctr->relax = 0;
+        uint32_t inc = 1;
+        asm volatile("" : "+v"(inc) : "v"(dep));
+        const uint32_t sl = blockIdx.x % kSsspSlots;
+        const uint32_t members = (gridDim.x - sl + kSsspSlots - 1) / kSsspSlots;  // workgroups of line sl
+        const uint32_t nslots = gridDim.x < kSsspSlots ? gridDim.x : kSsspSlots;
+        bool last = false;
+        if (atomicAdd(&ctr->sub[sl].v, inc) == members - 1) {
+            uint32_t inc2 = 1;
+            asm volatile("" : "+v"(inc2) : "v"(members));
+            last = atomicAdd(&ctr->top, inc2) == nslots - 1;
+        }
+        s_last = last ? 1u : 0u;
     }
+    __syncthreads();
+    return s_last != 0;
+}
+
+template <bool kLoc>
+__global__ __launch_bounds__(256) void k_sssp_work(SsspCtl* ctl,
+                                                   unsigned long long* __restrict__ near0,
+                                                   unsigned long long* __restrict__ near1, uint32_t near_cap,
+                                                   uint32_t* __restrict__ far0, uint32_t* __restrict__ far1,
+                                                   uint32_t far_cap, const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ col, const uint32_t* __restrict__ w,
+                                                   uint32_t* dist, uint32_t* __restrict__ stamp,
+                                                   SsspDevCounters* __restrict__ ctr,
+                                                   unsigned long long* __restrict__ marks) {
+    __shared__ Stage st;
+    LocalList* loc = nullptr;
+    if constexpr (kLoc) {
+        __shared__ LocalList s_loc;
+        loc = &s_loc;
+    }
+    const SsspCtl c = *ctl;
+    // every field this kernel reads, loaded in one round trip
+    keep_here(c.done);
+    keep_here(c.local_rounds);
+    keep_here(c.split_pending);
+    keep_here(c.n_near);
+    keep_here(c.epoch);
+    keep_here(c.thr);
+    keep_here(c.old_thr);
+    keep_here(c.near_sel);
+    keep_here(c.far_sel);
+    keep_here(c.overflow);
+    keep_here(c.n_far);
+    keep_here((uint32_t)c.n);
+    if (c.done) return;  // uniform: no workgroup takes a ticket
+    const uint32_t epoch = c.epoch, thr = c.thr;
+    unsigned long long ndeg = 0;
+    unsigned long long dep = 0;  // thread 0's last counter atomic result
+    if (c.split_pending) {
+        const bool rescan = c.overflow != 0;
+        const uint32_t nin = rescan ? (uint32_t)c.n : c.n_far;
+        const Sinks k{c.near_sel ? near1 : near0, &ctr->n_near, near_cap, c.far_sel ? far0 : far1,
+                      &ctr->far_out, far_cap, row_ptr};
+        LocalList* sl = kLoc && c.local_rounds ? loc : nullptr;
+        if (sl && threadIdx.x == 0) sl->n[0] = 0;
+        stage_init(st);  // its barrier publishes sl->n[0]
+        uint32_t fmin = split_block(c.far_sel ? far1 : far0, nin, rescan, dist, stamp, epoch, c.old_thr, thr,
+                                    st, k, ndeg, sl);
+        if (sl) {
+            // the moved vertices this block kept are relaxed at once; their stamps use the next
+            // epoch, so a vertex this split kept far still enters the near queue if it is lowered
+            // below the threshold
+            const unsigned long long relax =
+                local_rounds<kLoc>(sl, c.local_rounds, col, w, dist, stamp, epoch + 1, thr, st, k, ndeg, fmin);
+            if (threadIdx.x == 0 && relax) dep += atomicAdd(&ctr->relax, relax);
+        }
+        stage_flush(st, k, true);  // thread 0 stored its atomics' results in LDS
+        fmin = wave_min_u32(fmin);
+        if ((threadIdx.x & (kWave - 1)) == 0 && fmin != 0xFFFFFFFFu) {
+            const uint32_t r = atomicMin(&ctr->far_min, fmin);
+            asm volatile("" ::"v"(r));  // wait for it before the barrier in sssp_last_block
+        }
+        if (sssp_last_block(ctr, dep) && threadIdx.x == 0) sssp_decide(ctl, ctr, far_cap, near_cap, marks);
+        return;
+    }
+    const uint32_t N = c.n_near;
+    if (N == 0) {  // not reached: a decision leaves either a split pending or entries to relax
+        if (sssp_last_block(ctr, dep) && threadIdx.x == 0) sssp_decide(ctl, ctr, far_cap, near_cap, marks);
+        return;
+    }
+    const Sinks k{c.near_sel ? near0 : near1, &ctr->n_near, near_cap, c.far_sel ? far1 : far0, &ctr->n_far,
+                  far_cap, row_ptr};
+    if (loc && threadIdx.x == 0) loc->n[0] = 0;
+    stage_init(st);  // its barrier publishes loc->n[0]
+    const unsigned long long relax = relax_block<kLoc>(c.near_sel ? near1 : near0, N, col, w, dist, stamp, epoch,
+                                                       thr, st, k, ndeg, loc, kLoc ? c.local_rounds : 0u);
+    stage_flush(st, k, true);  // thread 0 stored its atomics' results in LDS
+    if (threadIdx.x == 0 && relax) dep += atomicAdd(&ctr->relax, relax);
+    if (sssp_last_block(ctr, dep) && threadIdx.x == 0) sssp_decide(ctl, ctr, far_cap, near_cap, marks);
 }
 
 __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, unsigned long long m, uint32_t thr,
@@ -698,6 +744,8 @@ __global__ void k_sssp_seed_ctl(SsspCtl* ctl, SsspDevCounters* c, uint32_t n, un
     c->far_out = 0;
     c->far_min = 0xFFFFFFFFu;
     c->relax = 0;
+    for (uint32_t i = 0; i < kSsspSlots; ++i) c->sub[i].v = 0;
+    c->top = 0;
 }
 
 // Local rounds per superstep (while the near queue is short): the caller's count, or by
@@ -792,10 +840,11 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     // from the features at the first phase
     const uint32_t d0 = sched.sssp_delta(f, o.delta ? o.delta : dflt, wmean, 0);
     const uint32_t thr0 = d0 >= 0xFFFFFFFFu - 1 ? 0xFFFFFFFFu : d0;
-    g->sssp_ctl.ensure(1);
+    // control block in unit 0 of the storage, counters and tickets after it
+    g->sssp_ctl.ensure(kSsspCtlUnits);
     SsspCtl* ctl = reinterpret_cast<SsspCtl*>(g->sssp_ctl.p);
-    static_assert(sizeof(SsspCtl) <= sizeof(CtlStorage), "control block storage");
-    SsspDevCounters* ctr = dev_counters<SsspDevCounters>(g);
+    static_assert(sizeof(SsspCtl) <= sizeof(CtlStorage) - 16, "control block storage (its last 16 bytes: reached counts)");
+    SsspDevCounters* ctr = reinterpret_cast<SsspDevCounters*>(g->sssp_ctl.p + 1);
     uint32_t* far0 = g->q_aux.p;
     uint32_t* far1 = g->q_aux.p + sssp_far_alloc(g);
     const size_t near_cap = near_capacity(n, m);
@@ -807,24 +856,22 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
     unsigned long long* marks = g->sssp_phases.p;
     const uint32_t local_cfg = sssp_local_cfg(g, o);
     // the marks buffer is 256-byte aligned, so its low bit can carry the kernel variant
-    const uintptr_t key[5] = {(uintptr_t)g->dist.p, (uintptr_t)g->near_q.p, (uintptr_t)g->q_aux.p,
-                              (uintptr_t)g->w.p, (uintptr_t)marks | (local_cfg ? 1u : 0u)};
-    if (!gc.exec || !std::equal(key, key + 5, gc.key)) {
+    const uintptr_t key[6] = {(uintptr_t)g->dist.p, (uintptr_t)g->near_q.p, (uintptr_t)g->q_aux.p,
+                              (uintptr_t)g->w.p, (uintptr_t)marks | (local_cfg ? 1u : 0u), (uintptr_t)ctl};
+    if (!gc.exec || !std::equal(key, key + 6, gc.key)) {
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (uint32_t k = 0; k < kSsspBatch; ++k) {
             hipLaunchKernelGGL(local_cfg ? k_sssp_work<true> : k_sssp_work<false>, dim3(kSsspGrid), dim3(256), 0, s,
-                               (const SsspCtl*)ctl, near0, near1, (uint32_t)near_cap, far0, far1, (uint32_t)far_cap,
-                               g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, ctr);
-            hipLaunchKernelGGL(k_sssp_decide, dim3(1), dim3(256), 0, s, ctl, ctr, (uint32_t)far_cap,
-                               (uint32_t)near_cap, marks);
+                               ctl, near0, near1, (uint32_t)near_cap, far0, far1, (uint32_t)far_cap,
+                               g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, ctr, marks);
         }
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
         if (graph) (void)hipGraphDestroy(graph);
         MDS_HIP(e);
-        std::copy(key, key + 5, gc.key);
+        std::copy(key, key + 6, gc.key);
     }
 
     MDS_HIP(hipEventRecord(g->ev_begin, s));
