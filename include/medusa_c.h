@@ -226,12 +226,13 @@ int medusa_trace_read(const medusa_graph* g, double* out, uint32_t capacity);
  * A small MLP cost model (16 inputs -> 32 -> 32 -> 1, leaky ReLU) trained on the GPU by the
  * library's own forward/backward kernels.  Input row layout (MEDUSA_POLICY_INPUTS floats) is
  * produced by medusa_policy_features(); the engine evaluates every candidate action at each
- * superstep and takes the lowest predicted cost.  Results never depend on the choice.
+ * superstep (the SSSP width: once per run, at its start) and takes the lowest predicted cost.
+ * Results never depend on the choice.
  * No reference counterpart. */
 #define MEDUSA_POLICY_INPUTS 16
 #define MEDUSA_POLICY_PARAMS 1633
 #define MEDUSA_POLICY_DIRECTION 1 /* BFS / MBFS: push or pull per superstep */
-#define MEDUSA_POLICY_DELTA 2     /* SSSP: bucket width at each threshold advance */
+#define MEDUSA_POLICY_DELTA 2     /* SSSP: bucket width of a run, chosen at its start */
 #define MEDUSA_POLICY_VARIANT 3   /* PageRank: pull or push kernel variant per superstep */
 int medusa_policy_create(uint32_t kind, uint64_t seed, medusa_policy** out);
 int medusa_policy_destroy(medusa_policy* p);
