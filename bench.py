@@ -284,6 +284,25 @@ def _timed(fn, steps, warmup, torch):
     return time.perf_counter() - t0, res
 
 
+AB_REPS = 15
+
+
+def _ab(fa, fb, reps=AB_REPS):
+    """A/B ratio of two forms of one run (each a call returning its Stats): one run of each,
+    alternately, `reps` times after one of each as warm-up; returns (median device ms of A,
+    median device ms of B, A / B).  Alternating cancels clock and thermal drift, which
+    sequential timing of one form after the other (with policy training between them) does
+    not; device_ms is the library's own event span of the whole run on its stream."""
+    fa()
+    fb()
+    ta, tb = [], []
+    for _ in range(reps):
+        ta.append(fa().device_ms)
+        tb.append(fb().device_ms)
+    ma, mb = statistics.median(ta), statistics.median(tb)
+    return ma, mb, ma / mb
+
+
 def _roofline(stats, kernel, traffic=None):
     """HIP events around every superstep (all its launches, with the gaps between them) of
     host-driven timing runs on the library's stream; bytes = the engine's algorithmic model."""
@@ -401,8 +420,9 @@ def bfs_block(md, wl, torch, np, rp, col, src, steps, warmup, oracle_check=False
          "edges_reached": m_r,
          "roofline": _roofline([G.bfs(src, out=out, timing=1)[1] for _ in range(3)], BFS_KERNEL,
                                "bfs_rmat20_traffic.md"),
-         "gteps_learned": round(steps * m_r / wl_ / 1e9, 3), "learned_over_plain": round(wall / wl_, 3),
+         "gteps_learned": round(steps * m_r / wl_ / 1e9, 3),
          "levels_identical_to_plain": bool(np.array_equal(plain, out.cpu().numpy()))}
+    d["learned_over_plain"] = round(_ab(lambda: G.bfs(src, out=out)[1], lambda: G.bfs(src, out=out, policy=pol)[1])[2], 3)
     if oracle_check:
         import oracle
 
@@ -429,8 +449,9 @@ def sssp_block(md, wl, torch, np, rp, col, wt, src, family, steps, warmup, oracl
                                "k_sssp_relax (relaxation superstep)"),
          "gteps_learned": round(steps * m_r / wl_ / 1e9, 4), "supersteps_learned": rl_[-1].supersteps,
          "width_plain": int(_first_width(G, src, None)), "width_learned": int(_first_width(G, src, pol)),
-         "learned_over_plain": round(wp / wl_, 3),
          "distances_identical_to_plain": bool(np.array_equal(plain, out.cpu().numpy()))}
+    d["learned_over_plain"] = round(_ab(lambda: G.sssp(src, out=out)[1],
+                                        lambda: G.sssp(src, out=out, policy=pol)[1])[2], 3)
     if oracle_check:
         import oracle
 
@@ -473,7 +494,9 @@ def mbfs_block(md, wl, torch, np, rp, col, srcs, steps, warmup):
          "supersteps": rl_[-1].supersteps, "edges_reached_all_sources": m_r,
          "roofline": _roofline([G.mbfs(srcs, out=out, policy=pol, timing=1)[1] for _ in range(3)],
                                "k_mbfs_push / k_mbfs_pull (mask superstep)"),
-         "gteps_heuristic": round(steps * m_r / wh / 1e9, 3), "learned_over_heuristic": round(wh / wl_, 3),
+         "gteps_heuristic": round(steps * m_r / wh / 1e9, 3),
+         "learned_over_heuristic": round(_ab(lambda: G.mbfs(srcs, out=out)[1],
+                                             lambda: G.mbfs(srcs, out=out, policy=pol)[1])[2], 3),
          "levels_identical_learned_vs_heuristic": same,
          "plain_64_bfs_gteps": round(plain_edges / tp / 1e9, 3),
          "batched_over_plain": round((steps * m_r / wl_) / (plain_edges / tp), 3),
@@ -637,7 +660,8 @@ def run_workload(a, md, wl, torch, np):
         pol = _train_direction_policy(md, wl, np)
         wl_, rl_ = _timed(lambda: G.bfs(src, out=out, policy=pol)[1], a.steps, 1, torch)
         extra.update({"gteps_learned": round(a.steps * m_r / wl_ / 1e9, 3),
-                      "learned_over_plain": round(wall / wl_, 3),
+                      "learned_over_plain": round(_ab(lambda: G.bfs(src, out=out)[1],
+                                                      lambda: G.bfs(src, out=out, policy=pol)[1])[2], 3),
                       "learned_push_pull_steps": [rl_[-1].push_steps, rl_[-1].pull_steps],
                       "levels_identical_to_plain": bool(np.array_equal(plain_levels, out.cpu().numpy())),
                       "policy": "direction per superstep by the MLP, trained on the GPU from push-only/pull-only "
@@ -686,7 +710,8 @@ def run_workload(a, md, wl, torch, np):
                            "(other seeds; not this graph)",
                  "width_learned": int(_first_width(G, src, pol)),
                  "gteps_plain": round(a.steps * m_r / wp / 1e9, 4), "supersteps_plain": rp_[-1].supersteps,
-                 "learned_over_plain": round(wp / wall, 3),
+                 "learned_over_plain": round(_ab(lambda: G.sssp(src, out=out)[1],
+                                                 lambda: G.sssp(src, out=out, policy=pol)[1])[2], 3),
                  "distances_identical_to_plain": bool(np.array_equal(learned, out.cpu().numpy()))}
         # fixed widths, and the learned widths with one hop per superstep (no local rounds)
         for name, kw in (("bellman_ford", {"delta": 0xFFFFFFFF}), ("delta_1000", {"delta": 1000}),
@@ -744,7 +769,8 @@ def run_workload(a, md, wl, torch, np):
                  "policy": "direction per superstep by the MLP, trained on the GPU from push-only/pull-only "
                            "mask-engine traces of RMAT scale 14-17 (not this graph)",
                  "gteps_batched_heuristic": round(a.steps * m_r / wh / 1e9, 3),
-                 "learned_over_heuristic": round(wh / wall, 3),
+                 "learned_over_heuristic": round(_ab(lambda: G.mbfs(srcs, out=out)[1],
+                                                     lambda: G.mbfs(srcs, out=out, policy=pol)[1])[2], 3),
                  "levels_identical_learned_vs_heuristic": heur_same,
                  "plain_64_bfs_gteps": round(plain_edges / tp / 1e9, 3),
                  "batched_over_plain": round((a.steps * m_r / wall) / (plain_edges / tp), 3)}
@@ -822,6 +848,8 @@ def run_workload(a, md, wl, torch, np):
         extra = {"supersteps": res[-1].supersteps, "push_steps": res[-1].push_steps,
                  "pull_steps": res[-1].pull_steps, "edges_reached": m_r,
                  "gteps_batched_heuristic": round(a.steps * m_r / wh / 1e9, 3),
+                 "learned_over_heuristic": round(_ab(lambda: G.bfs_multi(sources, out=out)[1],
+                                                     lambda: G.bfs_multi(sources, out=out, policy=pol)[1])[2], 3),
                  "gteps_plain_per_graph": round(plain_edges / tp / 1e9, 3),
                  "learned_batched_over_plain": round((a.steps * m_r / wall) / (plain_edges / tp), 2),
                  "levels_identical_to_plain": bool(same),

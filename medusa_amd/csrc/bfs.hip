@@ -634,7 +634,11 @@ __device__ __forceinline__ void bfs_clear_counters(BfsCounters* __restrict__ ctr
 // while the policy is evaluated, so it holds no registers there.
 __device__ __forceinline__ void bfs_decide(const BfsArgs& a) {
     __shared__ BfsCtl s_c;
+    __shared__ float s_params[kMlpParamsPadded];
     if (threadIdx.x >= kWave) return;
+    // with a policy, its parameters start for LDS now, behind the counter loads, holding no
+    // registers; a decision that does not consult it only waits for them at the end
+    if (a.policy) mlp_stage_wave_begin(a.policy, s_params);
     BfsCounters* ctr = a.ctr;
     const uint32_t lane = threadIdx.x;
     uint32_t par;
@@ -710,10 +714,11 @@ __device__ __forceinline__ void bfs_decide(const BfsArgs& a) {
         if (lane == 0) s_c = c;
     }
     if (ask) {  // uniform across the wave
-        __shared__ float s_params[kMlpParams];
         float c_push, c_pull;
-        policy_direction_costs_wave(rec, a.policy, s_params, c_push, c_pull);
+        policy_direction_costs_wave(rec, a.policy, s_params, c_push, c_pull, true);
         pull = policy_pull(c_push, c_pull, plain);
+    } else if (a.policy) {
+        mlp_stage_wave_wait();
     }
     bfs_clear_counters(ctr, par);  // this wave read them above
     if (lane == 0) {
@@ -764,7 +769,7 @@ __device__ __forceinline__ void bfs_first_decision(const BfsArgs& a, unsigned lo
                                     Sched::kAlphaBfs, Sched::kBeta)) {
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
-        __shared__ float s_params[kMlpParams];
+        __shared__ float s_params[kMlpParamsPadded];
         float c_push, c_pull;
         policy_direction_costs_wave(rec, a.policy, s_params, c_push, c_pull);
         pull = policy_pull(c_push, c_pull, (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);

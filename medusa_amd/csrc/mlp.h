@@ -113,9 +113,11 @@ __device__ inline void mlp_eval_rows(const float* __restrict__ P, float* sP, con
 
 // Parameters copied into LDS by one wave: every lane issues its 16-byte loads before its first
 // store (one memory round trip), and LDS keeps one wave's accesses in order, so the wave may
-// read sP right after (no barrier; other waves must not use sP).
+// read sP right after (no barrier; other waves must not use sP).  sP holds kMlpParamsPadded
+// floats (whole 16-byte vectors).
 constexpr int kMlpVec = (int)((kMlpParams + 3) / 4);           // 16-byte vectors of the parameters
 constexpr int kMlpVecPerLane = (kMlpVec + 63) / 64;
+constexpr int kMlpParamsPadded = 4 * kMlpVec;  // a policy's device copy is zero-padded to this
 __device__ inline void mlp_stage_wave(const float* __restrict__ P, float* sP) {
     const int lane = threadIdx.x & 63;
     const float4* P4 = reinterpret_cast<const float4*>(P);
@@ -135,6 +137,25 @@ __device__ inline void mlp_stage_wave(const float* __restrict__ P, float* sP) {
         const int v = lane + 64 * j;
         if (v < kMlpVec) reinterpret_cast<float4*>(sP)[v] = r[j];
     }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The same copy started by one wave with direct global-to-LDS loads, which hold no registers
+// while in flight (lane l of load j writes the 16 bytes at sP + 256 j + 4 l); the wave must call
+// mlp_stage_wave_wait() before it reads sP, and before it ends.  P is a policy's device copy
+// (kMlpParamsPadded floats, so every vector is whole).
+__device__ inline void mlp_stage_wave_begin(const float* __restrict__ P, float* sP) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < kMlpVecPerLane; ++j) {
+        const int v = lane + 64 * j;
+        if (v < kMlpVec)
+            __builtin_amdgcn_global_load_lds(P + 4 * v, (__attribute__((address_space(3))) void*)(sP + 256 * j), 16, 0,
+                                             0);
+    }
+}
+__device__ inline void mlp_stage_wave_wait() {
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0) expcnt(0) lgkmcnt(0): the LDS writes have landed
     __builtin_amdgcn_wave_barrier();
 }
 

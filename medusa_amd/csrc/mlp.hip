@@ -289,7 +289,9 @@ extern "C" int medusa_policy_create(uint32_t kind, uint64_t seed, medusa_policy*
         MDS_CHECK(ndev > 0, MEDUSA_ERR_HIP, "policy_create: no GPU visible");
         MDS_HIP(hipGetDevice(&p->device));
         MDS_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-        p->d_params.upload(p->params.data(), kMlpParams, p->stream);
+        std::vector<float> padded(kMlpParamsPadded, 0.f);  // whole 16-byte vectors for LDS staging
+        std::copy(p->params.begin(), p->params.end(), padded.begin());
+        p->d_params.upload(padded.data(), kMlpParamsPadded, p->stream);
         p->w.grad.alloc(kMlpParams);
         p->w.M.alloc(kMlpParams);
         p->w.V.alloc(kMlpParams);

@@ -106,19 +106,21 @@ struct MlpShared {
 };
 
 // Predicted costs of push and pull for the superstep described by `rec`, by one converged wave
-// with `sP` (kMlpParams floats of LDS) to itself: the parameters are staged while lanes 0-6
-// compute the logarithms, then lanes 0-31 evaluate the push row and lanes 32-63 the pull row;
-// the rows are the same as direction_row's.
+// with `sP` (kMlpParamsPadded floats of LDS) to itself: the parameters are staged (or, if
+// `staged`, were started by mlp_stage_wave_begin) while lanes 0-6 compute the logarithms, then
+// lanes 0-31 evaluate the push row and lanes 32-63 the pull row; the rows are the same as
+// direction_row's.
 __device__ inline void policy_direction_costs_wave(const double* rec, const float* __restrict__ policy, float* sP,
-                                                   float& c_push, float& c_pull) {
+                                                   float& c_push, float& c_pull, bool staged = false) {
     const int lane = threadIdx.x & 63;
-    mlp_stage_wave(policy, sP);
+    if (!staged) mlp_stage_wave(policy, sP);
     const float lg = lane < kDirectionLogs ? direction_log(rec, lane) : 0.f;
     float L[kDirectionLogs];
 #pragma unroll
     for (int i = 0; i < kDirectionLogs; ++i) L[i] = __shfl(lg, i, 64);
     float x[MEDUSA_POLICY_INPUTS];
     direction_fill(rec, L, (double)(lane >> 5), x);
+    if (staged) mlp_stage_wave_wait();
     mlp_eval_wave2(sP, x, c_push, c_pull);
 }
 
