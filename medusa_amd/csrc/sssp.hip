@@ -465,7 +465,12 @@ __global__ __launch_bounds__(256) void k_sssp_split(const uint32_t* __restrict__
 // kSsspBatch slots are captured once as a hipGraph and replayed until done.
 
 constexpr uint32_t kSsspBatch = 32;
-constexpr uint32_t kSsspGrid = 2048;
+// Workgroups of a work launch: about the resident count (5 per CU at this LDS size) without local
+// rounds, so every tile is in flight at once (RMAT-20 at width 250: 1.51 ms against 1.79 with
+// 2048; ER 2^20/2^24: 1.15 against 1.36); twice that with local rounds, whose short queues spread
+// better over more workgroups (2.4 M-vertex Chung-Lu: 0.74 against 0.77 ms with 1024).
+constexpr uint32_t kSsspGrid = 1024;
+constexpr uint32_t kSsspGridLocal = 2048;
 
 struct SsspCtl {
     uint32_t done, epoch, thr, old_thr, delta, n_near, n_far, split_pending;
@@ -862,7 +867,8 @@ static void sssp_run_device(medusa_graph* g, uint32_t src, uint32_t* out, const 
         gc.reset();
         MDS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         for (uint32_t k = 0; k < kSsspBatch; ++k) {
-            hipLaunchKernelGGL(local_cfg ? k_sssp_work<true> : k_sssp_work<false>, dim3(kSsspGrid), dim3(256), 0, s,
+            hipLaunchKernelGGL(local_cfg ? k_sssp_work<true> : k_sssp_work<false>,
+                               dim3(local_cfg ? kSsspGridLocal : kSsspGrid), dim3(256), 0, s,
                                ctl, near0, near1, (uint32_t)near_cap, far0, far1, (uint32_t)far_cap,
                                g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p, ctr, marks);
         }
@@ -1005,7 +1011,8 @@ static void sssp_run_host(medusa_graph* g, uint32_t src, uint32_t* out, const me
         if (timing) MDS_HIP(hipEventRecord(g->events[0], s));  // brackets the relaxation launch alone
         // 256 entries per workgroup chunk; the far queue keeps growing at far_a[n_far..)
         const uint32_t rounds = n_near <= kLocalQueue ? local_cfg : 0u;
-        hipLaunchKernelGGL(rounds ? k_sssp_relax<true> : k_sssp_relax<false>, dim3(grid_for(n_near, 1, kSsspGrid)),
+        hipLaunchKernelGGL(rounds ? k_sssp_relax<true> : k_sssp_relax<false>,
+                           dim3(grid_for(n_near, 1, local_cfg ? kSsspGridLocal : kSsspGrid)),
                            dim3(256), 0, s, near_cur, n_near, g->row_ptr.p, g->col.p, g->w.p, g->dist.p, g->stamp.p,
                            epoch, thr, near_nxt, (uint32_t)near_cap, far_a, (uint32_t)far_cap, C, rounds);
         MDS_LAUNCHED();
