@@ -78,6 +78,28 @@ def test_learned_direction_bfs(oracle, direction_policy, which):
         assert c_learn <= c_heur
 
 
+def test_learned_direction_rmat20_gain(oracle, direction_policy):
+    """Measured gain of the learned direction on the benchmark's RMAT-20 graph (seed 2, from the
+    largest out-degree vertex), where it pulls one superstep more than the heuristic: at least
+    1.1x by the median device time of 9 interleaved pairs of runs, with the levels of both equal
+    to the oracle's.  The policy is trained on other graphs (RMAT 14-17, ER, Chung-Lu)."""
+    rp, col = wl.rmat_graph()
+    src = wl.max_outdeg_vertex(rp)
+    want = oracle.bfs_queue(rp, col, src)
+    with md.Graph(rp, col) as G:
+        lv_h, st_h = G.bfs(src)
+        lv_l, st_l = G.bfs(src, policy=direction_policy)
+        assert np.array_equal(lv_h, want) and np.array_equal(lv_l, want)
+        th, tl = [], []
+        for _ in range(9):
+            th.append(G.bfs(src)[1].device_ms)
+            tl.append(G.bfs(src, policy=direction_policy)[1].device_ms)
+    t_h, t_l = float(np.median(th)), float(np.median(tl))
+    print(f"[bfs rmat20] heuristic {t_h:.4f} ms ({st_h.push_steps} push, {st_h.pull_steps} pull), learned "
+          f"{t_l:.4f} ms ({st_l.push_steps} push, {st_l.pull_steps} pull); gain {t_h / t_l:.2f}x", flush=True)
+    assert t_h >= 1.1 * t_l
+
+
 def test_learned_delta_sssp(oracle, delta_policy):
     """The learned bucket width on a held-out 700x700 lattice: distances equal Dijkstra's on both
     drivers.  Parity only: on lattices the engine's default width (8x the mean weight with local
