@@ -288,16 +288,19 @@ AB_REPS = 15
 
 
 def _ab(fa, fb, reps=AB_REPS):
-    """A/B ratio of two forms of one run (each a call returning its Stats): one run of each,
-    alternately, `reps` times after one of each as warm-up; returns (median device ms of A,
-    median device ms of B, A / B).  Alternating cancels clock and thermal drift, which
+    """A/B ratio of two forms of one run (each a call returning its Stats): `reps` rounds of two
+    runs of A then two runs of B, timing the second of each pair; returns (median device ms of
+    A, median device ms of B, A / B).  Alternating cancels clock and thermal drift, which
     sequential timing of one form after the other (with policy training between them) does
-    not; device_ms is the library's own event span of the whole run on its stream."""
-    fa()
-    fb()
+    not.  The first run of a pair is not timed because a handle sizes a run's first replay by
+    the slots its previous run used: right after the other form, a run that needs more slots
+    pays a second replay and host round trip.  device_ms is the library's own event span of the
+    whole run on its stream."""
     ta, tb = [], []
     for _ in range(reps):
+        fa()
         ta.append(fa().device_ms)
+        fb()
         tb.append(fb().device_ms)
     ma, mb = statistics.median(ta), statistics.median(tb)
     return ma, mb, ma / mb

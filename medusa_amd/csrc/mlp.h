@@ -17,9 +17,9 @@ constexpr float kMlpSlope = 1.0f / 64.0f;
 // be cheaper by more than a margin (log2 units of cost); otherwise the plain engine's choice
 // stands, so an uncertain model falls back to it.  Direction and kernel variant: 0.25 (about
 // 19 %; a 2x margin kept every BFS decision the heuristic's, while at 0.25 the direction policy
-// pulls one more superstep on RMAT-20 and six more on the 2.4 M-vertex Chung-Lu graph, 1.2x and
-// 1.1x faster); bucket width: 0.1 (7 %; the width model's fit error on its training runs is
-// about 0.06, and the best width on RMAT-20 beats the default by 12-41 %).  The bucket width is chosen once per run, from the features at its first phase,
+// pulls on about six more supersteps of the 2.4 M-vertex Chung-Lu graph, 1.07x faster); bucket
+// width: 0.1 (7 %; the width model's fit error on its training runs is about 0.06, and the best
+// width on RMAT-20 beats the default by 12-41 %).  The bucket width is chosen once per run, from the features at its first phase,
 // whose training rows carry the cost of whole runs at each fixed width; switching widths at
 // later phases on the network's predictions lost 5-22 % on 1400x1400 lattices (its training
 // runs never switch).
