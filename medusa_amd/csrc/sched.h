@@ -70,8 +70,8 @@ __host__ __device__ inline void delta_row(const double* r, double action, float*
 }
 
 // Kernel-variant rows (PageRank record = step, dangling, wave-run vertices, lane-run vertices,
-// 0, n, m, ...): size, mean degree, the shares of dangling and high in-degree vertices, step,
-// and the candidate variant (push, binned or pull) as indicators with their interactions.
+// 0, n, m, ...): size, mean degree, the shares of dangling and high in-degree vertices, and
+// the candidate variant (push, binned or pull) as indicators with their interactions.
 __host__ __device__ inline void variant_row(const double* r, double action, float* x) {
     for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) x[i] = 0.f;
     const double n = r[5] > 1.0 ? r[5] : 1.0, m = r[6] > 1.0 ? r[6] : 1.0;
@@ -82,7 +82,9 @@ __host__ __device__ inline void variant_row(const double* r, double action, floa
     x[2] = (float)(r[1] / n);
     x[3] = (float)(r[2] / n);
     x[4] = (float)(r[3] / n);
-    x[5] = (float)(r[0] < 64.0 ? r[0] : 64.0) / 64.f;
+    // x[5] stays 0: PageRank supersteps repeat the same work, so the row carries no step
+    // index; one (training runs cover 6 supersteps) let the policy switch variants mid-run
+    // outside that range, 2.5x slower on a held-out Chung-Lu graph
     x[6] = push;
     x[7] = push * x[0];
     x[8] = push * x[1];

@@ -228,8 +228,9 @@ def test_learned_pagerank_variant(oracle, variant_data, variant_policy):
     graphs.  The plain engine runs one fixed variant (the binned exchange) everywhere; on small
     and dense graphs its two launches cost more than the pull kernel, and on large sparse ones
     less.  On held-out graphs the learned choice keeps the ranks' bits and never costs more than
-    the fixed variant (per-superstep costs measured on that graph), and on RMAT-16 it runs a
-    30-superstep PageRank at least 1.1x faster, timed on the GPU.  A policy trained on costs
+    the fixed variant (per-superstep costs measured on that graph over 12 supersteps, and a timed
+    30-superstep run within 5 %), and on RMAT-16 it runs a 30-superstep PageRank at least 1.1x
+    faster, timed on the GPU.  A policy trained on costs
     that make push cheapest really switches to push."""
     X, y = variant_data
     pol = variant_policy
@@ -258,5 +259,6 @@ def test_learned_pagerank_variant(oracle, variant_data, variant_policy):
                   f"30-superstep run: fixed {t_plain:.3f} ms, learned {t_learn:.3f} ms "
                   f"({t_plain / t_learn:.2f}x); variants {np.bincount(acts, minlength=4)[1:].tolist()}", flush=True)
             assert c_learn <= c_plain, name
+            assert t_learn <= 1.05 * t_plain, (name, t_plain, t_learn)  # the whole 30-superstep run
             if name == "rmat16":
                 assert t_plain / t_learn >= 1.1, (t_plain, t_learn)
