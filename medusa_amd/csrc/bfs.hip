@@ -634,7 +634,7 @@ __device__ __forceinline__ void bfs_clear_counters(BfsCounters* __restrict__ ctr
 // while the policy is evaluated, so it holds no registers there.
 __device__ __forceinline__ void bfs_decide(const BfsArgs& a) {
     __shared__ BfsCtl s_c;
-    __shared__ float s_params[kMlpParamsPadded];
+    __shared__ float s_params[kMlpWaveLds];
     if (threadIdx.x >= kWave) return;
     // with a policy, its parameters start for LDS now, behind the counter loads, holding no
     // registers; a decision that does not consult it only waits for them at the end
@@ -769,7 +769,7 @@ __device__ __forceinline__ void bfs_first_decision(const BfsArgs& a, unsigned lo
                                     Sched::kAlphaBfs, Sched::kBeta)) {
         const double rec[MEDUSA_TRACE_FIELDS] = {0.0, (double)c.n_f, (double)c.m_f, (double)c.n_u, (double)c.m_u,
                                                  (double)c.n, (double)c.m, 0.0, 0.0};
-        __shared__ float s_params[kMlpParamsPadded];
+        __shared__ float s_params[kMlpWaveLds];
         float c_push, c_pull;
         policy_direction_costs_wave(rec, a.policy, s_params, c_push, c_pull);
         pull = policy_pull(c_push, c_pull, (double)c.m_f > (double)c.m_u / Sched::kAlphaBfs);

@@ -159,31 +159,43 @@ __device__ inline void mlp_stage_wave_wait() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// One wave evaluates the MLP on two input rows with the parameters in LDS (mlp_stage_wave):
-// lanes 0-31 pass row 0 in x, lanes 32-63 row 1, and lane 32 * r + i computes hidden unit i of
-// row r; activations move by shuffles, so there is no barrier.  The FP32 operations and their
-// order per row are those of mlp_eval_host.  Every lane gets both outputs; the calling wave
-// must be converged.
-__device__ inline void mlp_eval_wave2(const float* sP, const float* x, float& out0, float& out1) {
+// LDS of a one-wave evaluation: the parameters (mlp_stage_wave), then 64 floats through which
+// the activations cross the wave.
+constexpr int kMlpWaveLds = kMlpParamsPadded + 2 * kMlpH;
+
+// One wave evaluates the MLP on two input rows with the parameters in sP (kMlpWaveLds floats of
+// LDS, staged by mlp_stage_wave): lanes 0-31 pass row 0 in x, lanes 32-63 row 1, and lane
+// 32 * r + i computes hidden unit i of row r.  A layer's activations are written to LDS and
+// every lane reads its row's 32 (the same addresses across a half-wave: broadcasts); one wave's
+// LDS accesses complete in order, so there is no barrier.  The FP32 operations and their order
+// per row are those of mlp_eval_host.  Every lane gets both outputs; the calling wave must be
+// converged.
+__device__ inline void mlp_eval_wave2(float* sP, const float* x, float& out0, float& out1) {
     const int lane = threadIdx.x & 63, r = lane >> 5, i = lane & 31;
     const float* W1 = sP;
     const float* b1 = W1 + kMlpH * kMlpIn;
     const float* W2 = b1 + kMlpH;
     const float* b2 = W2 + kMlpH * kMlpH;
     const float* W3 = b2 + kMlpH;
+    float* act = sP + kMlpParamsPadded;
+    const float* ar = act + r * kMlpH;
     float z = b1[i];
 #pragma unroll
     for (int k = 0; k < kMlpIn; ++k) z = fmaf(W1[i * kMlpIn + k], x[k], z);
-    const float a1 = z > 0.f ? z : z * kMlpSlope;
+    act[lane] = z > 0.f ? z : z * kMlpSlope;
+    __builtin_amdgcn_wave_barrier();
     float z2 = b2[i];
 #pragma unroll 8
-    for (int k = 0; k < kMlpH; ++k) z2 = fmaf(W2[i * kMlpH + k], __shfl(a1, r * kMlpH + k, 64), z2);
+    for (int k = 0; k < kMlpH; ++k) z2 = fmaf(W2[i * kMlpH + k], ar[k], z2);
     const float a2 = z2 > 0.f ? z2 : z2 * kMlpSlope;
+    __builtin_amdgcn_wave_barrier();
+    act[lane] = a2;  // after every lane's reads of the first layer (in order in LDS)
+    __builtin_amdgcn_wave_barrier();
     float o = W3[kMlpH];
 #pragma unroll 8
-    for (int k = 0; k < kMlpH; ++k) o = fmaf(W3[k], __shfl(a2, r * kMlpH + k, 64), o);
-    out0 = __shfl(o, 0, 64);
-    out1 = __shfl(o, kMlpH, 64);
+    for (int k = 0; k < kMlpH; ++k) o = fmaf(W3[k], ar[k], o);
+    out0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, o), 0));
+    out1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, o), kMlpH));
 }
 }  // namespace medusa
 #endif

@@ -108,7 +108,7 @@ struct MlpShared {
 };
 
 // Predicted costs of push and pull for the superstep described by `rec`, by one converged wave
-// with `sP` (kMlpParamsPadded floats of LDS) to itself: the parameters are staged (or, if
+// with `sP` (kMlpWaveLds floats of LDS) to itself: the parameters are staged (or, if
 // `staged`, were started by mlp_stage_wave_begin) while lanes 0-6 compute the logarithms, then
 // lanes 0-31 evaluate the push row and lanes 32-63 the pull row; the rows are the same as
 // direction_row's.
