@@ -107,19 +107,29 @@ struct MlpShared {
     float out[12];
 };
 
+// direction_log in FP32 with the hardware logarithm: within about 2e-6 of it, far below any
+// decision margin, and a few instructions instead of an FP64 logarithm's long dependent chain.
+__device__ inline float direction_log_f32(const double* r, int i) {
+    const float n = fmaxf((float)r[5], 1.f), m = fmaxf((float)r[6], 1.f);
+    auto lg = [](float v) { return __log2f(1.f + fmaxf(v, 0.f)); };
+    if (i == 0) return lg(n);
+    if (i == 1) return lg(m);
+    if (i == 6) return lg(m / n);
+    return lg((float)r[i - 1]);  // i = 2..5: n_f, m_f, n_u, m_u
+}
+
 // Predicted costs of push and pull for the superstep described by `rec`, by one converged wave
-// with `sP` (kMlpWaveLds floats of LDS) to itself: the parameters are staged (or, if
-// `staged`, were started by mlp_stage_wave_begin) while lanes 0-6 compute the logarithms, then
-// lanes 0-31 evaluate the push row and lanes 32-63 the pull row; the rows are the same as
-// direction_row's.
+// with `sP` (kMlpWaveLds floats of LDS) to itself: the parameters are staged (or, if `staged`,
+// were started by mlp_stage_wave_begin) while every lane computes the seven logarithms in
+// FP32, then lanes 0-31 evaluate the push row and lanes 32-63 the pull row (direction_row's
+// rows, up to the logarithms' rounding).
 __device__ inline void policy_direction_costs_wave(const double* rec, const float* __restrict__ policy, float* sP,
                                                    float& c_push, float& c_pull, bool staged = false) {
     const int lane = threadIdx.x & 63;
     if (!staged) mlp_stage_wave(policy, sP);
-    const float lg = lane < kDirectionLogs ? direction_log(rec, lane) : 0.f;
     float L[kDirectionLogs];
 #pragma unroll
-    for (int i = 0; i < kDirectionLogs; ++i) L[i] = __shfl(lg, i, 64);
+    for (int i = 0; i < kDirectionLogs; ++i) L[i] = direction_log_f32(rec, i);
     float x[MEDUSA_POLICY_INPUTS];
     direction_fill(rec, L, (double)(lane >> 5), x);
     if (staged) mlp_stage_wave_wait();
