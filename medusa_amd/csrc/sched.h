@@ -138,15 +138,15 @@ __device__ inline void policy_direction_costs_wave(const double* rec, const floa
 
 // Predicted costs of push (out[0]) and pull (out[1]) for the superstep described by `rec`.
 // Every thread of the block calls it; `staged`: the kernel already ran mlp_stage into m.P.
-// The logarithms are computed by seven threads at once (in double, a long dependent chain
-// each); the rows are the same as direction_row's.
+// Threads 0 and 1 build the push and pull rows with FP32 logarithms (direction_row's rows, up
+// to their rounding); mlp_eval_rows' first barrier publishes them.
 __device__ inline void policy_direction_costs(const double* rec, const float* __restrict__ policy, MlpShared& m,
                                               bool staged = false) {
-    float* L = m.out;  // scratch until the network writes its outputs
-    if (threadIdx.x < kDirectionLogs) L[threadIdx.x] = direction_log(rec, threadIdx.x);
-    __syncthreads();
     float x[MEDUSA_POLICY_INPUTS];
     if (threadIdx.x < 2) {
+        float L[kDirectionLogs];  // in FP32, as the one-wave form does
+#pragma unroll
+        for (int i = 0; i < kDirectionLogs; ++i) L[i] = direction_log_f32(rec, i);
         direction_fill(rec, L, (double)threadIdx.x, x);
         for (int i = 0; i < MEDUSA_POLICY_INPUTS; ++i) m.x[threadIdx.x * MEDUSA_POLICY_INPUTS + i] = x[i];
     }
