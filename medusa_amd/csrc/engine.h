@@ -314,6 +314,9 @@ void build_pr_layout(medusa_graph* g);
 // Plans and uploads the binned PageRank layout (once per layout).  Returns false when the graph
 // cannot be planned; with `required` that is a MEDUSA_ERR_STATE error instead.
 bool build_pr_bins(medusa_graph* g, bool required);
+// Send-tile workgroups of the binned PageRank that run at once on g's device, after the ones
+// that sum `nblk` dangling blocks in the same launch.
+uint32_t pr_send_tile_slots(const medusa_graph* g, uint32_t nblk);
 void build_pull_layout(medusa_graph* g);
 
 constexpr size_t kCounterWords = 256;  // 2 KiB

@@ -264,7 +264,8 @@ bool build_pr_bins(medusa_graph* g, bool required) {
     if (L.bins_refused && !required) return false;
     PrBinsPlan P;
     const std::string err = plan_pr_bins(g->n, L.h_cptr.data(), L.h_crow.data(), g->parts.data(),
-                                         (uint32_t)g->parts.size() - 1, P);
+                                         (uint32_t)g->parts.size() - 1, P,
+                                         pr_send_tile_slots(g, (uint32_t)L.blocks.size()));
     if (!err.empty()) {
         L.bins_refused = true;
         MDS_CHECK(!required, MEDUSA_ERR_STATE, "%s", err.c_str());

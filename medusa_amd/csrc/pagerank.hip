@@ -463,6 +463,16 @@ __global__ __launch_bounds__(kSendThreads) void k_pr_send(const PrTile* __restri
     }
 }
 
+uint32_t pr_send_tile_slots(const medusa_graph* g, uint32_t nblk) {
+    int cus = 0, per_cu = 0;
+    MDS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    MDS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_pr_send),
+                                                         (int)kSendThreads, 0));
+    const uint64_t slots = (uint64_t)std::max(cus, 0) * (uint64_t)std::max(per_cu, 0);
+    const uint64_t dwg = (nblk + kSendWaves - 1) / kSendWaves;
+    return slots > dwg ? (uint32_t)std::min<uint64_t>(slots - dwg, 1u << 20) : 0u;
+}
+
 constexpr uint32_t kCombineThreads = 512;
 constexpr uint32_t kCombineWaves = kCombineThreads / kWave;
 constexpr uint32_t kCombineUnroll = 4;  // groups of four entries per thread in flight

@@ -37,7 +37,7 @@ constexpr uint32_t kPrBinMaxVertices = 8192;  // destinations of one bin
 constexpr uint32_t kPrBinMaxItems = 512;      // items of one bin (8 waves hold 64 records each)
 constexpr uint32_t kPrBinRuns = 2048;         // runs (chunks sending to it) one bin may have
 constexpr uint32_t kPrChunkSources = 8192;    // sources of one send chunk (their contributions fit LDS)
-constexpr uint32_t kPrTileEdges = 16384;      // message entries written by one send workgroup
+constexpr uint32_t kPrTileEdges = 16384;      // message entries written by one send workgroup (at least)
 constexpr uint32_t kPrHubDeg = 4096;          // in-degree above which a destination is summed by gather
 constexpr uint32_t kPrRunAlign = 4;           // runs start and end on multiples of this many entries
 constexpr uint16_t kPrDummySlot = (uint16_t)kPrBinValues;  // LDS slot of padding entries
@@ -90,15 +90,17 @@ struct PrBinsPlan {
                                    // position / kPrRunAlign (a group never straddles two runs)
     uint32_t nhubs = 0;
     uint32_t max_runs = 0;  // most runs of any bin
+    uint32_t tile_edges = kPrTileEdges;  // message entries per send tile (a chunk's last tile may hold fewer)
     uint64_t nmsg = 0;      // messages (in-edges of non-hub destinations)
     uint64_t nentry = 0;    // message-array entries, padding included
 };
 
 // Plans the binned superstep for a graph in the PageRank internal order: cptr[n+1] in-edge
 // offsets, crow[m] in-edge sources (internal ids, canonical order per destination),
-// part_begin[nparts+1] packed-graph ranges.  Returns "" or the reason the plan cannot be made.
+// part_begin[nparts+1] packed-graph ranges.  send_slots = send workgroups that run at once on
+// the device (0: unknown).  Returns "" or the reason the plan cannot be made.
 inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t* crow, const uint32_t* part_begin,
-                                uint32_t nparts, PrBinsPlan& P) {
+                                uint32_t nparts, PrBinsPlan& P, uint32_t send_slots = 0) {
     P = PrBinsPlan{};
     const uint32_t kNone = 0xFFFFFFFFu;
     std::vector<uint32_t> bin_of(n, kNone), item_of(n, kNone);
@@ -231,11 +233,24 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
                 P.qmsg[(Bn.slot_begin + k) / kPrRunAlign] = (R.msg + (k - R.k)) / kPrRunAlign;
         }
     }
+    // Tiles of kPrTileEdges entries, or more when the tiles would not all run at once: a tile
+    // count a little above the device's slots leaves a second round of a few tiles that runs on
+    // an otherwise idle device (RMAT-20: 1112 tiles for 1024 slots).  With T entries per tile the
+    // chunks give at most nentry / T + (chunks that send) tiles, so T = nentry / (slots - those
+    // chunks) fits one round.  Tiles never get smaller: each recomputes its chunk's contributions.
+    uint64_t sending = 0;
+    for (uint64_t c = 0; c < C; ++c) sending += chunk_start[c + 1] > chunk_start[c];
+    if (send_slots > sending) {
+        const uint64_t even = (P.nentry + (send_slots - sending) - 1) / (send_slots - sending);
+        const uint64_t T = (even + kPrRunAlign - 1) / kPrRunAlign * kPrRunAlign;
+        if (T > kPrTileEdges) P.tile_edges = (uint32_t)std::min<uint64_t>(T, 1ull << 30);
+    }
     for (uint64_t c = 0; c < C; ++c) {
         const uint32_t src = (uint32_t)(c * kPrChunkSources);
         const uint32_t cnt_src = std::min<uint32_t>(kPrChunkSources, n - src);
-        for (uint32_t e = chunk_start[c]; e < chunk_start[c + 1]; e += kPrTileEdges)
-            P.tiles.push_back({e, std::min<uint32_t>(e + kPrTileEdges, chunk_start[c + 1]), src, cnt_src});
+        for (uint64_t e = chunk_start[c]; e < chunk_start[c + 1]; e += P.tile_edges)
+            P.tiles.push_back({(uint32_t)e, (uint32_t)std::min<uint64_t>(e + P.tile_edges, chunk_start[c + 1]), src,
+                               cnt_src});
     }
     P.nhubs = (uint32_t)hubs.size();
     P.items.insert(P.items.end(), hubs.begin(), hubs.end());
@@ -297,8 +312,9 @@ inline std::string check_pr_bins(uint32_t n, const uint32_t* cptr, const PrBinsP
         if (verts > kPrBinMaxVertices) return "bins: vertices per bin";
     }
     if (real != P.nmsg) return "bins: message count";
+    if (P.tile_edges < kPrTileEdges || P.tile_edges % kPrRunAlign) return "bins: tile size";
     for (const PrTile& T : P.tiles) {
-        if (T.msg_begin > T.msg_end || T.msg_end > P.nentry || T.msg_end - T.msg_begin > kPrTileEdges)
+        if (T.msg_begin > T.msg_end || T.msg_end > P.nentry || T.msg_end - T.msg_begin > P.tile_edges)
             return "bins: tile entries";
         if (T.msg_begin % kPrRunAlign || T.msg_end % kPrRunAlign) return "bins: tile alignment";
         if (T.src_count == 0 || T.src_count > kPrChunkSources || (uint64_t)T.src_begin + T.src_count > n ||
