@@ -233,17 +233,23 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
                 P.qmsg[(Bn.slot_begin + k) / kPrRunAlign] = (R.msg + (k - R.k)) / kPrRunAlign;
         }
     }
-    // Tiles of kPrTileEdges entries, or more when the tiles would not all run at once: a tile
-    // count a little above the device's slots leaves a second round of a few tiles that runs on
-    // an otherwise idle device (RMAT-20: 1112 tiles for 1024 slots).  With T entries per tile the
-    // chunks give at most nentry / T + (chunks that send) tiles, so T = nentry / (slots - those
-    // chunks) fits one round.  Tiles never get smaller: each recomputes its chunk's contributions.
-    uint64_t sending = 0;
-    for (uint64_t c = 0; c < C; ++c) sending += chunk_start[c + 1] > chunk_start[c];
-    if (send_slots > sending) {
+    // Tiles of kPrTileEdges entries, or up to twice that when the tiles would overflow one round
+    // of the device's resident send workgroups: a tile count a little above the slots leaves a
+    // second round of a few tiles that runs on an otherwise idle device (RMAT-20: 1112 tiles for
+    // 1024 slots).  With T entries per tile the chunks give at most nentry / T + (chunks that
+    // send) tiles, so T = nentry / (slots - those chunks) fits one round.  A larger T is not
+    // taken: tiles of many small chunks would then differ widely in length, the longest alone at
+    // the end (a batch of 128 graphs, 789 chunks for 1023 slots, ran 17 % slower that way).
+    // Tiles never get smaller: each recomputes its chunk's contributions.
+    uint64_t sending = 0, fixed_tiles = 0;
+    for (uint64_t c = 0; c < C; ++c) {
+        sending += chunk_start[c + 1] > chunk_start[c];
+        fixed_tiles += (chunk_start[c + 1] - chunk_start[c] + kPrTileEdges - 1) / kPrTileEdges;
+    }
+    if (send_slots > sending && fixed_tiles > send_slots) {
         const uint64_t even = (P.nentry + (send_slots - sending) - 1) / (send_slots - sending);
         const uint64_t T = (even + kPrRunAlign - 1) / kPrRunAlign * kPrRunAlign;
-        if (T > kPrTileEdges) P.tile_edges = (uint32_t)std::min<uint64_t>(T, 1ull << 30);
+        if (T > kPrTileEdges && T <= 2ull * kPrTileEdges) P.tile_edges = (uint32_t)T;
     }
     for (uint64_t c = 0; c < C; ++c) {
         const uint32_t src = (uint32_t)(c * kPrChunkSources);
