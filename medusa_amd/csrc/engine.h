@@ -317,6 +317,8 @@ bool build_pr_bins(medusa_graph* g, bool required);
 // Send-tile workgroups of the binned PageRank that run at once on g's device, after the ones
 // that sum `nblk` dangling blocks in the same launch.
 uint32_t pr_send_tile_slots(const medusa_graph* g, uint32_t nblk);
+// Binned-combine workgroups that run at once on g's device.
+uint32_t pr_combine_slots(const medusa_graph* g);
 void build_pull_layout(medusa_graph* g);
 
 constexpr size_t kCounterWords = 256;  // 2 KiB

@@ -265,7 +265,7 @@ bool build_pr_bins(medusa_graph* g, bool required) {
     PrBinsPlan P;
     const std::string err = plan_pr_bins(g->n, L.h_cptr.data(), L.h_crow.data(), g->parts.data(),
                                          (uint32_t)g->parts.size() - 1, P,
-                                         pr_send_tile_slots(g, (uint32_t)L.blocks.size()));
+                                         pr_send_tile_slots(g, (uint32_t)L.blocks.size()), pr_combine_slots(g));
     if (!err.empty()) {
         L.bins_refused = true;
         MDS_CHECK(!required, MEDUSA_ERR_STATE, "%s", err.c_str());

@@ -627,6 +627,14 @@ __global__ __launch_bounds__(kCombineThreads) void k_pr_combine(
     }
 }
 
+uint32_t pr_combine_slots(const medusa_graph* g) {
+    int cus = 0, per_cu = 0;
+    MDS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    MDS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_pr_combine<true, false>), (int)kCombineThreads, 0));
+    return (uint32_t)std::min<uint64_t>((uint64_t)std::max(cus, 0) * (uint64_t)std::max(per_cu, 0), 1u << 20);
+}
+
 // r_0 = (float)(1 / n_g) for every vertex of graph g (graph found by binary search
 // over the range starts, which internal ids keep).
 __global__ void k_pr_init(uint32_t n, const uint32_t* __restrict__ part_begin, uint32_t nparts,

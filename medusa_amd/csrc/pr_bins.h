@@ -97,10 +97,12 @@ struct PrBinsPlan {
 
 // Plans the binned superstep for a graph in the PageRank internal order: cptr[n+1] in-edge
 // offsets, crow[m] in-edge sources (internal ids, canonical order per destination),
-// part_begin[nparts+1] packed-graph ranges.  send_slots = send workgroups that run at once on
-// the device (0: unknown).  Returns "" or the reason the plan cannot be made.
+// part_begin[nparts+1] packed-graph ranges.  send_slots / combine_slots = send / combine
+// workgroups that run at once on the device (0: unknown).  Returns "" or the reason the plan
+// cannot be made.
 inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t* crow, const uint32_t* part_begin,
-                                uint32_t nparts, PrBinsPlan& P, uint32_t send_slots = 0) {
+                                uint32_t nparts, PrBinsPlan& P, uint32_t send_slots = 0,
+                                uint32_t combine_slots = 0) {
     P = PrBinsPlan{};
     const uint32_t kNone = 0xFFFFFFFFu;
     std::vector<uint32_t> bin_of(n, kNone), item_of(n, kNone);
@@ -108,6 +110,7 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
     uint32_t item_begin = 0, used = 0, verts = 0, part = 0;
     int64_t group = -1;  // open lane / zero group
     uint32_t group_max = 0;
+    uint32_t cap_values = kPrBinValues, cap_verts = kPrBinMaxVertices;  // of the bin being filled
     auto close_bin = [&] {
         if (P.items.size() > item_begin) {
             P.bins.push_back({item_begin, (uint32_t)P.items.size(), 0, 0, 0, 0, {0, 0}});
@@ -119,46 +122,74 @@ inline std::string plan_pr_bins(uint32_t n, const uint32_t* cptr, const uint32_t
         group = -1;
     };
     auto full = [&](uint32_t foot) {
-        return used + foot > kPrBinValues || verts + 1 > kPrBinMaxVertices ||
-               P.items.size() - item_begin >= kPrBinMaxItems;
+        return used + foot > cap_values || verts + 1 > cap_verts || P.items.size() - item_begin >= kPrBinMaxItems;
     };
-    for (uint32_t p = 0; p < n; ++p) {
-        while (part + 1 < nparts && p >= part_begin[part + 1]) {
-            ++part;
-            group = -1;
-        }
-        const uint32_t deg = cptr[p + 1] - cptr[p];
-        if (deg > kPrHubDeg) {
-            hubs.push_back({p, 1, kPrWave, 0, part, deg, {0, 0}});
-            group = -1;
-            continue;
-        }
-        const uint32_t kind = deg == 0 ? kPrZero : deg <= kPrLaneMax ? kPrLane : kPrWave;
-        if (kind == kPrWave) {
-            if (full(deg)) close_bin();
-            item_of[p] = (uint32_t)P.items.size();
-            P.items.push_back({p, 1, kPrWave, used, part, deg, {0, 0}});
-            used += deg;
-            group = -1;
-        } else {
-            const bool extend = group >= 0 && P.items[group].kind == kind && P.items[group].count < 64 &&
-                                P.items[group].v + P.items[group].count == p && deg <= group_max &&
-                                verts + 1 <= kPrBinMaxVertices;
-            if (!extend) {
-                const uint32_t foot = kind == kPrLane ? 64u * deg : 0u;
-                if (full(foot)) close_bin();
-                group = (int64_t)P.items.size();
-                P.items.push_back({p, 0, kind, used, part, deg, {0, 0}});
-                group_max = deg;
-                used += foot;
+    // Items and bins in vertex order; bins from vertex tail_from on hold a quarter of the slots
+    // and vertices (an item larger than that gets a bin of its own, within kPrBinValues).
+    auto plan_items = [&](uint32_t tail_from) {
+        P.items.clear();
+        P.bins.clear();
+        P.bin_slots.clear();
+        hubs.clear();
+        std::fill(bin_of.begin(), bin_of.end(), kNone);
+        std::fill(item_of.begin(), item_of.end(), kNone);
+        item_begin = used = verts = part = 0;
+        group = -1;
+        group_max = 0;
+        cap_values = kPrBinValues;
+        cap_verts = kPrBinMaxVertices;
+        for (uint32_t p = 0; p < n; ++p) {
+            if (p == tail_from) {
+                close_bin();
+                cap_values = kPrBinValues / 4;
+                cap_verts = kPrBinMaxVertices / 4;
             }
-            ++P.items[group].count;
-            item_of[p] = (uint32_t)group;
+            while (part + 1 < nparts && p >= part_begin[part + 1]) {
+                ++part;
+                group = -1;
+            }
+            const uint32_t deg = cptr[p + 1] - cptr[p];
+            if (deg > kPrHubDeg) {
+                hubs.push_back({p, 1, kPrWave, 0, part, deg, {0, 0}});
+                group = -1;
+                continue;
+            }
+            const uint32_t kind = deg == 0 ? kPrZero : deg <= kPrLaneMax ? kPrLane : kPrWave;
+            if (kind == kPrWave) {
+                if (full(deg)) close_bin();
+                item_of[p] = (uint32_t)P.items.size();
+                P.items.push_back({p, 1, kPrWave, used, part, deg, {0, 0}});
+                used += deg;
+                group = -1;
+            } else {
+                const bool extend = group >= 0 && P.items[group].kind == kind && P.items[group].count < 64 &&
+                                    P.items[group].v + P.items[group].count == p && deg <= group_max &&
+                                    verts + 1 <= cap_verts;
+                if (!extend) {
+                    const uint32_t foot = kind == kPrLane ? 64u * deg : 0u;
+                    if (full(foot)) close_bin();
+                    group = (int64_t)P.items.size();
+                    P.items.push_back({p, 0, kind, used, part, deg, {0, 0}});
+                    group_max = deg;
+                    used += foot;
+                }
+                ++P.items[group].count;
+                item_of[p] = (uint32_t)group;
+            }
+            ++verts;
+            bin_of[p] = (uint32_t)P.bins.size();
         }
-        ++verts;
-        bin_of[p] = (uint32_t)P.bins.size();
+        close_bin();
+    };
+    // The combine runs its bins in rounds of combine_slots workgroups.  When the last round is
+    // short (at most a quarter of the slots: RMAT-20, 2231 bins for 1024), its bins would run
+    // nearly alone at the end; the vertices of those bins are planned again into quarter-size
+    // bins, which spread that work over the workgroups as they free up.
+    plan_items(n);
+    if (combine_slots && P.bins.size() > combine_slots) {
+        const uint64_t r = P.bins.size() % combine_slots;
+        if (r && r <= combine_slots / 4) plan_items(P.items[P.bins[P.bins.size() - r].item_begin].v);
     }
-    close_bin();
     const uint64_t B = P.bins.size();
     const uint64_t C = ((uint64_t)n + kPrChunkSources - 1) / kPrChunkSources;
     if (C * B > (1ull << 27)) return "binned PageRank: too many chunk x bin pairs for this graph";
