@@ -86,6 +86,33 @@ def test_packed_pagerank_equals_single_runs():
         assert np.array_equal(_bits(r[begin[k]:begin[k + 1]]), _bits(rk)), k
 
 
+def test_packed_pagerank_quarter_bins_large_items(oracle):
+    """A batch large enough that the binned plan's bins overflow one round of the device's
+    resident combine workgroups by a short last round (RMAT scale 19 plus a small graph: 1122
+    full-size bins for 1024 workgroups on a 256-CU device), so the vertices of that round's bins
+    are planned into quarter-size bins.  The small graph lies in that tail and has vertices whose
+    slots exceed a quarter-size bin (wave vertices of in-degree 2100-4000, lane groups of
+    in-degree 40-64), each of which gets a bin of its own; every graph's ranks stay bit-identical
+    to the oracle's FP32 restatement."""
+    big = wl.rmat_graph(scale=19)
+    rng = np.random.default_rng(13)
+    n = 3000
+    edges = [np.stack([rng.integers(0, n, deg), np.full(deg, v)], 1) for v, deg in ((0, 4000), (1, 2100), (2, 3000))]
+    for v in range(3, 200):
+        d = int(rng.integers(40, 65))
+        edges.append(np.stack([rng.integers(0, n, d), np.full(d, v)], 1))
+    edges.append(np.stack([rng.integers(0, n, 5000), rng.integers(200, n, 5000)], 1))
+    graphs = [big, csr_from_edges(n, np.concatenate(edges))[:2]]
+    with md.Graph.packed(graphs) as G:
+        begin = G.batch()
+        for steps in (1, 5):
+            for use_graph in (0, 1):
+                r, _ = G.pagerank(0.85, steps, variant=md.PR_BINNED, use_graph=use_graph)
+                for k, (grp, gcol) in enumerate(graphs):
+                    want = oracle.pagerank_f32(grp, gcol, 0.85, steps)
+                    assert np.array_equal(_bits(r[begin[k]:begin[k + 1]]), _bits(want)), (k, steps, use_graph)
+
+
 def test_set_batch_back_to_plain_graph(oracle):
     graphs = _mixed_batch()[:6]
     rp, col, _, begin = md.pack_graphs(graphs)
