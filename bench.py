@@ -45,7 +45,8 @@ def parse():
     p.add_argument("--profile", action="store_true",
                    help="short run for hardware counters: 3 PageRank runs (non-graph launches), or with "
                         "--workload bfs_rmat20 / bfs_er 4 BFS runs launched kernel by kernel, or with --workload "
-                        "sssp_rmat20 / sssp_lattice 3 host-driven SSSP runs of the plain engine (default width), "
+                        "sssp_rmat20 / sssp_lattice 3 host-driven SSSP runs of the plain engine (default width), or with "
+                        "--workload mbfs_rmat20 3 host-driven runs of the 64-source heuristic, "
                         "and one 256 MiB device copy as the byte-count calibration; prints no JSON")
     return p.parse_args()
 
@@ -323,6 +324,7 @@ def _roofline(stats, kernel, traffic=None):
 
 
 BFS_KERNEL = "k_bfs_super (superstep: pull, exact-claim push, or plain push + compaction)"
+MBFS_KERNEL = "k_mbfs_expand + k_mbfs_update (mask superstep: push or pull expansion, then the level update)"
 
 
 def _emit(a, metric_cfg, value, ms_per_step, roofline, cpu, extra):
@@ -498,7 +500,7 @@ def mbfs_block(md, wl, torch, np, rp, col, srcs, steps, warmup):
     d = {"gteps": round(steps * m_r / wl_ / 1e9, 3), "ms": round(wl_ / steps * 1e3, 4), "sources": len(srcs),
          "supersteps": rl_[-1].supersteps, "edges_reached_all_sources": m_r,
          "roofline": _roofline([G.mbfs(srcs, out=out, policy=pol, timing=1)[1] for _ in range(3)],
-                               "k_mbfs_push / k_mbfs_pull (mask superstep)"),
+                               MBFS_KERNEL, "mbfs_rmat20_traffic.md"),
          "gteps_heuristic": round(steps * m_r / wh / 1e9, 3),
          "learned_over_heuristic": round(_ab(lambda: G.mbfs(srcs, out=out)[1],
                                              lambda: G.mbfs(srcs, out=out, policy=pol)[1])[2], 3),
@@ -762,6 +764,14 @@ def run_workload(a, md, wl, torch, np):
                "graph": "RMAT scale 20, edge factor 16, seed 2; sources seed 4", "dtype": "u64 masks, u8 levels"}
         G = md.Graph(rp, col)
         out = torch.empty(64 * n, dtype=torch.uint8, device="cuda")
+        if a.profile:  # counter pass: 3 host-driven runs of the heuristic, then the calibration copy
+            for _ in range(3):
+                G.mbfs(srcs, out=out, timing=1)
+            torch.cuda.synchronize()
+            _calibration_copy(torch)
+            print("profile pass done: 3 host-driven MBFS runs, copy of 268435456 bytes", flush=True)
+            G.close()
+            return
         pol = _train_mbfs_policy(md, wl, np)
         wall, res = _timed(lambda: G.mbfs(srcs, out=out, policy=pol)[1], a.steps, a.warmup, torch)
         m_r = res[-1].edges_traversed
@@ -790,7 +800,7 @@ def run_workload(a, md, wl, torch, np):
                  "plain_64_bfs_gteps": round(plain_edges / tp / 1e9, 3),
                  "batched_over_plain": round((a.steps * m_r / wall) / (plain_edges / tp), 3)}
         roof = _roofline([G.mbfs(srcs, out=out, policy=pol, timing=1)[1] for _ in range(3)],
-                         "k_mbfs_push / k_mbfs_pull")
+                         MBFS_KERNEL, "mbfs_rmat20_traffic.md")
         cpu = None
         if not a.no_cpu:
             deg = np.diff(rp.astype(np.int64))
@@ -873,7 +883,7 @@ def run_workload(a, md, wl, torch, np):
                  "pagerank20_batched_over_plain": round((a.steps / wpr) / (1.0 / tpr), 2),
                  "pagerank_bits_identical_to_plain": bool(pr_same)}
         roof = _roofline([G.bfs_multi(sources, out=out, policy=pol, timing=1)[1] for _ in range(3)],
-                         "k_bfs_push / k_bfs_pull")
+                         BFS_KERNEL)
         cpu = None
         if not a.no_cpu:
             done = []
